@@ -1,0 +1,145 @@
+"""ctypes binding of libswe2d_hip.so (the C ABI declared in include/swe2d_hip.h).
+
+The library is built in-tree by ``diffswe2d_amd.build``.  There is no fallback:
+if the library is missing or fails to load, every call raises.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from pathlib import Path
+
+import torch
+
+HERE = Path(__file__).resolve().parent
+LIB_PATH = HERE / "libswe2d_hip.so"
+
+SWE_ABI_VERSION = 1
+BC_CODES = {"transmissive": 0, "periodic": 1, "constant": 2, "water_level": 3, "wall": 4, "absorbing": 5}
+FRICTION_CODES = {"manning": 0}
+
+ERR_DT_NONFINITE = -5
+ERR_DT_NONPOSITIVE = -6
+
+
+class SweGrid(ctypes.Structure):
+    _fields_ = [("batch", ctypes.c_int32), ("ny", ctypes.c_int32), ("nx", ctypes.c_int32),
+                ("bed_batch", ctypes.c_int32), ("rough_batch", ctypes.c_int32), ("pad", ctypes.c_int32),
+                ("dx", ctypes.c_double), ("dy", ctypes.c_double)]
+
+
+class SweConfigC(ctypes.Structure):
+    _fields_ = [("gravity", ctypes.c_double), ("dry_depth", ctypes.c_double), ("epsilon", ctypes.c_double),
+                ("cfl", ctypes.c_double), ("limiter_theta", ctypes.c_double),
+                ("constant_value", ctypes.c_double), ("boundary", ctypes.c_int32 * 4),
+                ("water_slope_reset", ctypes.c_int32), ("positivity_limiter", ctypes.c_int32),
+                ("apply_friction", ctypes.c_int32), ("friction_model", ctypes.c_int32)]
+
+
+class SweControl(ctypes.Structure):
+    _fields_ = [("elapsed", ctypes.c_double * 2), ("duration", ctypes.c_double), ("tol", ctypes.c_double),
+                ("cap", ctypes.c_double), ("bad_dt", ctypes.c_double), ("done", ctypes.c_int32),
+                ("steps", ctypes.c_int32), ("status", ctypes.c_int32), ("pad", ctypes.c_int32),
+                ("last_dt", ctypes.c_double)]
+
+
+_P = ctypes.c_void_p
+_I = ctypes.c_int
+_D = ctypes.c_double
+_G = ctypes.POINTER(SweGrid)
+_C = ctypes.POINTER(SweConfigC)
+
+_SIGNATURES = {
+    "swe_abi_version": ([], _I),
+    "swe_strerror": ([_I], ctypes.c_char_p),
+    "swe_last_error_message": ([], ctypes.c_char_p),
+    "swe_control_bytes": ([], ctypes.c_size_t),
+    "swe_workspace_bytes": ([_G, _I], ctypes.c_size_t),
+    "swe_workspace_init": ([_P, ctypes.c_size_t, _P], _I),
+    "swe_spatial_operator_f32": ([_G, _C, _P, _P, _P, _I, _P, _P, _P, _P, _I, _P], _I),
+    "swe_spatial_operator_f64": ([_G, _C, _P, _P, _P, _I, _P, _P, _P, _P, _I, _P], _I),
+    "swe_step_f32": ([_G, _C, _P, _P, _P, _P, _I, _D, _I, _P, _P, _I, _P], _I),
+    "swe_step_f64": ([_G, _C, _P, _P, _P, _P, _I, _D, _I, _P, _P, _I, _P], _I),
+    "swe_run_begin": ([_P, _D, _D, _I, _P], _I),
+    "swe_advance_f32": ([_G, _C, _P, _P, _P, _I, _P, _P, _I, _I, _P, _P, _I, _P, _I, _P, _P, _I, _P, _I, _P], _I),
+    "swe_advance_f64": ([_G, _C, _P, _P, _P, _I, _P, _P, _I, _I, _P, _P, _I, _P, _I, _P, _P, _I, _P, _I, _P], _I),
+}
+
+_lib = None
+
+
+def exported_symbols():
+    """Names every library build must export (the declarations of include/swe2d_hip.h)."""
+    return tuple(_SIGNATURES)
+
+
+def load(path: Path | None = None):
+    """Load (once) and type the library; raises if it is missing or incompatible."""
+    global _lib
+    if _lib is not None and path is None:
+        return _lib
+    p = Path(path) if path is not None else LIB_PATH
+    if not p.exists():
+        raise RuntimeError(
+            f"HIP library {p} is missing; build it with `python -c 'import __graft_entry__ as g; g.build()'`"
+        )
+    lib = ctypes.CDLL(str(p))
+    for name, (args, res) in _SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.argtypes = args
+        fn.restype = res
+    if lib.swe_abi_version() != SWE_ABI_VERSION:
+        raise RuntimeError(f"libswe2d_hip ABI {lib.swe_abi_version()} != expected {SWE_ABI_VERSION}")
+    if lib.swe_control_bytes() != ctypes.sizeof(SweControl):
+        raise RuntimeError("swe_control layout mismatch between header and binding")
+    if path is None:
+        _lib = lib
+    return lib
+
+
+def check(code: int, what: str):
+    if code != 0:
+        lib = load()
+        msg = lib.swe_last_error_message().decode(errors="replace")
+        raise RuntimeError(f"{what} failed ({code}: {lib.swe_strerror(code).decode()}): {msg}")
+
+
+def ptr(t):
+    return ctypes.c_void_p(0) if t is None else ctypes.c_void_p(t.data_ptr())
+
+
+def stream_ptr(device) -> ctypes.c_void_p:
+    return ctypes.c_void_p(torch.cuda.current_stream(device).cuda_stream)
+
+
+def make_config(cfg) -> SweConfigC:
+    """Translate an SWEConfig into the ABI struct (strings -> codes)."""
+    c = SweConfigC()
+    c.gravity, c.dry_depth, c.epsilon = float(cfg.gravity), float(cfg.dry_depth), float(cfg.epsilon)
+    c.cfl, c.limiter_theta = float(cfg.cfl), float(cfg.limiter_theta)
+    c.constant_value = float(cfg.constant_value)
+    for i, side in enumerate((cfg.boundary_left, cfg.boundary_right, cfg.boundary_top, cfg.boundary_bottom)):
+        if side not in BC_CODES:
+            raise ValueError(
+                f"Unsupported {('left', 'right', 'top', 'bottom')[i]} boundary type {side!r}. "
+                f"Valid types are {sorted(BC_CODES)}."
+            )
+        c.boundary[i] = BC_CODES[side]
+    c.water_slope_reset = int(bool(cfg.water_slope_reset))
+    c.positivity_limiter = int(bool(cfg.positivity_limiter))
+    c.apply_friction = int(bool(cfg.apply_friction))
+    c.friction_model = 0 if cfg.frictionmodel == "manning" else 1
+    return c
+
+
+def make_grid(batch, ny, nx, bed_batch, rough_batch, dx, dy) -> SweGrid:
+    g = SweGrid()
+    g.batch, g.ny, g.nx = int(batch), int(ny), int(nx)
+    g.bed_batch, g.rough_batch = int(bed_batch), int(rough_batch)
+    g.dx, g.dy = float(dx), float(dy)
+    return g
+
+
+def read_control(ws: torch.Tensor) -> SweControl:
+    raw = ws[: ctypes.sizeof(SweControl)].cpu().numpy().tobytes()
+    return SweControl.from_buffer_copy(raw)

@@ -1,0 +1,272 @@
+// Scalar math of the shallow-water time step, shared by every kernel.
+//
+// Each function restates one piece of the reference with the same operations
+// in the same order, so that FP32 results round like the reference's eager
+// PyTorch ops (build with -ffp-contract=off; IEEE division and square root).
+// Constants that the reference folds in Python double arithmetic arrive here
+// already folded and cast once (see SweConsts, filled on the host).
+//
+// Min/max/clamp helpers follow torch semantics, including NaN propagation,
+// so a NaN state still produces a NaN wave speed and a non-finite dt.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#define SWE_HD __host__ __device__ inline
+
+namespace swe {
+
+// ---------------------------------------------------------------- torch-like helpers
+template <class T> SWE_HD T tmax(T a, T b) { return (a > b || a != a) ? a : b; }
+template <class T> SWE_HD T tmin(T a, T b) { return (a < b || a != a) ? a : b; }
+template <class T> SWE_HD T clamp_lo(T x, T lo) { return x < lo ? lo : x; }
+template <class T> SWE_HD T clamp_hi(T x, T hi) { return x > hi ? hi : x; }
+template <class T> SWE_HD T clamp2(T x, T lo, T hi) { return x < lo ? lo : (x > hi ? hi : x); }
+template <class T> SWE_HD T tabs(T x) { return x < T(0) ? -x : x; }
+
+// Constants of one run, folded on the host in double then cast to T once.
+template <class T> struct SweConsts {
+  T g;            // float(gravity)                      riemann.py:36
+  T half_g;       // 0.5 * gravity                       riemann.py:69-70, reconstruction.py:101-106
+  T dry;          // dry_depth
+  T eps;          // epsilon
+  T floor;        // max(eps, dry, prec)                 riemann.py:6-10, reconstruction.py:12-15
+  T hll_floor;    // max(eps, prec)                      riemann.py:13-17
+  T fric_floor;   // max(eps, prec)                      friction.py:4-9 (dry_depth = 0 form)
+  T fric_floor2;  // max(eps, prec)^2 (double product)   friction.py:30-33
+  T theta;        // limiter theta                       numerics.py:16
+  T cfl;
+  T dx, dy;       // float(dx), float(dy)
+  T cval;         // constant-boundary value
+  T manning_g;    // 9.81 (hard-coded in friction.py:66)
+  T third;        // 1.0 / 3.0                           friction.py:66
+};
+
+// ---------------------------------------------------------------- limiter (numerics.py:4-27)
+// minmod of (theta*backward, centred, theta*forward): the smallest magnitude when
+// all three share a strict sign, else 0 -- equal to sign(a)*min(|a|,|b|,|c|).
+template <class T> SWE_HD T minmod3(T a, T b, T c) {
+  if (a > T(0) && b > T(0) && c > T(0)) return tmin(a, tmin(b, c));
+  if (a < T(0) && b < T(0) && c < T(0)) return tmax(a, tmax(b, c));
+  return T(0);
+}
+
+template <class T> SWE_HD T limited_increment(T qm, T q, T qp, T theta) {
+  T back = q - qm;
+  T centred = T(0.5) * (qp - qm);
+  T fwd = qp - q;
+  return minmod3(theta * back, centred, theta * fwd);
+}
+
+// ---------------------------------------------------------------- cells
+// Primitive state of a (possibly ghost) cell as the reconstruction sees it
+// (riemann.py:20-38 via reconstruction.py:27-28): depth clamped at 0, velocity 0
+// where dry.  un/ut are the velocity normal/tangential to the sweep direction.
+template <class T> struct Cell {
+  T h, un, ut, eta, z;
+};
+
+template <class T>
+SWE_HD void primitive_cell(T h_raw, T hu, T hv, T z, const SweConsts<T>& k, T& h, T& u, T& v, T& eta) {
+  h = clamp_lo(h_raw, T(0));
+  bool wet = h > k.dry;
+  T safe = clamp_lo(h, k.floor);
+  u = wet ? hu / safe : T(0);
+  v = wet ? hv / safe : T(0);
+  eta = h + z;
+}
+
+// Limited increments of one cell along the sweep direction, after the
+// water-slope reset (reconstruction.py:26-42), plus the velocity-factor
+// quotient 0.5*dh/max(h, floor) (reconstruction.py:62-73).
+template <class T> struct Incr {
+  T dh, deta, dz, dun, dut, q;
+};
+
+template <class T>
+SWE_HD Incr<T> increments(const Cell<T>& m, const Cell<T>& c, const Cell<T>& p, const SweConsts<T>& k,
+                          bool water_slope_reset) {
+  Incr<T> r;
+  r.dh = limited_increment(m.h, c.h, p.h, k.theta);
+  r.deta = limited_increment(m.eta, c.eta, p.eta, k.theta);
+  r.dz = limited_increment(m.z, c.z, p.z, k.theta);
+  r.dun = limited_increment(m.un, c.un, p.un, k.theta);
+  r.dut = limited_increment(m.ut, c.ut, p.ut, k.theta);
+  if (water_slope_reset) {
+    T base = c.eta - c.h;
+    T half = T(0.5) * (r.deta - r.dh);
+    T bed_minus = base - half;
+    T bed_plus = base + half;
+    bool reset = (r.deta * r.deta > r.dz * r.dz) && ((bed_minus > m.eta) || (bed_plus > p.eta));
+    if (reset) r.deta = r.dh + r.dz;
+  }
+  r.q = (T(0.5) * r.dh) / clamp_lo(c.h, k.floor);
+  return r;
+}
+
+// Reconstructed values of a cell on one of its two faces
+// (reconstruction.py:50-57, 71-96).
+template <class T> struct Side {
+  T hlin, eta, z, un, ut;
+};
+
+// cell as the LEFT state of its +1 face
+template <class T> SWE_HD Side<T> side_plus(const Cell<T>& c, const Incr<T>& d, const SweConsts<T>& k) {
+  T fac = c.h > k.dry ? clamp2(T(1) - d.q, T(-10), T(10)) : T(1);
+  Side<T> s;
+  s.hlin = c.h + T(0.5) * d.dh;
+  s.eta = c.eta + T(0.5) * d.deta;
+  s.z = c.z + T(0.5) * d.dz;
+  T hf = T(0.5) * fac;
+  s.un = c.un + hf * d.dun;
+  s.ut = c.ut + hf * d.dut;
+  return s;
+}
+
+// cell as the RIGHT state of its -1 face
+template <class T> SWE_HD Side<T> side_minus(const Cell<T>& c, const Incr<T>& d, const SweConsts<T>& k) {
+  T fac = c.h > k.dry ? clamp2(T(1) + d.q, T(-10), T(10)) : T(1);
+  Side<T> s;
+  s.hlin = c.h - T(0.5) * d.dh;
+  s.eta = c.eta - T(0.5) * d.deta;
+  s.z = c.z - T(0.5) * d.dz;
+  T hf = T(0.5) * fac;
+  s.un = c.un - hf * d.dun;
+  s.ut = c.ut - hf * d.dut;
+  return s;
+}
+
+// ---------------------------------------------------------------- face flux
+// Hydrostatic face states (reconstruction.py:59-69, 98-106) and the two-wave HLL
+// flux with dry-front speeds (riemann.py:41-106).  fn is the normal-momentum
+// flux; the cell on the left of the face sees fn - sL, the cell on the right
+// fn - sR (solver.py:503-509).
+template <class T> struct FaceFlux {
+  T fh, fn, ft, sL, sR, speed;
+};
+
+template <class T>
+SWE_HD FaceFlux<T> face_flux(const Side<T>& L, T hL_cell, T zL_cell, const Side<T>& R, T hR_cell, T zR_cell,
+                             const SweConsts<T>& k) {
+  T z_face = tmin(tmax(L.z, R.z), tmin(L.eta, R.eta));
+  T hcL = clamp_lo(tmin(L.eta - z_face, L.hlin), T(0));
+  T hcR = clamp_lo(tmin(R.eta - z_face, R.hlin), T(0));
+
+  FaceFlux<T> f;
+  f.sL = (k.half_g * (hL_cell + hcL)) * (zL_cell - z_face);
+  f.sR = (k.half_g * (hcR + hR_cell)) * (zR_cell - z_face);
+
+  // primitive() of the face states
+  T qnL = hcL * L.un, qtL = hcL * L.ut;
+  T qnR = hcR * R.un, qtR = hcR * R.ut;
+  T hL = clamp_lo(hcL, T(0)), hR = clamp_lo(hcR, T(0));
+  bool wetL = hL > k.dry, wetR = hR > k.dry;
+  T sfL = clamp_lo(hL, k.floor), sfR = clamp_lo(hR, k.floor);
+  T nL = wetL ? qnL / sfL : T(0);
+  T tL = wetL ? qtL / sfL : T(0);
+  T cL = wetL ? sqrt(k.g * sfL) : T(0);
+  T nR = wetR ? qnR / sfR : T(0);
+  T tR = wetR ? qtR / sfR : T(0);
+  T cR = wetR ? sqrt(k.g * sfR) : T(0);
+
+  T um = (T(0.5) * (nL + nR) + cL) - cR;
+  T cm = T(0.5) * (cL + cR) + T(0.25) * (nL - nR);
+  bool dryL = hL <= k.dry;
+  bool dryR = hR <= k.dry;
+  T SL = dryL ? nR - T(2) * cR : tmin(nL - cL, um - cm);
+  T SR = dryR ? nL + T(2) * cL : tmax(nR + cR, um + cm);
+
+  T mL = hL * nL, mR = hR * nR;
+  T pL = hL * (nL * nL + k.half_g * hL);
+  T pR = hR * (nR * nR + k.half_g * hR);
+  T den = clamp_lo(SR - SL, k.hll_floor);
+  T slsr = SL * SR;
+  T m_star = ((SR * mL - SL * mR) + slsr * (hR - hL)) / den;
+  T p_star = ((SR * pL - SL * pR) + slsr * (hR * nR - hL * nL)) / den;
+
+  T fh, fn;
+  if (SL >= T(0)) {
+    fh = mL;
+    fn = pL;
+  } else if (SR <= T(0)) {
+    fh = mR;
+    fn = pR;
+  } else {
+    fh = m_star;
+    fn = p_star;
+  }
+  T ft = (fh > T(0) ? tL : tR) * fh;
+  T speed = tmax(tabs(SL), tabs(SR));
+  if (dryL && dryR) {
+    fh = T(0);
+    fn = T(0);
+    ft = T(0);
+    speed = T(0);
+  }
+  f.fh = fh;
+  f.fn = fn;
+  f.ft = ft;
+  f.speed = speed;
+  return f;
+}
+
+// ---------------------------------------------------------------- positivity and friction
+// Positivity limiter and dry-cell momentum removal (solver.py:547-570).
+template <class T>
+SWE_HD void clean_and_limit(T oh, T ou, T ov, T& ch, T& cu, T& cv, const SweConsts<T>& k, bool positivity) {
+  if (positivity) {
+    T dh = ch - oh, du = cu - ou, dv = cv - ov;
+    T theta = T(1);
+    if (dh < T(0)) theta = clamp_hi(oh / clamp_lo(-dh, k.eps), T(1));
+    ch = oh + theta * dh;
+    cu = ou + theta * du;
+    cv = ov + theta * dv;
+  }
+  T h = clamp_lo(ch, T(0));
+  bool wet = h > k.dry;
+  ch = h;
+  cu = wet ? cu : T(0);
+  cv = wet ? cv : T(0);
+}
+
+// Semi-implicit Manning friction (friction.py:12-38, 62-66, 102-131).
+template <class T> SWE_HD T pow_third(T x, T third);
+template <> SWE_HD float pow_third<float>(float x, float third) { return powf(x, third); }
+template <> SWE_HD double pow_third<double>(double x, double third) { return pow(x, third); }
+
+template <class T> SWE_HD T log_t(T x);
+template <> SWE_HD float log_t<float>(float x) { return logf(x); }
+template <> SWE_HD double log_t<double>(double x) { return log(x); }
+
+// model 0: Manning n; model 1: roughness length z0 (BG_Flood smart friction, friction.py:41-59)
+template <class T>
+SWE_HD void friction(T& h_io, T& hu, T& hv, T coef, T dt, const SweConsts<T>& k, int model) {
+  T h = clamp_lo(h_io, T(0));
+  bool wet = h > k.dry;
+  T safe = clamp_lo(h, k.floor);
+  T u = (wet ? hu : T(0)) / safe;
+  T v = (wet ? hv : T(0)) / safe;
+  T s2 = u * u + v * v;
+  bool moving = wet && (s2 > T(0));
+  T speed = moving ? sqrt(clamp_lo(s2, k.fric_floor2)) : T(0);
+  T cf;
+  if (model == 0) {
+    T sh = clamp_lo(h, k.fric_floor);
+    T n = clamp_lo(coef, T(0));
+    cf = (k.manning_g * (n * n)) / pow_third(sh, k.third);
+  } else {
+    T fl = k.fric_floor;
+    T ratio = clamp_lo(h, fl) / clamp_lo(coef, fl);
+    T shallow = T(1) / clamp_lo(T(0.46) * ratio, fl);
+    T r = clamp_lo(ratio, fl);
+    T lt = T(2.5) * ((log_t(r) - T(1)) + (T(1) / r) * T(1.359));  // scalar / tensor = reciprocal * scalar
+    T deep = T(1) / clamp_lo(lt * lt, fl);
+    cf = ratio < T(2.718281828459045) ? shallow : deep;
+  }
+  T factor = T(1) / (T(1) + ((cf * speed) * dt) / safe);
+  h_io = h;
+  hu = wet ? hu * factor : T(0);
+  hv = wet ? hv * factor : T(0);
+}
+
+}  // namespace swe

@@ -1,0 +1,471 @@
+"""Drop-in ``SWE2D`` whose time step runs on the gfx950 HIP library.
+
+Same constructor, attributes and methods as the reference ``SWE2D``
+(src/solver.py:34-801); the numerics of ``spatial_operator``, ``step`` and
+``forward`` run in ``libswe2d_hip.so``.  There is no CPU path: tensors must be
+on a ROCm device, and a missing library raises at the first call.
+
+Differences a caller can see:
+  * ``forward`` integrates on the device without a host synchronisation per
+    step (the reference syncs 2-3 times per step); a callback or a dynamic
+    boundary series falls back to host-driven steps with identical numerics.
+  * gradients flow through ``step``/``forward`` via the hand-written adjoint
+    (``diffswe2d_amd.adjoint``) to U and the boundary level; bed / DEM
+    corrections are not differentiated.
+"""
+from __future__ import annotations
+
+import math
+from numbers import Integral, Real
+from typing import Optional, Union
+
+import torch
+import torch.nn as nn
+
+from . import _lib
+from .config import SWEConfig
+
+ScalarOrTensor = Union[float, int, torch.Tensor]
+
+_DTYPES = {torch.float32: "f32", torch.float64: "f64"}
+
+
+class SWE2D(nn.Module):
+    """B-Flood-style 2-D shallow-water solver; state channels are [h, hu, hv].
+
+    ``forward(..., t_end=...)`` interprets ``t_end`` as the duration of the
+    call, not an absolute end time (src/solver.py:37-46).
+    """
+
+    def __init__(self, bed_reference, roughness_length, dx, dy, config=None, train_dem=True,
+                 maximum_dem_correction=None, valid_mask=None, x=None, y=None, rainfall=None):
+        super().__init__()
+        self.cfg = config or SWEConfig()
+        self.dx, self.dy = float(dx), float(dy)
+        if self.dx <= 0.0 or self.dy <= 0.0:
+            raise ValueError(f"dx and dy must be positive, got {self.dx}, {self.dy}.")
+        if bed_reference.ndim == 2:
+            bed_reference = bed_reference[None, None]
+        if roughness_length.ndim == 2:
+            roughness_length = roughness_length[None, None]
+        if bed_reference.ndim != 4 or bed_reference.shape[1] != 1:
+            raise ValueError(
+                f"bed_reference must have shape [ny,nx] or [B,1,ny,nx], got {tuple(bed_reference.shape)}.")
+        if roughness_length.ndim != 4:
+            raise ValueError(f"roughness_length must be 2-D or 4-D, got {tuple(roughness_length.shape)}.")
+        if bed_reference.shape != roughness_length.shape:
+            raise ValueError(
+                f"DEM and roughness shapes differ: {bed_reference.shape} != {roughness_length.shape}.")
+        if not torch.isfinite(bed_reference).all():
+            raise ValueError("bed_reference contains non-finite values.")
+        if not torch.isfinite(roughness_length).all():
+            raise ValueError("roughness_length contains non-finite values.")
+        self.register_buffer("bed_reference", bed_reference.clone())
+        self.register_buffer("roughness_length", roughness_length.clone())
+        mask = (torch.ones_like(bed_reference, dtype=torch.bool) if valid_mask is None
+                else valid_mask.reshape_as(bed_reference).bool())
+        self.register_buffer("valid_mask", mask)
+        if x is not None:
+            self.register_buffer("x", x.clone())
+        if y is not None:
+            self.register_buffer("y", y.clone())
+        self.maximum_dem_correction = maximum_dem_correction
+        self.dem_correction_parameter = nn.Parameter(torch.zeros_like(bed_reference),
+                                                     requires_grad=bool(train_dem))
+        self.train_dem = bool(train_dem)
+        self.rainfall = rainfall
+        self.use_dynamic_bc = False
+        self.bc_times = None
+        self.bc_wls = None
+        self.track_gauges = False
+        self.gauge_indices = {}
+        self.bed_elevations = {}
+        self.ts_time, self.ts_dt = [], []
+        self.ts_h, self.ts_z = {}, {}
+        self._ws = {}
+        self._field_cache = {}
+        self.rows_hint = 0  # wavefront rows per strip; 0 = library heuristic / autotuned value
+
+    # ------------------------------------------------------------------ properties
+    @property
+    def bed(self):
+        """Current bed: bed_reference + correction on valid cells (src/solver.py:134-147)."""
+        correction = self.dem_correction_parameter
+        if self.maximum_dem_correction is not None:
+            correction = float(self.maximum_dem_correction) * torch.tanh(correction)
+        return self.bed_reference + correction * self.valid_mask.to(self.bed_reference.dtype)
+
+    @classmethod
+    def from_netcdf(cls, *args, **kwargs):
+        raise NotImplementedError(
+            "NetCDF DEM loading is not part of the HIP time-step build; construct SWE2D from tensors.")
+
+    @classmethod
+    def from_ascii(cls, *args, **kwargs):
+        raise NotImplementedError(
+            "ESRI-ASCII DEM loading is not part of the HIP time-step build; construct SWE2D from tensors.")
+
+    # ------------------------------------------------------------------ configuration helpers
+    def set_dynamic_boundary(self, times: torch.Tensor, water_levels: torch.Tensor, enabled: bool = True) -> None:
+        """Time-varying boundary level (src/solver.py:329-354)."""
+        if not torch.is_tensor(times) or not torch.is_tensor(water_levels):
+            raise TypeError("times and water_levels must be tensors.")
+        if times.ndim != 1 or water_levels.ndim != 1:
+            raise ValueError("times and water_levels must be one-dimensional.")
+        if times.numel() == 0 or times.numel() != water_levels.numel():
+            raise ValueError("Boundary arrays must be non-empty and equal length.")
+        if times.numel() > 1 and not torch.all(times[1:] > times[:-1]):
+            raise ValueError("Dynamic boundary times must be strictly increasing.")
+        if not torch.isfinite(times).all() or not torch.isfinite(water_levels).all():
+            raise ValueError("Dynamic boundary arrays contain non-finite values.")
+        self.bc_times, self.bc_wls = times, water_levels
+        self.use_dynamic_bc = bool(enabled)
+
+    def clear_dynamic_boundary(self) -> None:
+        self.use_dynamic_bc = False
+        self.bc_times = self.bc_wls = None
+
+    def reset_gauge_tracking(self) -> None:
+        self.ts_time, self.ts_dt = [], []
+        self.ts_h = {name: [] for name in self.gauge_indices}
+        self.ts_z = {name: [] for name in self.gauge_indices}
+
+    @staticmethod
+    def _expand(field: torch.Tensor, batch: int) -> torch.Tensor:
+        if field.shape[0] == batch:
+            return field
+        if field.shape[0] != 1:
+            raise ValueError(f"Field batch must be 1 or {batch}, got {field.shape[0]}.")
+        return field.expand(batch, -1, -1, -1)
+
+    @staticmethod
+    def _prepare_boundary_level(level: ScalarOrTensor, reference: torch.Tensor) -> torch.Tensor:
+        if torch.is_tensor(level):
+            return level.to(dtype=reference.dtype, device=reference.device)
+        return reference.new_tensor(level)
+
+    def _interpolate_dynamic_boundary(self, current_time: float, reference: torch.Tensor) -> torch.Tensor:
+        """Linear interpolation of the dynamic series at a step start (src/solver.py:391-415)."""
+        if self.bc_times is None or self.bc_wls is None:
+            raise RuntimeError("Dynamic boundary forcing is not configured.")
+        times = self.bc_times.to(dtype=reference.dtype, device=reference.device)
+        levels = self.bc_wls.to(dtype=reference.dtype, device=reference.device)
+        time = reference.new_tensor(current_time)
+        index = torch.clamp(torch.searchsorted(times, time, right=False), 1, times.numel() - 1)
+        t0, t1 = times[index - 1], times[index]
+        w0, w1 = levels[index - 1], levels[index]
+        weight = (time - t0) / torch.clamp(t1 - t0, min=torch.finfo(times.dtype).eps)
+        return w0 + weight * (w1 - w0)
+
+    def _resolve_boundary_level(self, current_time, U, explicit):
+        if explicit is not None:
+            return self._prepare_boundary_level(explicit, U)
+        if self.use_dynamic_bc:
+            return self._interpolate_dynamic_boundary(current_time, U)
+        return self._prepare_boundary_level(self.cfg.constant_level, U)
+
+    # ------------------------------------------------------------------ device plumbing
+    def _check_state(self, U, name="U"):
+        if U.ndim != 4 or U.shape[1] != 3:
+            raise ValueError(f"{name} must have shape [batch,3,ny,nx], got {tuple(U.shape)}.")
+        if U.dtype not in _DTYPES:
+            raise TypeError(f"{name} must be float32 or float64, got {U.dtype}.")
+        if U.device.type != "cuda":
+            raise RuntimeError(
+                f"{name} is on {U.device}; the HIP time step runs on a ROCm device only (no CPU path).")
+        if tuple(U.shape[-2:]) != tuple(self.bed_reference.shape[-2:]):
+            raise ValueError(
+                "U and bed must have identical spatial dimensions. "
+                f"U has {tuple(U.shape[-2:])}; bed has {tuple(self.bed_reference.shape[-2:])}.")
+
+    def _fields(self, U):
+        """Contiguous bed and roughness on U's device/dtype (cached between calls)."""
+        p = self.dem_correction_parameter
+        key = (U.device, U.dtype, self.bed_reference.data_ptr(), self.bed_reference._version, p.data_ptr(),
+               p._version, self.valid_mask._version, self.roughness_length.data_ptr(),
+               self.roughness_length._version, self.maximum_dem_correction)
+        hit = self._field_cache.get("fields")
+        if hit is not None and hit[0] == key:
+            return hit[1], hit[2]
+        with torch.no_grad():
+            bed = self.bed.detach().to(device=U.device, dtype=U.dtype).contiguous()
+            rough = self.roughness_length.detach().to(device=U.device, dtype=U.dtype).contiguous()
+        B = U.shape[0]
+        self._expand(bed, B)
+        self._expand(rough, B)
+        self._field_cache["fields"] = (key, bed, rough)
+        return bed, rough
+
+    def _workspace(self, U, bed, rough):
+        B, _, ny, nx = U.shape
+        key = (U.device, U.dtype, B, ny, nx, bed.shape[0], rough.shape[0])
+        ws = self._ws.get(key)
+        if ws is None:
+            lib = _lib.load()
+            grid = _lib.make_grid(B, ny, nx, bed.shape[0], rough.shape[0], self.dx, self.dy)
+            nbytes = lib.swe_workspace_bytes(grid, U.element_size())
+            if nbytes == 0:
+                raise ValueError(f"invalid grid {B}x{ny}x{nx}: {lib.swe_last_error_message().decode()}")
+            buf = torch.empty(nbytes, dtype=torch.uint8, device=U.device)
+            _lib.check(lib.swe_workspace_init(_lib.ptr(buf), nbytes, _lib.stream_ptr(U.device)), "workspace init")
+            ws = dict(buf=buf, grid=grid, parity=0)
+            self._ws[key] = ws
+        ws["cfg"] = _lib.make_config(self.cfg)
+        return ws
+
+    def _level_arg(self, level, U):
+        lvl = self._prepare_boundary_level(level, U).detach()
+        B = U.shape[0]
+        if lvl.numel() == 1:
+            return lvl.reshape(1).contiguous(), 0
+        if lvl.numel() == B and (lvl.ndim == 1 or lvl.shape == (B, 1, 1, 1)):
+            return lvl.reshape(B).contiguous(), 1
+        raise ValueError(
+            "Boundary values must be a scalar, a one-element tensor, or a tensor with one value per batch "
+            f"member. Received shape {tuple(lvl.shape)}.")
+
+    def _unsupported_forcing(self, infiltration):
+        if self.rainfall is not None or infiltration is not None:
+            raise NotImplementedError("rainfall / infiltration forcing is not built into the HIP step yet.")
+
+    # ------------------------------------------------------------------ spatial operator
+    def spatial_operator(self, U, rainfall_rate=None, infiltration=None, return_wave_speed=False,
+                         boundary_level=None):
+        """Tendency dU/dt (src/solver.py:448-531); wave speeds are maxima over the batch."""
+        self._check_state(U)
+        U = U.contiguous()
+        bed, rough = self._fields(U)
+        ws = self._workspace(U, bed, rough)
+        lvl, stride = self._level_arg(self.cfg.constant_level if boundary_level is None else boundary_level, U)
+        lib = _lib.load()
+        rhs = torch.empty_like(U)
+        B = U.shape[0]
+
+        def field(v):
+            if v is None:
+                return None
+            t = v.to(dtype=U.dtype, device=U.device) if torch.is_tensor(v) else U.new_tensor(v)
+            t = torch.broadcast_to(t, U[:, 0:1].shape)
+            if B > 1 and not bool((t == t[:1]).all()):
+                raise NotImplementedError("per-member rain / infiltration fields are not supported")
+            return t[:1].contiguous()
+
+        rain, infil = field(rainfall_rate), field(infiltration)
+        fn = lib.swe_spatial_operator_f32 if U.dtype == torch.float32 else lib.swe_spatial_operator_f64
+        _lib.check(fn(ws["grid"], ws["cfg"], _lib.ptr(U), _lib.ptr(bed), _lib.ptr(lvl), stride, _lib.ptr(rain),
+                      _lib.ptr(infil), _lib.ptr(rhs), _lib.ptr(ws["buf"]), int(self.rows_hint),
+                      _lib.stream_ptr(U.device)), "spatial_operator")
+        if return_wave_speed:
+            ax, ay = self._bank_speeds(ws, U)
+            return rhs, ax, ay
+        return rhs
+
+    def _bank_speeds(self, ws, U):
+        nslots = 64
+        word = 4 if U.dtype == torch.float32 else 8
+        off = 256 + 2 * 2 * nslots * word
+        raw = ws["buf"][off:off + 2 * nslots * word]
+        vals = raw.view(U.dtype).reshape(2, nslots)
+        bits = raw.view(torch.int32 if word == 4 else torch.int64).reshape(2, nslots)
+        if word == 4:
+            bits = bits.to(torch.int64) & 0xFFFFFFFF  # unsigned order, as the device atomics compare
+        ix = bits.argmax(dim=1)  # largest bit pattern == largest non-negative value (NaN above all)
+        return vals[0, ix[0]].clone(), vals[1, ix[1]].clone()
+
+    def stable_dt(self, U, boundary_level=None):
+        """CFL-stable timestep (src/solver.py:537-545)."""
+        _, ax, ay = self.spatial_operator(U, return_wave_speed=True, boundary_level=boundary_level)
+        denominator = torch.clamp(ax / self.dx + ay / self.dy, min=self.cfg.epsilon)
+        return self.cfg.cfl / denominator
+
+    # ------------------------------------------------------------------ time stepping
+    def step(self, U, model_time, max_dt, infiltration=None, boundary_level=None):
+        """One adaptive midpoint step (src/solver.py:572-663); returns (U_new, dt as float).
+
+        dt is detached exactly like the reference; with autograd enabled and U
+        or the level requiring grad, the hand-written adjoint carries gradients.
+        """
+        self._check_state(U)
+        self._unsupported_forcing(infiltration)
+        level = self._prepare_boundary_level(
+            self.cfg.constant_level if boundary_level is None else boundary_level, U)
+        max_dt_value = self._scalar_float(max_dt, "max_dt") if torch.is_tensor(max_dt) else float(max_dt)
+        needs_grad = torch.is_grad_enabled() and (U.requires_grad or level.requires_grad)
+        if needs_grad:
+            from .adjoint import differentiable_step
+            return differentiable_step(self, U, level, max_dt_value)
+        out, dt = self._step_raw(U.detach().contiguous(), level.detach(), max_dt_value)
+        return out, dt
+
+    def _step_raw(self, U, level, max_dt_value):
+        bed, rough = self._fields(U)
+        ws = self._workspace(U, bed, rough)
+        lvl, stride = self._level_arg(level, U)
+        lib = _lib.load()
+        out = torch.empty_like(U)
+        fn = lib.swe_step_f32 if U.dtype == torch.float32 else lib.swe_step_f64
+        _lib.check(fn(ws["grid"], ws["cfg"], _lib.ptr(U), _lib.ptr(bed), _lib.ptr(rough), _lib.ptr(lvl), stride,
+                      float(max_dt_value), ws["parity"], _lib.ptr(out), _lib.ptr(ws["buf"]), int(self.rows_hint),
+                      _lib.stream_ptr(U.device)), "step")
+        ws["parity"] ^= 1
+        dt = _lib.read_control(ws["buf"]).last_dt
+        if not math.isfinite(dt):
+            raise FloatingPointError("The calculated timestep is non-finite.")
+        if dt <= 0.0:
+            raise FloatingPointError(f"Non-positive timestep: {dt}.")
+        return out, dt
+
+    # ------------------------------------------------------------------ forward integration
+    @staticmethod
+    def _scalar_float(value, name: str) -> float:
+        if torch.is_tensor(value):
+            if value.numel() != 1:
+                raise ValueError(f"{name} must be scalar.")
+            return float(value.detach().cpu())
+        if isinstance(value, Real):
+            return float(value)
+        raise TypeError(f"{name} must be a real scalar or scalar tensor.")
+
+    def forward(self, U0, t_end, start_time=0.0, max_steps=1_000_000, infiltration=None, callback=None,
+                boundary_level=None, max_dt_cap=None):
+        """Advance for ``t_end`` seconds (src/solver.py:680-779); returns (U, hmax)."""
+        if U0.ndim != 4 or U0.shape[1] != 3:
+            raise ValueError(f"U0 must have shape [batch,3,ny,nx], got {tuple(U0.shape)}.")
+        duration = self._scalar_float(t_end, "t_end")
+        absolute_start = self._scalar_float(start_time, "start_time")
+        if not isinstance(max_steps, Integral) or isinstance(max_steps, bool) or max_steps <= 0:
+            raise ValueError(f"max_steps must be positive, got {max_steps!r}.")
+        if duration < 0.0:
+            raise ValueError(f"t_end duration cannot be negative: {duration}.")
+        if duration == 0.0:
+            return U0, U0[:, 0].clone()
+        self._check_state(U0, "U0")
+        self._unsupported_forcing(infiltration)
+        cap = None if max_dt_cap is None else self._scalar_float(max_dt_cap, "max_dt_cap")
+        level_req = boundary_level is not None and torch.is_tensor(boundary_level) and boundary_level.requires_grad
+        needs_grad = torch.is_grad_enabled() and (U0.requires_grad or level_req)
+        if needs_grad:
+            from .adjoint import differentiable_forward
+            return differentiable_forward(self, U0, duration, absolute_start, max_steps, boundary_level, cap)
+        if callback is not None or (boundary_level is None and self.use_dynamic_bc):
+            return self._forward_host(U0, duration, absolute_start, max_steps, callback, boundary_level, cap)
+        return self._forward_device(U0.detach().contiguous(), duration, absolute_start, max_steps,
+                                    boundary_level, cap)
+
+    def _forward_host(self, U0, duration, absolute_start, max_steps, callback, boundary_level, cap):
+        U, hmax, elapsed = U0, U0[:, 0].clone(), 0.0
+        tolerance = max(1e-12, 1e-12 * abs(duration))
+        for iteration in range(max_steps):
+            remaining = duration - elapsed
+            if remaining <= tolerance:
+                return U, hmax
+            step_max_dt = remaining if cap is None else min(remaining, cap)
+            current_time = absolute_start + elapsed
+            level = self._resolve_boundary_level(current_time, U, boundary_level)
+            U, dt_value = self.step(U, model_time=current_time, max_dt=float(U.new_tensor(step_max_dt)),
+                                    boundary_level=level)
+            hmax = torch.maximum(hmax, U[:, 0])
+            elapsed += dt_value
+            if elapsed > duration and elapsed - duration <= tolerance:
+                elapsed = duration
+            if self.track_gauges:
+                self._record_gauges(U, absolute_start + elapsed, dt_value)
+            if callback is not None:
+                callback(iteration, absolute_start + elapsed, U)
+        raise RuntimeError(
+            f"max_steps={max_steps} reached after {elapsed:.8f} s; requested duration={duration:.8f} s.")
+
+    def _forward_device(self, U0, duration, absolute_start, max_steps, boundary_level, cap):
+        """Device-driven loop: dt, elapsed time and stopping live on the GPU."""
+        lib = _lib.load()
+        bed, rough = self._fields(U0)
+        ws = self._workspace(U0, bed, rough)
+        level = self._resolve_boundary_level(absolute_start, U0, boundary_level)
+        lvl, stride = self._level_arg(level, U0)
+        stream = _lib.stream_ptr(U0.device)
+        Ua = U0.clone()
+        Ub = torch.empty_like(U0)
+        hmax = U0[:, 0].clone()
+        names = list(self.gauge_indices) if self.track_gauges else []
+        gauges = (torch.tensor([list(self.gauge_indices[n]) for n in names], dtype=torch.int32,
+                               device=U0.device).contiguous() if names else None)
+        capacity = 256
+        dts = torch.empty(capacity, dtype=U0.dtype, device=U0.device)
+        gd = torch.empty(capacity, max(len(names), 1), dtype=U0.dtype, device=U0.device)
+        gl = torch.empty_like(gd)
+        _lib.check(lib.swe_run_begin(_lib.ptr(ws["buf"]), float(duration), -1.0 if cap is None else float(cap),
+                                     U0.element_size(), stream), "run_begin")
+        fn = lib.swe_advance_f32 if U0.dtype == torch.float32 else lib.swe_advance_f64
+        launched, chunk = 0, 8
+        ctl = None
+        while launched < max_steps:
+            n = min(chunk, max_steps - launched)
+            if launched + n > capacity:
+                capacity = max(2 * capacity, launched + n)
+                dts = torch.cat((dts, torch.empty(capacity - dts.numel(), dtype=dts.dtype, device=dts.device)))
+                grow = torch.empty(capacity - gd.shape[0], gd.shape[1], dtype=gd.dtype, device=gd.device)
+                gd, gl = torch.cat((gd, grow)), torch.cat((gl, grow.clone()))
+            _lib.check(fn(ws["grid"], ws["cfg"], _lib.ptr(bed), _lib.ptr(rough), _lib.ptr(lvl), stride,
+                          _lib.ptr(Ua), _lib.ptr(Ub), launched, n, _lib.ptr(hmax), _lib.ptr(dts), capacity,
+                          _lib.ptr(gauges), len(names), _lib.ptr(gd) if names else None,
+                          _lib.ptr(gl) if names else None, capacity, _lib.ptr(ws["buf"]), int(self.rows_hint),
+                          stream), "advance")
+            launched += n
+            ctl = _lib.read_control(ws["buf"])
+            if ctl.done:
+                break
+            # aim the next chunk at the remaining duration, from the latest dt
+            remaining = duration - ctl.elapsed[ctl.steps & 1]
+            est = int(remaining / ctl.last_dt) + 2 if ctl.last_dt > 0 else chunk
+            chunk = max(4, min(1024, est))
+        steps = ctl.steps
+        if ctl.status != 0:
+            if ctl.status == 1:
+                raise FloatingPointError("The calculated timestep is non-finite.")
+            raise FloatingPointError(f"Non-positive timestep: {ctl.bad_dt}.")
+        U = Ua if steps % 2 == 0 else Ub
+        if self.track_gauges and steps > 0:
+            dt_host = dts[:steps].cpu().tolist()
+            depth = gd[:steps].cpu().tolist()
+            lev = gl[:steps].cpu().tolist()
+            elapsed = 0.0
+            tol = max(1e-12, 1e-12 * abs(duration))
+            for k in range(steps):
+                elapsed += dt_host[k]
+                if elapsed > duration and elapsed - duration <= tol:
+                    elapsed = duration
+                self.ts_time.append(absolute_start + elapsed)
+                self.ts_dt.append(dt_host[k])
+                for j, name in enumerate(names):
+                    self.ts_h.setdefault(name, []).append(depth[k][j])
+                    self.ts_z.setdefault(name, []).append(lev[k][j])
+        self.last_dt_series = dts[:steps]
+        if not ctl.done:
+            elapsed = ctl.elapsed[steps & 1]
+            raise RuntimeError(
+                f"max_steps={max_steps} reached after {elapsed:.8f} s; requested duration={duration:.8f} s.")
+        return U, hmax
+
+    def _record_gauges(self, U, absolute_time: float, dt_value: float) -> None:
+        """Per-step gauge records of member 0 (src/solver.py:781-801)."""
+        self.ts_time.append(absolute_time)
+        self.ts_dt.append(dt_value)
+        bed = self.bed
+        for name, (iy, ix) in self.gauge_indices.items():
+            depth = float(U[0, 0, iy, ix].detach().cpu())
+            z = float(bed[0, 0, iy, ix].detach().cpu())
+            self.ts_h.setdefault(name, []).append(depth)
+            self.ts_z.setdefault(name, []).append(depth + z)
+
+    # ------------------------------------------------------------------ DEM regularisation
+    def dem_regularization(self, prior_weight=1.0, slope_weight=0.0, curvature_weight=0.0):
+        """Prior / slope / curvature penalty of the DEM correction (src/solver.py:807-836)."""
+        bed = self.bed
+        correction = bed - self.bed_reference
+        gx, gy = bed[..., 1:] - bed[..., :-1], bed[..., 1:, :] - bed[..., :-1, :]
+        prior = correction.square().mean()
+        slope = gx.square().mean() + gy.square().mean()
+        cx = (gx[..., 1:] - gx[..., :-1]).square().mean() if gx.shape[-1] >= 2 else bed.new_zeros(())
+        cy = (gy[..., 1:, :] - gy[..., :-1, :]).square().mean() if gy.shape[-2] >= 2 else bed.new_zeros(())
+        return (float(prior_weight) * prior + float(slope_weight) * slope
+                + float(curvature_weight) * (cx + cy))

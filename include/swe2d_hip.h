@@ -1,0 +1,137 @@
+/*
+ * swe2d_hip.h -- C ABI of the MI355X (gfx950) shallow-water time step.
+ *
+ * The reference (DiffSWE2d) exposes this path only as a Python object API:
+ * SWEConfig (src/config.py:4-28) and SWE2D.spatial_operator / step / forward
+ * (src/solver.py:448-779).  It has no FFI.  The entry points below are what a
+ * binding of that API needs; each cites the reference function it replaces.
+ *
+ * Conventions
+ *  - All arrays are device pointers, row-major, float32 (_f32) or float64 (_f64).
+ *  - State U is [batch, 3, ny, nx] (planes h, hu, hv per member), exactly the
+ *    reference layout, so torch tensors pass without copies.
+ *  - bed / roughness are [bed_batch, 1, ny, nx] with bed_batch 1 or batch.
+ *  - level is the water-level boundary value: one value (level_stride 0) or one
+ *    per member (level_stride 1) -- boundary.py:113-147.
+ *  - Calls are stream-ordered on `stream` (a hipStream_t) and allocate nothing;
+ *    scratch lives in a caller-provided workspace of swe_workspace_bytes().
+ *  - Return 0 on success, a negative SWE_ERR_* code otherwise; no exceptions
+ *    cross the ABI.  swe_last_error_message() is per calling thread.
+ */
+#ifndef SWE2D_HIP_H
+#define SWE2D_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SWE_ABI_VERSION 1
+
+enum {
+  SWE_OK = 0,
+  SWE_ERR_ARG = -1,
+  SWE_ERR_SHAPE = -2,
+  SWE_ERR_BC = -3,
+  SWE_ERR_HIP = -4,
+  SWE_ERR_DT_NONFINITE = -5,
+  SWE_ERR_DT_NONPOSITIVE = -6,
+  SWE_ERR_CAPACITY = -7
+};
+
+/* Boundary type codes; names as in boundary.py:11-18. */
+enum {
+  SWE_BC_TRANSMISSIVE = 0,
+  SWE_BC_PERIODIC = 1,
+  SWE_BC_CONSTANT = 2,
+  SWE_BC_WATER_LEVEL = 3,
+  SWE_BC_WALL = 4,
+  SWE_BC_ABSORBING = 5
+};
+
+/* Friction models: SWEConfig.frictionmodel (solver.py:648-661). */
+enum { SWE_FRICTION_MANNING = 0, SWE_FRICTION_ROUGHNESS_LENGTH = 1 };
+
+typedef struct swe_grid {
+  int32_t batch, ny, nx;
+  int32_t bed_batch;   /* 1 or batch */
+  int32_t rough_batch; /* 1 or batch */
+  int32_t pad;
+  double dx, dy;       /* SWE2D(dx, dy): solver.py:64-67 */
+} swe_grid;
+
+/* SWEConfig field for field (config.py:4-28); strings become codes. */
+typedef struct swe_config {
+  double gravity, dry_depth, epsilon, cfl, limiter_theta;
+  double constant_value;          /* "constant" boundary value */
+  int32_t boundary[4];            /* left, right, top, bottom */
+  int32_t water_slope_reset;
+  int32_t positivity_limiter;
+  int32_t apply_friction;
+  int32_t friction_model;
+} swe_config;
+
+/* Device control block at the head of the workspace (read back by the host). */
+typedef struct swe_control {
+  double elapsed[2];
+  double duration, tol, cap, bad_dt;
+  int32_t done, steps, status, pad;
+  double last_dt;
+} swe_control;
+
+int swe_abi_version(void);
+const char* swe_strerror(int code);
+const char* swe_last_error_message(void);
+size_t swe_control_bytes(void);
+
+/* Workspace size for a grid (dtype_bytes 4 or 8), and its one-time clearing. */
+size_t swe_workspace_bytes(const swe_grid* grid, int dtype_bytes);
+int swe_workspace_init(void* ws, size_t bytes, void* stream);
+
+/* Tendency dU/dt and wave speeds -- replaces SWE2D.spatial_operator
+ * (solver.py:448-531) with ghost cells, both reconstructions and HLL fused.
+ * rain / infiltration are optional [ny, nx] rate fields (NULL for none). */
+int swe_spatial_operator_f32(const swe_grid* grid, const swe_config* cfg, const float* U, const float* bed,
+                             const float* level, int level_stride, const float* rain, const float* infiltration,
+                             float* rhs, void* ws, int rows_hint, void* stream);
+int swe_spatial_operator_f64(const swe_grid* grid, const swe_config* cfg, const double* U, const double* bed,
+                             const double* level, int level_stride, const double* rain,
+                             const double* infiltration, double* rhs, void* ws, int rows_hint, void* stream);
+
+/* One adaptive midpoint step -- replaces SWE2D.step (solver.py:572-663).
+ * The step's dt lands in swe_control.last_dt; U_out is left untouched when dt
+ * is non-finite or non-positive (the host raises FloatingPointError, :627-632).
+ * parity alternates 0/1 between consecutive steps on one workspace. */
+int swe_step_f32(const swe_grid* grid, const swe_config* cfg, const float* U, const float* bed,
+                 const float* roughness, const float* level, int level_stride, double max_dt, int parity,
+                 float* U_out, void* ws, int rows_hint, void* stream);
+int swe_step_f64(const swe_grid* grid, const swe_config* cfg, const double* U, const double* bed,
+                 const double* roughness, const double* level, int level_stride, double max_dt, int parity,
+                 double* U_out, void* ws, int rows_hint, void* stream);
+
+/* Device-driven integration of a duration -- replaces the SWE2D.forward loop
+ * (solver.py:719-779) without per-step host synchronisation.  swe_run_begin
+ * arms the device clock; each swe_advance call launches n_steps further steps
+ * (step k reads U_a when k is even, U_b when odd, and writes the other).  Steps
+ * after the duration is reached (or after a bad dt) exit at once.  Optional
+ * outputs: running max depth hmax [batch, ny, nx] (:765), per-step dt
+ * (:766), member-0 depth and h+z at gauge cells (iy, ix pairs) (:781-801). */
+int swe_run_begin(void* ws, double duration, double max_dt_cap, int dtype_bytes, void* stream);
+int swe_advance_f32(const swe_grid* grid, const swe_config* cfg, const float* bed, const float* roughness,
+                    const float* level, int level_stride, float* U_a, float* U_b, int first_step, int n_steps,
+                    float* hmax, float* dt_series, int dt_capacity, const int32_t* gauges, int n_gauges,
+                    float* gauge_depth, float* gauge_level, int gauge_capacity, void* ws, int rows_hint,
+                    void* stream);
+int swe_advance_f64(const swe_grid* grid, const swe_config* cfg, const double* bed, const double* roughness,
+                    const double* level, int level_stride, double* U_a, double* U_b, int first_step,
+                    int n_steps, double* hmax, double* dt_series, int dt_capacity, const int32_t* gauges,
+                    int n_gauges, double* gauge_depth, double* gauge_level, int gauge_capacity, void* ws,
+                    int rows_hint, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SWE2D_HIP_H */

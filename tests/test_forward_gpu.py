@@ -1,0 +1,194 @@
+"""HIP forward step against the CPU oracle and the reference's golden vectors.
+
+Bars (FP32): tendency and states bitwise or within a few ulps of the oracle on
+small grids; after the full 500-step dam break rel-L2 <= 1e-5, identical
+wet/dry masks and dt series within 1e-6 relative.
+"""
+import math
+
+import numpy as np
+import pytest
+import torch
+
+import diffswe2d_amd as swe
+from diffswe2d_amd import scenarios
+from oracle import swe_oracle as O
+
+pytestmark = pytest.mark.gpu
+
+DT = {"f32": torch.float32, "f64": torch.float64}
+
+STEP_CASES = [
+    dict(bc=("wall", "wall", "wall", "wall"), friction=True, level=0.0),
+    dict(bc=("water_level", "wall", "transmissive", "water_level"), friction=True, level=0.45),
+    dict(bc=("periodic", "periodic", "absorbing", "constant"), friction=False, level=0.3),
+    dict(bc=("transmissive", "absorbing", "periodic", "periodic"), friction=True, level=0.2),
+]
+
+
+def relerr(a, b):
+    a = torch.as_tensor(a, dtype=torch.float64).cpu()
+    b = torch.as_tensor(b, dtype=torch.float64).cpu()
+    return float(torch.linalg.norm(a - b) / max(float(torch.linalg.norm(b)), 1e-300))
+
+
+def make_model(d, case, dtype, dev):
+    c = STEP_CASES[case]
+    cfg = swe.SWEConfig(boundary_left=c["bc"][0], boundary_right=c["bc"][1], boundary_top=c["bc"][2],
+                        boundary_bottom=c["bc"][3], apply_friction=c["friction"], constant_value=0.05)
+    m = swe.SWE2D(torch.tensor(d["bed"], dtype=dtype), torch.tensor(d["rough"], dtype=dtype), 0.1, 0.1,
+                  config=cfg).to(dev)
+    return m, cfg
+
+
+@pytest.mark.parametrize("case", range(len(STEP_CASES)))
+@pytest.mark.parametrize("name", ["f32", "f64"])
+def test_spatial_operator_matches_golden(golden, cuda_device, case, name):
+    d = golden("steps")
+    dtype = DT[name]
+    m, _ = make_model(d, case, dtype, cuda_device)
+    U = torch.tensor(d["U0"], dtype=dtype, device=cuda_device)
+    lvl = torch.tensor(STEP_CASES[case]["level"], dtype=dtype)
+    rhs, ax, ay = m.spatial_operator(U, return_wave_speed=True, boundary_level=lvl)
+    ref = torch.tensor(d[f"s{case}_{name}_rhs"])
+    tol = 2e-6 if name == "f32" else 1e-13
+    diff = (rhs.cpu().double() - ref.double()).abs().max().item()
+    scale = ref.abs().max().item()
+    print(f"case {case} {name}: max |rhs diff| {diff:.3e} (scale {scale:.3e})")
+    assert diff <= tol * scale
+    axay = d[f"s{case}_{name}_axay"]
+    assert abs(float(ax) - axay[0]) <= tol * axay[0]
+    assert abs(float(ay) - axay[1]) <= tol * axay[1]
+
+
+@pytest.mark.parametrize("case", range(len(STEP_CASES)))
+@pytest.mark.parametrize("name", ["f32", "f64"])
+def test_steps_match_golden(golden, cuda_device, case, name):
+    d = golden("steps")
+    dtype = DT[name]
+    m, _ = make_model(d, case, dtype, cuda_device)
+    U = torch.tensor(d["U0"], dtype=dtype, device=cuda_device)
+    lvl = torch.tensor(STEP_CASES[case]["level"], dtype=dtype)
+    tol = 1e-6 if name == "f32" else 1e-12
+    for k in range(12):
+        U, dt = m.step(U, 0.0, 1e30, boundary_level=lvl)
+        ref = d[f"s{case}_{name}_U"][k]
+        e = relerr(U, ref)
+        assert e <= tol, f"step {k}: rel-L2 {e:.3e}"
+        assert abs(dt - d[f"s{case}_{name}_dt"][k]) <= tol * d[f"s{case}_{name}_dt"][k]
+        wet = (U[:, 0] > 1e-6).cpu().numpy()
+        assert np.array_equal(wet, ref[:, 0] > 1e-6), f"wet/dry mask differs at step {k}"
+
+
+def test_dam_break_full_run(golden, cuda_device):
+    """Config: 256^2 circular dam break, walls, 500 steps, FP32 (rel-L2 <= 1e-5 after the run)."""
+    d = golden("dam_break")
+    sc = scenarios.dam_break()
+    cfg = swe.SWEConfig(boundary_left="wall", boundary_right="wall", boundary_top="wall", boundary_bottom="wall")
+    m = swe.SWE2D(torch.tensor(sc["bed"], dtype=torch.float32), torch.tensor(sc["rough"], dtype=torch.float32),
+                  sc["dx"], sc["dy"], config=cfg).to(cuda_device)
+    U = torch.tensor(sc["U"], dtype=torch.float32, device=cuda_device)
+    dts = []
+    for k in range(500):
+        U, dt = m.step(U, 0.0, 1e30)
+        dts.append(dt)
+        if k + 1 in (1, 10, 100):
+            e = relerr(U, d[f"U{k + 1}_f32"])
+            print(f"step {k + 1}: rel-L2 vs reference FP32 {e:.3e}")
+    dts = np.array(dts)
+    e32 = relerr(U, d["U500_f32"])
+    e64 = relerr(U, d["U500_f64"])
+    e_ref = relerr(d["U500_f32"], d["U500_f64"])
+    dt_err = float(np.max(np.abs(dts - d["dt_f32"]) / d["dt_f32"]))
+    print(f"dam break 500 steps: rel-L2 vs ref FP32 {e32:.3e}; vs ref FP64 {e64:.3e} "
+          f"(ref FP32 vs FP64 {e_ref:.3e}); dt max rel {dt_err:.3e}")
+    assert e32 <= 1e-5
+    assert dt_err <= 1e-6
+    wet = (U[:, 0] > 1e-6).cpu().numpy()
+    assert np.array_equal(wet, d["U500_f32"][:, 0] > 1e-6)
+    # symmetry of the circular problem: h(i, j) = h(j, i), hu(i, j) = hv(j, i)
+    h = U[0, 0].cpu()
+    assert torch.equal(h, h.T)
+    assert torch.equal(U[0, 1].cpu(), U[0, 2].cpu().T)
+    # walls and no dry cells: mass is conserved to rounding
+    m0 = float(torch.tensor(sc["U"][0, 0]).sum())
+    m1 = float(U[0, 0].double().sum())
+    assert abs(m1 - m0) / m0 < 1e-5
+
+
+def test_forward_device_loop_matches_host_steps(cuda_device):
+    sc = scenarios.coastal_runup(nx=68, ny=43, dx=0.08)
+    cfg = swe.SWEConfig(boundary_left="water_level", boundary_right="wall", boundary_top="wall",
+                        boundary_bottom="wall")
+    m = swe.SWE2D(torch.tensor(sc["bed"], dtype=torch.float32), torch.tensor(sc["rough"], dtype=torch.float32),
+                  0.08, 0.08, config=cfg).to(cuda_device)
+    U0 = torch.tensor(sc["U"], dtype=torch.float32, device=cuda_device)
+    lvl = torch.tensor(0.01)
+    m.track_gauges = True
+    m.gauge_indices = {f"g{i}": g for i, g in enumerate(sc["gauges"])}
+    m.reset_gauge_tracking()
+    Ud, hd = m.forward(U0, t_end=0.5, start_time=1.0, boundary_level=lvl)
+    dts_dev = list(m.ts_dt)
+    g_dev = list(m.ts_h["g0"])
+    m.reset_gauge_tracking()
+    Uh, hh = m.forward(U0, t_end=0.5, start_time=1.0, boundary_level=lvl, callback=lambda *a: None)
+    assert torch.equal(Ud, Uh)
+    assert torch.equal(hd, hh)
+    assert dts_dev == m.ts_dt
+    assert g_dev == m.ts_h["g0"]
+    assert abs(m.ts_time[-1] - 1.5) < 1e-12
+
+
+def test_forward_matches_oracle_runup(golden, cuda_device):
+    """Reduced coastal run-up (136 x 86, dx 0.04), 40 chunks of 0.05 s: moving wet/dry front."""
+    d = golden("runup")
+    cfg = swe.SWEConfig(boundary_left="water_level", boundary_right="wall", boundary_top="wall",
+                        boundary_bottom="wall")
+    m = swe.SWE2D(torch.tensor(d["bed"], dtype=torch.float32), torch.tensor(d["rough"], dtype=torch.float32),
+                  0.04, 0.04, config=cfg).to(cuda_device)
+    U = torch.tensor(d["U0"], dtype=torch.float32, device=cuda_device)
+    m.track_gauges = True
+    m.gauge_indices = {f"g{i}": tuple(int(v) for v in g) for i, g in enumerate(d["gauges"])}
+    m.reset_gauge_tracking()
+    masks = []
+    gauge = []
+    for k in range(len(d["levels"])):
+        U, _ = m.forward(U, t_end=0.05, start_time=0.05 * k, boundary_level=float(d["levels"][k]))
+        gauge.append([float(U[0, 0, iy, ix]) for iy, ix in d["gauges"]])
+        if (k + 1) % 10 == 0:
+            masks.append((U[0, 0] > 1e-6).cpu().numpy())
+    dts = np.array(m.ts_dt)
+    ref_dt = d["dt_f32"]
+    assert len(dts) == len(ref_dt)
+    dt_err = float(np.max(np.abs(dts - ref_dt) / ref_dt))
+    e32 = relerr(U, d["U_f32"])
+    e64 = relerr(U, d["U_f64"])
+    ref_masks = np.unpackbits(d["mask_f32"])[: np.stack(masks).size].reshape(np.stack(masks).shape).astype(bool)
+    mismatch = int((np.stack(masks) != ref_masks).sum())
+    print(f"run-up: {len(dts)} steps, rel-L2 vs ref FP32 {e32:.3e}, vs FP64 {e64:.3e}, dt max rel "
+          f"{dt_err:.3e}, mask mismatches {mismatch}")
+    assert e32 <= 1e-5
+    assert dt_err <= 1e-6
+    assert mismatch == 0
+
+
+def test_error_paths(cuda_device):
+    cfg = swe.SWEConfig(boundary_left="wall", boundary_right="wall", boundary_top="wall", boundary_bottom="wall")
+    z = torch.zeros(1, 1, 8, 8)
+    m = swe.SWE2D(z, z, 0.1, 0.1, config=cfg).to(cuda_device)
+    U = torch.zeros(1, 3, 8, 8, device=cuda_device)
+    U[:, 0] = 1.0
+    with pytest.raises(FloatingPointError, match="Non-positive timestep"):
+        m.step(U, 0.0, 0.0)
+    Un = U.clone()
+    Un[0, 1, 3, 3] = float("nan")
+    with pytest.raises(FloatingPointError, match="non-finite"):
+        m.step(Un, 0.0, 1.0)
+    with pytest.raises(FloatingPointError, match="non-finite"):
+        m.forward(Un, t_end=1.0)
+    with pytest.raises(RuntimeError, match="max_steps=2 reached"):
+        m.forward(U, t_end=10.0, max_steps=2)
+    with pytest.raises(ValueError, match="cannot be negative"):
+        m.forward(U, t_end=-1.0)
+    with pytest.raises(RuntimeError, match="ROCm device only"):
+        m.step(U.cpu(), 0.0, 1.0)
