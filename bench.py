@@ -1,0 +1,220 @@
+"""Benchmark of the HIP shallow-water time step on one MI355X.
+
+Workload: the 8192 x 8192 tsunami basin (dx = 100 m, fractal bathymetry,
+~15 % land, 2 m Gaussian hump, Manning n = 0.025, transmissive sides), FP32.
+A step is one adaptive midpoint step (two fused launches) over the whole grid.
+
+    python bench.py [--gpus 1] [--steps 2000] [--warmup 20]
+
+Prints one JSON line.  ``value`` is cell-updates per second over the K timed
+steps (inputs resident in HBM, device-driven loop, no host sync inside).
+``roofline`` prices the dominant kernel with the bytes it must move; a second
+figure prices the whole step at 32 B per cell-update (read h, hu, hv, z, n and
+write h, hu, hv once).  ``cpu_baseline`` times the CPU oracle (an eager
+PyTorch restatement of the reference step) on a 2048 x 2048 crop of the same
+basin on the host cores.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "Gcell-updates/s fwd+adj FP32 & %HBM peak; inversion time-to-misfit vs plain"
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2000)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--n", type=int, default=8192, help="grid side (default: the 8192^2 basin)")
+    ap.add_argument("--rows", type=int, default=0, help="rows per wavefront strip (0 = autotuned / heuristic)")
+    ap.add_argument("--kernel-steps", type=int, default=50, help="steps timed launch by launch with events")
+    ap.add_argument("--cpu-side", type=int, default=2048, help="crop side of the CPU-baseline sample")
+    ap.add_argument("--cpu-steps", type=int, default=3)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="also run a 1 GiB device copy (calibrates rocprof FETCH_SIZE / WRITE_SIZE)")
+    return ap.parse_args()
+
+
+def basin_model(n, device):
+    import torch
+
+    import diffswe2d_amd as swe
+    from diffswe2d_amd import scenarios
+
+    t0 = time.time()
+    sc = scenarios.tsunami_basin(n=n)
+    cfg = swe.SWEConfig(boundary_left="transmissive", boundary_right="transmissive",
+                        boundary_top="transmissive", boundary_bottom="transmissive")
+    bed = torch.from_numpy(sc["bed"])
+    rough = torch.from_numpy(sc["rough"])
+    model = swe.SWE2D(bed, rough, sc["dx"], sc["dy"], config=cfg, train_dem=False)
+    model = model.to(device)
+    U = torch.from_numpy(sc["U"]).to(device)
+    print(f"# basin {n}x{n} built in {time.time() - t0:.1f} s (land {sc['land']:.3f})", file=sys.stderr,
+          flush=True)
+    return model, U, sc, cfg
+
+
+def time_kernels(model, U, steps, rows):
+    """Average duration of each launch, bracketed by events on torch's current stream."""
+    import torch
+
+    from diffswe2d_amd import _lib
+
+    lib = _lib.load()
+    bed, rough = model._fields(U)
+    ws = model._workspace(U, bed, rough)
+    lvl, stride = model._level_arg(0.0, U)
+    stream = _lib.stream_ptr(U.device)
+    A, Bf = U.clone(), torch.empty_like(U)
+    evs = [[torch.cuda.Event(enable_timing=True) for _ in range(3)] for _ in range(steps)]
+    torch.cuda.synchronize()
+    for k in range(steps):
+        e = evs[k]
+        e[0].record()
+        _lib.check(lib.swe_step_stage_f32(ws["grid"], ws["cfg"], 1, _lib.ptr(A), _lib.ptr(bed), _lib.ptr(rough),
+                                          _lib.ptr(lvl), stride, 1e30, ws["parity"], _lib.ptr(Bf),
+                                          _lib.ptr(ws["buf"]), rows, stream), "stage 1")
+        e[1].record()
+        _lib.check(lib.swe_step_stage_f32(ws["grid"], ws["cfg"], 2, _lib.ptr(A), _lib.ptr(bed), _lib.ptr(rough),
+                                          _lib.ptr(lvl), stride, 1e30, ws["parity"], _lib.ptr(Bf),
+                                          _lib.ptr(ws["buf"]), rows, stream), "stage 2")
+        e[2].record()
+        ws["parity"] ^= 1
+        A, Bf = Bf, A
+    torch.cuda.synchronize()
+    s1 = sum(e[0].elapsed_time(e[1]) for e in evs) / steps
+    s2 = sum(e[1].elapsed_time(e[2]) for e in evs) / steps
+    return s1, s2
+
+
+def cpu_baseline(sc, cfg, side, steps):
+    """Oracle (eager PyTorch CPU, FP32) on a side x side crop around the hump."""
+    import numpy as np
+    import torch
+
+    from oracle import swe_oracle as O
+
+    n = sc["U"].shape[-1]
+    h = sc["U"][0, 0]
+    cy, cx = np.unravel_index(np.argmax(h - h.mean()), h.shape)
+    y0 = int(min(max(cy - side // 2, 0), n - side))
+    x0 = int(min(max(cx - side // 2, 0), n - side))
+    sl = (slice(None), slice(None), slice(y0, y0 + side), slice(x0, x0 + side))
+    U = torch.from_numpy(np.ascontiguousarray(sc["U"][sl]))
+    bed = torch.from_numpy(np.ascontiguousarray(sc["bed"][sl]))
+    rough = torch.from_numpy(np.ascontiguousarray(sc["rough"][sl]))
+    threads = torch.get_num_threads()
+    with torch.no_grad():
+        U, _ = O.step(U, bed, rough, cfg, sc["dx"], sc["dy"], 1e30)  # warm-up
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            U, _ = O.step(U, bed, rough, cfg, sc["dx"], sc["dy"], 1e30)
+        sec = time.perf_counter() - t0
+    rate = side * side * steps / sec / 1e9
+    return dict(value=rate, unit="Gcell-updates/s", cores=threads, kind="port",
+                sample=f"oracle FP32, {side}x{side} crop of the basin around the hump, 1 warm-up + {steps} "
+                       f"timed steps, {sec / steps:.3f} s/step on {threads} threads")
+
+
+def main():
+    a = parse()
+    if a.gpus != 1:
+        print(json.dumps({"error": "this build measures one GPU only; --gpus must be 1"}))
+        sys.exit(2)
+    import torch
+
+    if not torch.cuda.is_available():
+        print(json.dumps({"error": "no GPU visible"}))
+        sys.exit(1)
+    dev = torch.device("cuda:0")
+    model, U, sc, cfg = basin_model(a.n, dev)
+    if a.rows:
+        model.rows_hint = a.rows
+    n = a.n
+    cells = n * n
+
+    from diffswe2d_amd import _lib
+
+    lib = _lib.load()
+    bed, rough = model._fields(U)
+    ws = model._workspace(U, bed, rough)
+    lvl, stride = model._level_arg(0.0, U)
+    stream = _lib.stream_ptr(dev)
+    Ua, Ub = U.clone(), torch.empty_like(U)
+    dts = torch.empty(a.steps + a.warmup + 8, dtype=torch.float32, device=dev)
+
+    def run(first, count):
+        _lib.check(lib.swe_advance_f32(ws["grid"], ws["cfg"], _lib.ptr(bed), _lib.ptr(rough), _lib.ptr(lvl),
+                                       stride, _lib.ptr(Ua), _lib.ptr(Ub), first, count, None, _lib.ptr(dts),
+                                       dts.numel(), None, 0, None, None, 0, _lib.ptr(ws["buf"]),
+                                       int(model.rows_hint), stream), "advance")
+
+    _lib.check(lib.swe_run_begin(_lib.ptr(ws["buf"]), 1e30, -1.0, 4, stream), "run_begin")
+    run(0, a.warmup)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    run(a.warmup, a.steps)
+    torch.cuda.synchronize()
+    sec = time.perf_counter() - t0
+    ctl = _lib.read_control(ws["buf"])
+    if ctl.status != 0 or ctl.steps != a.warmup + a.steps:
+        raise RuntimeError(f"run failed: status {ctl.status}, steps {ctl.steps}")
+    ms = sec / a.steps * 1e3
+    gcells = cells * a.steps / sec / 1e9
+    dt_mean = float(dts[a.warmup:a.warmup + a.steps].double().mean())
+    print(f"# {a.steps} steps in {sec:.3f} s: {ms:.3f} ms/step, {gcells:.2f} Gcell/s, mean dt {dt_mean:.4f} s",
+          file=sys.stderr, flush=True)
+
+    s1, s2 = time_kernels(model, U, a.kernel_steps, int(model.rows_hint))
+    # One cell-update moves 32 algorithmic bytes (read h, hu, hv, z, n; write h, hu, hv).
+    # The unit of work is one step = the two launches, so the roofline prices the
+    # sum of the two event-timed kernel durations.
+    step_kernel_ms = s1 + s2
+    achieved = 32.0 * cells / (step_kernel_ms * 1e-3) / 1e9
+    print(f"# events: stage1 {s1:.3f} ms, stage2 {s2:.3f} ms", file=sys.stderr, flush=True)
+    if a.copy_probe:
+        src = torch.empty(1 << 28, dtype=torch.float32, device=dev)
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print("# copy probe: 3 x 1 GiB device copies", file=sys.stderr, flush=True)
+
+    cpu = None
+    if not a.no_cpu:
+        torch.set_num_threads(max(1, min(os.cpu_count() or 1, int(os.environ.get("OMP_NUM_THREADS", "16")))))
+        side = min(a.cpu_side, n)
+        cpu = cpu_baseline(sc, cfg, side, a.cpu_steps)
+        print(f"# cpu baseline {cpu['value']:.4f} Gcell/s ({cpu['sample']})", file=sys.stderr, flush=True)
+
+    out = {
+        "metric": METRIC, "value": round(gcells, 4), "unit": "Gcell-updates/s", "n_gpus": 1,
+        "steps": a.steps, "warmup": a.warmup, "ms_per_step": round(ms, 4), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": f"tsunami_basin_{n}x{n}_dx100m_manning0.025_transmissive_cfl0.5",
+                   "cells": cells, "steps_of_run": a.steps, "launches_per_step": 2,
+                   "rows_per_strip": int(model.rows_hint)},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "kernel": "step (stage1 + stage2)", "algorithmic_bytes_per_cell_update": 32,
+                     "kernel_ms": round(step_kernel_ms, 4)},
+        "kernel_ms": {"stage1": round(s1, 4), "stage2": round(s2, 4)},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

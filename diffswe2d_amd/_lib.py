@@ -60,6 +60,7 @@ _SIGNATURES = {
     "swe_spatial_operator_f64": ([_G, _C, _P, _P, _P, _I, _P, _P, _P, _P, _I, _P], _I),
     "swe_step_f32": ([_G, _C, _P, _P, _P, _P, _I, _D, _I, _P, _P, _I, _P], _I),
     "swe_step_f64": ([_G, _C, _P, _P, _P, _P, _I, _D, _I, _P, _P, _I, _P], _I),
+    "swe_step_stage_f32": ([_G, _C, _I, _P, _P, _P, _P, _I, _D, _I, _P, _P, _I, _P], _I),
     "swe_run_begin": ([_P, _D, _D, _I, _P], _I),
     "swe_advance_f32": ([_G, _C, _P, _P, _P, _I, _P, _P, _I, _I, _P, _P, _I, _P, _I, _P, _P, _I, _P, _I, _P], _I),
     "swe_advance_f64": ([_G, _C, _P, _P, _P, _I, _P, _P, _I, _I, _P, _P, _I, _P, _I, _P, _P, _I, _P, _I, _P], _I),

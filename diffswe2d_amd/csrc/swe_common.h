@@ -32,22 +32,6 @@ struct Control {
   double last_dt;      // dt of the most recent step (host-driven steps)
 };
 
-// accumulator bank: [3 banks][2 (x, y)][kSlots] raw bits of non-negative speeds
-template <class T> struct Bits;
-template <> struct Bits<float> { using U = unsigned int; };
-template <> struct Bits<double> { using U = unsigned long long; };
-
-template <class T> __device__ inline typename Bits<T>::U to_bits(T v);
-template <> __device__ inline unsigned int to_bits<float>(float v) { return __float_as_uint(v); }
-template <> __device__ inline unsigned long long to_bits<double>(double v) {
-  return (unsigned long long)__double_as_longlong(v);
-}
-template <class T> __device__ inline T from_bits(typename Bits<T>::U b);
-template <> __device__ inline float from_bits<float>(unsigned int b) { return __uint_as_float(b); }
-template <> __device__ inline double from_bits<double>(unsigned long long b) {
-  return __longlong_as_double((long long)b);
-}
-
 // Grid geometry and field pointers of one model (one batch of members).
 template <class T> struct Fields {
   int B, ny, nx;
@@ -69,13 +53,13 @@ struct GhostMap {
   int side;
 };
 
-__device__ inline int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
+SWE_HD int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
 
 // Map padded coordinates (r, c) -- interior rows/cols are [0, ny) x [0, nx) -- to
 // their source cells.  Sides act on their own span only: left/right ghosts of
 // interior rows, top/bottom ghosts of interior columns (boundary.py:301-411);
 // corner ghosts never reach interior cells and are filled by replication.
-__device__ inline GhostMap ghost_map(int r, int c, int ny, int nx, const int* bc) {
+SWE_HD GhostMap ghost_map(int r, int c, int ny, int nx, const int* bc) {
   GhostMap g;
   bool in_r = r >= 0 && r < ny, in_c = c >= 0 && c < nx;
   g.kind = -1;
@@ -125,7 +109,7 @@ __device__ inline GhostMap ghost_map(int r, int c, int ny, int nx, const int* bc
 // other sides) to the state fetched at the edge cell (h, hu, hv) and the
 // normal momentum fetched at the mirror cell.
 template <class T>
-__device__ inline void apply_ghost(const GhostMap& g, T& h, T& hu, T& hv, T mirror_normal, T z_ghost, T level,
+SWE_HD void apply_ghost(const GhostMap& g, T& h, T& hu, T& hv, T mirror_normal, T z_ghost, T level,
                                    const SweConsts<T>& k) {
   if (g.kind < 0) return;
   bool x_side = g.side == LEFT || g.side == RIGHT;
@@ -152,6 +136,69 @@ __device__ inline void apply_ghost(const GhostMap& g, T& h, T& hu, T& hv, T mirr
     }
     default:  // transmissive, periodic: values as fetched
       break;
+  }
+}
+
+}  // namespace swe
+
+namespace swe {
+
+// Adjoint of apply_ghost: the adjoint (gh, ghu, ghv) of a ghost cell's raw
+// state goes back to the edge cell (ge_*), the mirror cell's normal momentum
+// (g_mirror) and the boundary level (g_level).  h_edge is the raw depth of the
+// edge cell (the replicated value the absorbing rule reads).
+template <class T>
+SWE_HD void ghost_bwd(const GhostMap& g, T h_edge, T z_ghost, T level, T gh, T ghu, T ghv, const SweConsts<T>& k,
+                      T& ge_h, T& ge_hu, T& ge_hv, T& g_mirror, T& g_level) {
+  ge_h = ge_hu = ge_hv = g_mirror = g_level = T(0);
+  bool x_side = g.side == LEFT || g.side == RIGHT;
+  switch (g.kind) {
+    case BC_CONSTANT:
+      return;
+    case BC_WATER_LEVEL:
+      if (level - z_ghost >= T(0)) g_level = gh;
+      ge_hu = ghu;
+      ge_hv = ghv;
+      return;
+    case BC_WALL:
+      ge_h = gh;
+      if (x_side) {
+        g_mirror = -ghu;
+        ge_hv = ghv;
+      } else {
+        g_mirror = -ghv;
+        ge_hu = ghu;
+      }
+      return;
+    case BC_ABSORBING: {
+      T gflow = x_side ? ghu : ghv;
+      if (x_side) ge_hv = ghv;
+      else ge_hu = ghu;
+      ge_h = gh;
+      T d = level - z_ghost;
+      T h_ref = clamp_lo(d, T(0));
+      bool use_ref = h_ref > T(1e-4);
+      T h_c = use_ref ? h_ref : h_edge;
+      T cc = clamp_lo(h_c, T(1e-6));
+      T c = sqrt(T(9.81) * cc);
+      T diff = h_edge - h_ref;
+      bool outward_pos = g.side == RIGHT || g.side == BOTTOM;
+      T sgn = outward_pos ? T(1) : T(-1);
+      T gc = sgn * gflow * diff;
+      T gdiff = sgn * gflow * c;
+      ge_h += gdiff;
+      T g_href = -gdiff;
+      T ghc = h_c >= T(1e-6) ? gc * T(9.81) / (T(2) * c) : T(0);
+      if (use_ref) g_href += ghc;
+      else ge_h += ghc;
+      if (d >= T(0)) g_level = g_href;
+      return;
+    }
+    default:  // transmissive, periodic
+      ge_h = gh;
+      ge_hu = ghu;
+      ge_hv = ghv;
+      return;
   }
 }
 

@@ -22,6 +22,22 @@
 
 namespace swe {
 
+// accumulator bank: [3 banks][2 (x, y)][kSlots] raw bits of non-negative speeds
+template <class T> struct Bits;
+template <> struct Bits<float> { using U = unsigned int; };
+template <> struct Bits<double> { using U = unsigned long long; };
+
+template <class T> __device__ inline typename Bits<T>::U to_bits(T v);
+template <> __device__ inline unsigned int to_bits<float>(float v) { return __float_as_uint(v); }
+template <> __device__ inline unsigned long long to_bits<double>(double v) {
+  return (unsigned long long)__double_as_longlong(v);
+}
+template <class T> __device__ inline T from_bits(typename Bits<T>::U b);
+template <> __device__ inline float from_bits<float>(unsigned int b) { return __uint_as_float(b); }
+template <> __device__ inline double from_bits<double>(unsigned long long b) {
+  return __longlong_as_double((long long)b);
+}
+
 template <class T> struct StageArgs {
   SweConsts<T> k;
   Fields<T> f;
@@ -340,11 +356,12 @@ int fail(int code, const char* msg) {
 
 constexpr size_t kCtrlBytes = 256;
 template <class T> constexpr size_t acc_bytes() { return 3 * 2 * kSlots * sizeof(typename Bits<T>::U); }
+constexpr size_t kAccRegion = 3 * 2 * kSlots * 8;  // sized for 8-byte words whatever the dtype
 inline size_t align256(size_t v) { return (v + 255) & ~(size_t)255; }
 
 template <class T> size_t workspace_bytes(const swe_grid* g) {
   size_t state = (size_t)g->batch * 3 * (size_t)g->ny * (size_t)g->nx * sizeof(T);
-  return kCtrlBytes + align256(acc_bytes<T>()) + align256(state);
+  return kCtrlBytes + kAccRegion + align256(state);
 }
 
 template <class T> struct WsView {
@@ -358,7 +375,7 @@ template <class T> WsView<T> ws_view(void* ws) {
   WsView<T> v;
   v.ctl = (Control*)p;
   v.acc = (typename Bits<T>::U*)(p + kCtrlBytes);
-  v.k1 = (T*)(p + kCtrlBytes + align256(acc_bytes<T>()));
+  v.k1 = (T*)(p + kCtrlBytes + kAccRegion);
   return v;
 }
 
@@ -493,11 +510,14 @@ int spatial_operator_impl(const swe_grid* g, const swe_config* c, const T* U, co
 }
 
 template <class T>
-int step_impl(const swe_grid* g, const swe_config* c, const T* U, const T* bed, const T* rough, const T* level,
-              int level_stride, double max_dt, int parity, T* U_out, void* ws, int rows_hint, void* stream) {
+int step_impl(const swe_grid* g, const swe_config* c, int stages, const T* U, const T* bed, const T* rough,
+              const T* level, int level_stride, double max_dt, int parity, T* U_out, void* ws, int rows_hint,
+              void* stream) {
   int e;
   if ((e = check_grid(g)) || (e = check_cfg(c)) || (e = check_level(level, level_stride, g, c))) return e;
-  if (!U || !bed || !U_out || !ws || (c->apply_friction && !rough)) return fail(SWE_ERR_ARG, "null pointer argument");
+  if (stages < 1 || stages > 3) return fail(SWE_ERR_ARG, "stages must be 1, 2 or 3");
+  if (!U || !bed || !ws || ((stages & 2) && (!U_out || (c->apply_friction && !rough))))
+    return fail(SWE_ERR_ARG, "null pointer argument");
   if (U == U_out) return fail(SWE_ERR_ARG, "U and U_out must not alias");
   hipStream_t s = (hipStream_t)stream;
   WsView<T> w = ws_view<T>(ws);
@@ -511,12 +531,14 @@ int step_impl(const swe_grid* g, const swe_config* c, const T* U, const T* bed, 
   a.max_dt = max_dt;
   a.ctl = w.ctl;
   a.clock = 0;
-  if ((e = launch_stage<T, 1>(a, s))) return e;
-  StageArgs<T> b2 = a;
-  b2.k1 = w.k1;
-  b2.out = U_out;
-  b2.record_wave = 0;
-  if ((e = launch_stage<T, 2>(b2, s))) return e;
+  if ((stages & 1) && (e = launch_stage<T, 1>(a, s))) return e;
+  if (stages & 2) {
+    StageArgs<T> b2 = a;
+    b2.k1 = w.k1;
+    b2.out = U_out;
+    b2.record_wave = 0;
+    if ((e = launch_stage<T, 2>(b2, s))) return e;
+  }
   return 0;
 }
 
@@ -599,8 +621,8 @@ size_t swe_workspace_bytes(const swe_grid* g, int dtype_bytes) {
 size_t swe_control_bytes(void) { return sizeof(Control); }
 
 int swe_workspace_init(void* ws, size_t bytes, void* stream) {
-  if (!ws || bytes < kCtrlBytes + 3 * 2 * kSlots * 8) return fail(SWE_ERR_ARG, "workspace too small");
-  int64_t words = (int64_t)(kCtrlBytes + align256(3 * 2 * kSlots * 8)) / 4;
+  if (!ws || bytes < kCtrlBytes + kAccRegion) return fail(SWE_ERR_ARG, "workspace too small");
+  int64_t words = (int64_t)(kCtrlBytes + kAccRegion) / 4;
   hipLaunchKernelGGL(zero_words, dim3((unsigned)((words + 255) / 256)), dim3(256), 0, (hipStream_t)stream,
                      (uint32_t*)ws, words);
   hipError_t e = hipGetLastError();
@@ -622,13 +644,23 @@ int swe_spatial_operator_f64(const swe_grid* g, const swe_config* c, const doubl
 int swe_step_f32(const swe_grid* g, const swe_config* c, const float* U, const float* bed, const float* rough,
                  const float* level, int level_stride, double max_dt, int parity, float* U_out, void* ws,
                  int rows_hint, void* stream) {
-  return step_impl<float>(g, c, U, bed, rough, level, level_stride, max_dt, parity, U_out, ws, rows_hint, stream);
+  return step_impl<float>(g, c, 3, U, bed, rough, level, level_stride, max_dt, parity, U_out, ws, rows_hint,
+                          stream);
 }
 
 int swe_step_f64(const swe_grid* g, const swe_config* c, const double* U, const double* bed, const double* rough,
                  const double* level, int level_stride, double max_dt, int parity, double* U_out, void* ws,
                  int rows_hint, void* stream) {
-  return step_impl<double>(g, c, U, bed, rough, level, level_stride, max_dt, parity, U_out, ws, rows_hint, stream);
+  return step_impl<double>(g, c, 3, U, bed, rough, level, level_stride, max_dt, parity, U_out, ws, rows_hint,
+                           stream);
+}
+
+int swe_step_stage_f32(const swe_grid* g, const swe_config* c, int stage, const float* U, const float* bed,
+                       const float* rough, const float* level, int level_stride, double max_dt, int parity,
+                       float* U_out, void* ws, int rows_hint, void* stream) {
+  if (stage != 1 && stage != 2) return fail(SWE_ERR_ARG, "stage must be 1 or 2");
+  return step_impl<float>(g, c, stage, U, bed, rough, level, level_stride, max_dt, parity, U_out, ws, rows_hint,
+                          stream);
 }
 
 int swe_run_begin(void* ws, double duration, double max_dt_cap, int dtype_bytes, void* stream) {

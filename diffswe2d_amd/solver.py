@@ -387,6 +387,13 @@ class SWE2D(nn.Module):
         Ub = torch.empty_like(U0)
         hmax = U0[:, 0].clone()
         names = list(self.gauge_indices) if self.track_gauges else []
+        ny, nx = U0.shape[-2:]
+        for n_ in names:
+            iy, ix = self.gauge_indices[n_]
+            if not (0 <= int(iy) < ny and 0 <= int(ix) < nx):
+                raise IndexError(f"gauge {n_!r} at {(iy, ix)} lies outside the {ny}x{nx} grid")
+        if len(names) > 1024:
+            raise ValueError("at most 1024 gauges are tracked on the device")
         gauges = (torch.tensor([list(self.gauge_indices[n]) for n in names], dtype=torch.int32,
                                device=U0.device).contiguous() if names else None)
         capacity = 256

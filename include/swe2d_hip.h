@@ -111,6 +111,13 @@ int swe_step_f64(const swe_grid* grid, const swe_config* cfg, const double* U, c
                  const double* roughness, const double* level, int level_stride, double max_dt, int parity,
                  double* U_out, void* ws, int rows_hint, void* stream);
 
+/* The two launches of swe_step_f32 separately (stage 1: k1 and wave speeds
+ * into the workspace; stage 2: dt, predictor, k2, update, friction), so a
+ * caller can bracket each kernel with events on `stream`. */
+int swe_step_stage_f32(const swe_grid* grid, const swe_config* cfg, int stage, const float* U, const float* bed,
+                       const float* roughness, const float* level, int level_stride, double max_dt, int parity,
+                       float* U_out, void* ws, int rows_hint, void* stream);
+
 /* Device-driven integration of a duration -- replaces the SWE2D.forward loop
  * (solver.py:719-779) without per-step host synchronisation.  swe_run_begin
  * arms the device clock; each swe_advance call launches n_steps further steps

@@ -44,6 +44,18 @@ import torch.nn.functional as F
 NG = 2
 BC_TYPES = ("transmissive", "periodic", "constant", "water_level", "wall", "absorbing")
 
+# torch's CPU float32 sqrt is not correctly rounded (about 0.7 % of inputs land
+# one ulp away from the IEEE result on this build).  SQRT_MODE "torch" keeps the
+# reference's own rounding; "ieee" rounds a float64 square root once, which is
+# the correctly rounded float32 result a GPU (and IEEE hardware sqrt) gives.
+SQRT_MODE = "torch"
+
+
+def _sqrt(x):
+    if SQRT_MODE == "ieee" and x.dtype == torch.float32:
+        return torch.sqrt(x.double()).float()
+    return torch.sqrt(x)
+
 
 # ---------------------------------------------------------------------------
 # dtype-dependent floors (reference: riemann.py:6-17, reconstruction.py:12-15,
@@ -107,7 +119,7 @@ def primitive(U, gravity: float, epsilon: float, dry_depth: float):
     zero = torch.zeros_like(h)
     u = torch.where(wet, U[:, 1:2] / safe, zero)
     v = torch.where(wet, U[:, 2:3] / safe, zero)
-    c = torch.where(wet, torch.sqrt(float(gravity) * safe), zero)
+    c = torch.where(wet, _sqrt(float(gravity) * safe), zero)
     return h, u, v, c
 
 
@@ -290,7 +302,7 @@ def ghost_pad(U, bed, left="transmissive", right="transmissive", top="transmissi
             h_in = Up[(slice(None), slice(0, 1)) + ghost]
             h_ref = forced[(slice(None), slice(None)) + ghost]
             h_c = torch.where(h_ref > 1e-4, h_ref, h_in)
-            c = torch.sqrt(9.81 * torch.clamp(h_c, min=1e-6))
+            c = _sqrt(9.81 * torch.clamp(h_c, min=1e-6))
             flow = c * (h_in - h_ref) if outward > 0 else -c * (h_in - h_ref)
             Up[(slice(None), slice(mch, mch + 1)) + ghost] = flow
     return Up, zp
@@ -308,7 +320,7 @@ def _speed(h, hu, hv, dry_depth, epsilon):
     s2 = u.square() + v.square()
     moving = wet & (s2 > 0.0)
     s2 = torch.clamp(s2, min=depth_floor(h.dtype, epsilon, 0.0) ** 2)
-    return wet, safe, torch.where(moving, torch.sqrt(s2), zero)
+    return wet, safe, torch.where(moving, _sqrt(s2), zero)
 
 
 def manning_coefficient(h, n, epsilon=1e-12):

@@ -110,10 +110,14 @@ def test_dam_break_full_run(golden, cuda_device):
     h = U[0, 0].cpu()
     assert torch.equal(h, h.T)
     assert torch.equal(U[0, 1].cpu(), U[0, 2].cpu().T)
-    # walls and no dry cells: mass is conserved to rounding
-    m0 = float(torch.tensor(sc["U"][0, 0]).sum())
-    m1 = float(U[0, 0].double().sum())
-    assert abs(m1 - m0) / m0 < 1e-5
+    # no dry cell appears (the positivity limiter never fires) ...
+    assert float(U[0, 0].min()) > 0.2
+    # ... and the mass change over the run (the reference's wall faces are not
+    # exactly conservative: +2.55e-4 relative after 500 steps) is the reference's
+    m0 = float(np.asarray(sc["U"][0, 0], dtype=np.float64).sum())
+    dm_gpu = float(U[0, 0].double().sum()) - m0
+    dm_ref = float(d["U500_f32"][0, 0].astype(np.float64).sum()) - m0
+    assert abs(dm_gpu - dm_ref) / m0 < 1e-7
 
 
 def test_forward_device_loop_matches_host_steps(cuda_device):
@@ -139,8 +143,37 @@ def test_forward_device_loop_matches_host_steps(cuda_device):
     assert abs(m.ts_time[-1] - 1.5) < 1e-12
 
 
+def oracle_runup_ieee(d):
+    """The oracle with correctly rounded FP32 sqrt on the same run-up (dt series, final state)."""
+    import oracle.swe_oracle as OO
+
+    cfg = swe.SWEConfig(boundary_left="water_level", boundary_right="wall", boundary_top="wall",
+                        boundary_bottom="wall")
+    old = OO.SQRT_MODE
+    OO.SQRT_MODE = "ieee"
+    try:
+        U = torch.tensor(d["U0"], dtype=torch.float32)
+        bed, rough = torch.tensor(d["bed"], dtype=torch.float32), torch.tensor(d["rough"], dtype=torch.float32)
+        dts = []
+        with torch.no_grad():
+            for k in range(len(d["levels"])):
+                U, _ = OO.forward(U, bed, rough, cfg, 0.04, 0.04, 0.05, start_time=0.05 * k,
+                                  boundary_level=float(d["levels"][k]), dt_log=dts)
+    finally:
+        OO.SQRT_MODE = old
+    return np.array(dts), U
+
+
 def test_forward_matches_oracle_runup(golden, cuda_device):
-    """Reduced coastal run-up (136 x 86, dx 0.04), 40 chunks of 0.05 s: moving wet/dry front."""
+    """Reduced coastal run-up (136 x 86, dx 0.04), 40 chunks of 0.05 s: moving wet/dry front.
+
+    Bars against the reference (CPU, FP32): rel-L2 <= 1e-5, identical wet/dry
+    masks, same step count.  The dt series is held to 1e-6 against the oracle
+    with IEEE square roots; against the CPU reference it is held to 2e-6,
+    because torch's CPU float32 sqrt is itself off by up to one ulp and moves
+    the reference's dt series by 1.66e-6 on this run (measured: IEEE-sqrt
+    oracle vs reference).
+    """
     d = golden("runup")
     cfg = swe.SWEConfig(boundary_left="water_level", boundary_right="wall", boundary_top="wall",
                         boundary_bottom="wall")
@@ -165,11 +198,16 @@ def test_forward_matches_oracle_runup(golden, cuda_device):
     e64 = relerr(U, d["U_f64"])
     ref_masks = np.unpackbits(d["mask_f32"])[: np.stack(masks).size].reshape(np.stack(masks).shape).astype(bool)
     mismatch = int((np.stack(masks) != ref_masks).sum())
-    print(f"run-up: {len(dts)} steps, rel-L2 vs ref FP32 {e32:.3e}, vs FP64 {e64:.3e}, dt max rel "
-          f"{dt_err:.3e}, mask mismatches {mismatch}")
+    ieee_dt, ieee_U = oracle_runup_ieee(d)
+    dt_err_ieee = float(np.max(np.abs(dts - ieee_dt) / ieee_dt)) if len(ieee_dt) == len(dts) else float("inf")
+    e_ieee = relerr(U, ieee_U)
+    print(f"run-up: {len(dts)} steps, rel-L2 vs ref FP32 {e32:.3e}, vs FP64 {e64:.3e}, vs IEEE-sqrt oracle "
+          f"{e_ieee:.3e}; dt max rel vs ref {dt_err:.3e}, vs IEEE-sqrt oracle {dt_err_ieee:.3e}; "
+          f"mask mismatches {mismatch}")
     assert e32 <= 1e-5
-    assert dt_err <= 1e-6
     assert mismatch == 0
+    assert dt_err_ieee <= 1e-6
+    assert dt_err <= 2e-6
 
 
 def test_error_paths(cuda_device):
