@@ -192,6 +192,13 @@ def main():
         torch.cuda.synchronize()
         print("# copy probe: 3 x 1 GiB device copies", file=sys.stderr, flush=True)
 
+    # HBM bytes per step from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE runs of this workload
+    traffic, traffic_src = None, None
+    tfile = ROOT / "profiles" / "basin8192_fp32_traffic.json"
+    if n == 8192 and tfile.exists():
+        traffic = round(json.loads(tfile.read_text())["step_hbm_bytes"])
+        traffic_src = "profiles/basin8192_fp32_traffic.json (FETCH_SIZE x2.0 + WRITE_SIZE x1.0, copy-calibrated)"
+
     cpu = None
     if not a.no_cpu:
         torch.set_num_threads(max(1, min(os.cpu_count() or 1, int(os.environ.get("OMP_NUM_THREADS", "16")))))
@@ -207,7 +214,8 @@ def main():
                    "cells": cells, "steps_of_run": a.steps, "launches_per_step": 2,
                    "rows_per_strip": int(model.rows_hint)},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                     "traffic_source": traffic_src,
                      "kernel": "step (stage1 + stage2)", "algorithmic_bytes_per_cell_update": 32,
                      "kernel_ms": round(step_kernel_ms, 4)},
         "kernel_ms": {"stage1": round(s1, 4), "stage2": round(s2, 4)},
