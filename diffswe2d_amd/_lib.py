@@ -61,9 +61,17 @@ _SIGNATURES = {
     "swe_step_f32": ([_G, _C, _P, _P, _P, _P, _I, _D, _I, _P, _P, _I, _P], _I),
     "swe_step_f64": ([_G, _C, _P, _P, _P, _P, _I, _D, _I, _P, _P, _I, _P], _I),
     "swe_step_stage_f32": ([_G, _C, _I, _P, _P, _P, _P, _I, _D, _I, _P, _P, _I, _P], _I),
+    "swe_step_record_f32": ([_G, _C, _P, _P, _P, _P, _I, _D, _I, _P, _P, _P, _P, _I, _P], _I),
+    "swe_step_record_f64": ([_G, _C, _P, _P, _P, _P, _I, _D, _I, _P, _P, _P, _P, _I, _P], _I),
     "swe_run_begin": ([_P, _D, _D, _I, _P], _I),
     "swe_advance_f32": ([_G, _C, _P, _P, _P, _I, _P, _P, _I, _I, _P, _P, _I, _P, _I, _P, _P, _I, _P, _I, _P], _I),
     "swe_advance_f64": ([_G, _C, _P, _P, _P, _I, _P, _P, _I, _I, _P, _P, _I, _P, _I, _P, _P, _I, _P, _I, _P], _I),
+    "swe_adjoint_workspace_bytes": ([_G, _I], ctypes.c_size_t),
+    "swe_adjoint_last_error_message": ([], ctypes.c_char_p),
+    "swe_operator_adjoint_f32": ([_G, _C, _P, _P, _P, _I, _P, _P, _P, _P, _I, _P], _I),
+    "swe_operator_adjoint_f64": ([_G, _C, _P, _P, _P, _I, _P, _P, _P, _P, _I, _P], _I),
+    "swe_step_adjoint_f32": ([_G, _C, _P, _P, _P, _P, _P, _P, _I, _D, _P, _P, _P, _P, _I, _P], _I),
+    "swe_step_adjoint_f64": ([_G, _C, _P, _P, _P, _P, _P, _P, _I, _D, _P, _P, _P, _P, _I, _P], _I),
 }
 
 _lib = None

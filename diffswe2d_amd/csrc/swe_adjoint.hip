@@ -1,0 +1,557 @@
+// Hand-written adjoint of one time step on gfx950.
+//
+// The step U -> U' (solver.py:572-663) with dt held fixed (the reference
+// detaches dt, :625-626) is reversed by six launches:
+//   adj_final   friction and corrector clean-up reversed     -> lam_k2, lam_acc
+//   adj_L<2>    L^T at the predictor P = clean(U, U + dt/2 k1) -> lam_P (+ ghosts)
+//   adj_ghost   ghost-cell adjoints folded back onto the edge cells; level
+//   adj_mid     predictor clean-up reversed                  -> lam_k1, lam_acc
+//   adj_L<1>    L^T at U                                     -> lam_U = lam_acc + ...
+//   adj_ghost   as above
+// L^T mirrors the forward stage's decomposition: one wavefront streams down a
+// strip of 64 columns, now with a 3-cell halo (58 outputs): a cell's adjoint
+// depends on the four faces within two cells of it and those faces on the
+// cells within three.  Forward values are recomputed from the stored state, so
+// every branch (minmod selection, HLL region, wet/dry tests) is taken exactly
+// as in the forward.
+#include <hip/hip_runtime.h>
+
+#include <math.h>
+#include <stdint.h>
+#include <string.h>
+
+#include "swe_adjoint_math.h"
+#include "swe_common.h"
+#include "swe_fetch.h"
+#include "../../include/swe2d_hip.h"
+
+namespace swe {
+
+constexpr int kHaloA = 3;
+constexpr int kOutA = kWave - 2 * kHaloA;  // 58
+
+template <class T> struct AdjArgs {
+  SweConsts<T> k;
+  Fields<T> f;
+  int strips, segs, S;   // tiling of the padded plane [-2, ny+2) x [-2, nx+2)
+  int wsr, positivity;
+  const T* U;            // state at step start
+  const T* k1;           // first-stage tendency (stage 2 rebuilds the predictor from it)
+  T half_dt;
+  const T* lam;          // adjoint of the stage tendency [B,3,ny,nx]
+  const T* base;         // added to the interior output (nullable)
+  T* out;                // adjoint of the stage input state [B,3,ny,nx]
+  T* ghost;              // [B][4 sides][2 layers][lmax][3]
+  int lmax;
+};
+
+template <class T> struct Lam3 {
+  T h, u, v;
+};
+
+template <class T> __device__ inline Lam3<T> load_lam(const AdjArgs<T>& a, int b, int r, int c) {
+  if (r < 0 || r >= a.f.ny || c < 0 || c >= a.f.nx) return Lam3<T>{T(0), T(0), T(0)};
+  int64_t i = (int64_t)b * 3 * a.f.plane + (int64_t)r * a.f.nx + c;
+  return Lam3<T>{a.lam[i], a.lam[i + a.f.plane], a.lam[i + 2 * a.f.plane]};
+}
+
+template <class T> __device__ inline SideGrad<T> shupg(const SideGrad<T>& g) {
+  return SideGrad<T>{shup(g.hlin), shup(g.eta), shup(g.un), shup(g.ut), shup(g.hcell)};
+}
+template <class T> __device__ inline Side<T> shdns(const Side<T>& s) { return shdn(s); }
+
+template <class T> __device__ inline void addpg(PrimGrad<T>& a, const PrimGrad<T>& b) {
+  a.h += b.h;
+  a.eta += b.eta;
+  a.un += b.un;
+  a.ut += b.ut;
+}
+
+// ghost buffer slot of padded cell (r, c), or -1 for interior / corner / far cells
+__device__ inline int64_t ghost_slot(int b, int r, int c, int ny, int nx, int lmax) {
+  bool in_r = r >= 0 && r < ny, in_c = c >= 0 && c < nx;
+  int side, layer, i;
+  if (in_r && !in_c) {
+    if (c == -1 || c == -2) {
+      side = LEFT;
+      layer = -c - 1;
+    } else if (c == nx || c == nx + 1) {
+      side = RIGHT;
+      layer = c - nx;
+    } else {
+      return -1;
+    }
+    i = r;
+  } else if (in_c && !in_r) {
+    if (r == -1 || r == -2) {
+      side = TOP;
+      layer = -r - 1;
+    } else if (r == ny || r == ny + 1) {
+      side = BOTTOM;
+      layer = r - ny;
+    } else {
+      return -1;
+    }
+    i = c;
+  } else {
+    return -1;
+  }
+  return ((((int64_t)b * 4 + side) * 2 + layer) * lmax + i) * 3;
+}
+
+// x sweep of one row (all lanes take part): adjoint of the primitives of each
+// lane's cell from the x-faces of the row.
+template <class T>
+__device__ inline PrimGrad<T> x_sweep_bwd(const AdjArgs<T>& a, const Prim<T>& P, const Lam3<T>& L) {
+  const SweConsts<T>& k = a.k;
+  Cell<T> cx = P.x();
+  Cell<T> cm = shup(cx), cp = shdn(cx);
+  bool reset;
+  Incr<T> xi = increments_r(cm, cx, cp, k, a.wsr != 0, reset);
+  Side<T> plus = side_plus(cx, xi, k);
+  Side<T> minus_next = shdn(side_minus(cx, xi, k));
+  T lh1 = shdn(L.h), lu1 = shdn(L.u), lv1 = shdn(L.v);
+  // face (c | c+1): normal momentum hu, tangential hv
+  T gfh = (lh1 - L.h) / k.dx;
+  T gfn = (lu1 - L.u) / k.dx;
+  T gft = (lv1 - L.v) / k.dx;
+  T gsL = L.u / k.dx;
+  T gsR = -lu1 / k.dx;
+  SideGrad<T> gL, gR;
+  face_flux_bwd(plus, cx.h, cx.z, minus_next, cp.h, cp.z, k, gfh, gfn, gft, gsL, gsR, gL, gR);
+  SideGrad<T> gm = shupg(gR);  // this cell as the right state of face (c-1 | c)
+  CellGrad<T> cg = sides_bwd(cx, xi, reset, gL, gm, k);
+  PrimGrad<T> own{cg.h, cg.eta, cg.un, cg.ut};
+  PrimGrad<T> to_m{T(0), T(0), T(0), T(0)}, to_p{T(0), T(0), T(0), T(0)};
+  increments_bwd(cm, cx, cp, cg, k, to_m, own, to_p);
+  own.h += shdn(to_m.h) + shup(to_p.h);
+  own.eta += shdn(to_m.eta) + shup(to_p.eta);
+  own.un += shdn(to_m.un) + shup(to_p.un);
+  own.ut += shdn(to_m.ut) + shup(to_p.ut);
+  return own;
+}
+
+template <class T, int STAGE>
+__global__ __launch_bounds__(256) void adj_L_kernel(AdjArgs<T> a) {
+  const int lane = threadIdx.x & (kWave - 1);
+  const int wid = blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6);
+  const int b = blockIdx.y;
+  const int strip = wid % a.strips;
+  const int seg = wid / a.strips;
+  if (seg >= a.segs) return;  // uniform per wavefront
+  const Fields<T>& f = a.f;
+  const SweConsts<T>& k = a.k;
+  const int x0 = -2 + strip * kOutA;
+  const int col = x0 - kHaloA + lane;
+  const int y0 = -2 + seg * a.S;
+  const int rows = min(a.S, f.ny + 2 - y0);
+  const bool lane_out = lane >= kHaloA && lane < kWave - kHaloA && col < f.nx + 2;
+  const T half = a.half_dt;
+
+  Prim<T> P3{}, P2{}, P1{};          // rows n-3, n-2, n-1
+  Lam3<T> L3{}, L2{}, L1{};          // adjoint of the tendency, same rows
+  Incr<T> yi_prev{};                 // y increments of row n-2
+  bool reset_prev = false;
+  SideGrad<T> gm_pend{};             // row n-1 as the lower state of face (n-2 | n-1)
+  SideGrad<T> gm_prev{};             // row n-2 as the lower state of face (n-3 | n-2)
+  PrimGrad<T> A3{}, A2{}, A1{};      // y-sweep adjoints of primitives, rows n-3..n-1
+
+  for (int it = 0; it < rows + 6; ++it) {
+    const int n = y0 - kHaloA + it;
+    Prim<T> Pn = fetch<T, STAGE, AdjArgs<T>>(a, b, n, col, half);
+    Lam3<T> Ln = load_lam(a, b, n, col);
+    Incr<T> yi_cur{};
+    bool reset_cur = false;
+    PrimGrad<T> An{T(0), T(0), T(0), T(0)};
+    if (it >= 2) yi_cur = increments_r(P2.y(), P1.y(), Pn.y(), k, a.wsr != 0, reset_cur);  // row n-1
+    if (it >= 3) {
+      // face (n-2 | n-1): normal momentum hv, tangential hu
+      Side<T> up = side_plus(P2.y(), yi_prev, k);
+      Side<T> dn = side_minus(P1.y(), yi_cur, k);
+      T gfh = (L1.h - L2.h) / k.dy;
+      T gfn = (L1.v - L2.v) / k.dy;
+      T gft = (L1.u - L2.u) / k.dy;
+      T gsL = L2.v / k.dy;
+      T gsR = -L1.v / k.dy;
+      SideGrad<T> gL, gR;
+      face_flux_bwd(up, P2.h, P2.z, dn, P1.h, P1.z, k, gfh, gfn, gft, gsL, gsR, gL, gR);
+      // row n-2 has both face adjoints now
+      CellGrad<T> cg = sides_bwd(P2.y(), yi_prev, reset_prev, gL, gm_prev, k);
+      A2.h += cg.h;
+      A2.eta += cg.eta;
+      A2.un += cg.un;
+      A2.ut += cg.ut;
+      increments_bwd(P3.y(), P2.y(), P1.y(), cg, k, A3, A2, A1);
+      gm_pend = gR;
+    }
+    if (it >= 6) {
+      // row n-3 is complete in y; add its x sweep and emit
+      const int r = n - 3;
+      PrimGrad<T> gx{T(0), T(0), T(0), T(0)};
+      if (r >= 0 && r < f.ny) gx = x_sweep_bwd(a, P3, L3);  // x-faces exist on interior rows only
+      T gh = A3.h + gx.h, geta = A3.eta + gx.eta;
+      T gu = A3.ut + gx.un;  // y sweep: un = v, ut = u
+      T gv = A3.un + gx.ut;
+      T lh, lu, lv;
+      primitive_bwd(P3.hr, P3.qu, P3.qv, k, gh, gu, gv, geta, lh, lu, lv);
+      if (lane_out) {
+        if (r >= 0 && r < f.ny && col >= 0 && col < f.nx) {
+          int64_t i = (int64_t)b * 3 * f.plane + (int64_t)r * f.nx + col;
+          if (a.base != nullptr) {
+            lh += a.base[i];
+            lu += a.base[i + f.plane];
+            lv += a.base[i + 2 * f.plane];
+          }
+          a.out[i] = lh;
+          a.out[i + f.plane] = lu;
+          a.out[i + 2 * f.plane] = lv;
+        } else {
+          int64_t gs = ghost_slot(b, r, col, f.ny, f.nx, a.lmax);
+          if (gs >= 0) {
+            a.ghost[gs] = lh;
+            a.ghost[gs + 1] = lu;
+            a.ghost[gs + 2] = lv;
+          }
+        }
+      }
+    }
+    // shift the pipeline
+    P3 = P2;
+    P2 = P1;
+    P1 = Pn;
+    L3 = L2;
+    L2 = L1;
+    L1 = Ln;
+    A3 = A2;
+    A2 = A1;
+    A1 = An;
+    yi_prev = yi_cur;
+    reset_prev = reset_cur;
+    gm_prev = gm_pend;
+  }
+}
+
+// Fold the ghost-cell adjoints back onto the cells they were copied from and
+// accumulate the boundary-level gradient (one block per member; sides in turn,
+// so no two threads touch the same cell).
+template <class T>
+__global__ __launch_bounds__(1024) void adj_ghost_kernel(AdjArgs<T> a, int stage, T* lam_level) {
+  __shared__ T part[1024];
+  const Fields<T>& f = a.f;
+  const int b = blockIdx.x;
+  const int64_t base = (int64_t)b * 3 * f.plane;
+  T glev_sum = T(0);
+  for (int side = 0; side < 4; ++side) {
+    int len = (side == LEFT || side == RIGHT) ? f.ny : f.nx;
+    for (int i = threadIdx.x; i < len; i += blockDim.x) {
+      for (int layer = 0; layer < 2; ++layer) {
+        int r, c;
+        if (side == LEFT) { r = i; c = -1 - layer; }
+        else if (side == RIGHT) { r = i; c = f.nx + layer; }
+        else if (side == TOP) { r = -1 - layer; c = i; }
+        else { r = f.ny + layer; c = i; }
+        GhostMap g = ghost_map(r, c, f.ny, f.nx, f.bc);
+        int64_t gs = ((((int64_t)b * 4 + side) * 2 + layer) * a.lmax + i) * 3;
+        T gh = a.ghost[gs], gu = a.ghost[gs + 1], gv = a.ghost[gs + 2];
+        T eh, eu, ev;
+        if (stage == 2) load_state<T, 2, AdjArgs<T>>(a, base, g.r, g.c, a.half_dt, eh, eu, ev);
+        else load_state<T, 1, AdjArgs<T>>(a, base, g.r, g.c, a.half_dt, eh, eu, ev);
+        T z = f.bed[(int64_t)b * f.bed_bstride + (int64_t)g.r * f.nx + g.c];
+        T level = (g.kind == BC_WATER_LEVEL || g.kind == BC_ABSORBING) ? f.level[b * f.level_stride] : T(0);
+        T ge_h, ge_u, ge_v, gmir, glev;
+        ghost_bwd(g, eh, z, level, gh, gu, gv, a.k, ge_h, ge_u, ge_v, gmir, glev);
+        int64_t e = base + (int64_t)g.r * f.nx + g.c;
+        a.out[e] += ge_h;
+        a.out[e + f.plane] += ge_u;
+        a.out[e + 2 * f.plane] += ge_v;
+        if (gmir != T(0)) {
+          int64_t m = base + (int64_t)g.mr * f.nx + g.mc;
+          if (side == LEFT || side == RIGHT) a.out[m + f.plane] += gmir;
+          else a.out[m + 2 * f.plane] += gmir;
+        }
+        glev_sum += glev;
+      }
+    }
+    __syncthreads();
+  }
+  part[threadIdx.x] = glev_sum;
+  __syncthreads();
+  for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+    if (threadIdx.x < s) part[threadIdx.x] += part[threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0 && lam_level != nullptr) lam_level[b] += part[0];
+}
+
+// friction and the corrector clean-up reversed (solver.py:646-661)
+template <class T>
+__global__ void adj_final_kernel(SweConsts<T> k, Fields<T> f, const T* U, const T* k2, T dt, int positivity,
+                                 int friction, int model, const T* lam_out, T* lam_k2, T* lam_acc, int64_t n) {
+  int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  int64_t b = t / f.plane, cell = t - b * f.plane;
+  int64_t i = b * 3 * f.plane + cell, P = f.plane;
+  T oh = U[i], ou = U[i + P], ov = U[i + 2 * P];
+  T ch = oh + dt * k2[i], cu = ou + dt * k2[i + P], cv = ov + dt * k2[i + 2 * P];
+  T gh = lam_out[i], gu = lam_out[i + P], gv = lam_out[i + 2 * P];
+  if (friction) {
+    T vh = ch, vu = cu, vv = cv;
+    clean_and_limit(oh, ou, ov, vh, vu, vv, k, positivity != 0);
+    T coef = f.rough[b * f.rough_bstride + cell];
+    friction_bwd(vh, vu, vv, coef, dt, k, model, gh, gu, gv, gh, gu, gv);
+  }
+  T go_h = 0, go_u = 0, go_v = 0, gc_h, gc_u, gc_v;
+  clean_and_limit_bwd(oh, ou, ov, ch, cu, cv, k, positivity != 0, gh, gu, gv, go_h, go_u, go_v, gc_h, gc_u, gc_v);
+  lam_acc[i] = go_h + gc_h;
+  lam_acc[i + P] = go_u + gc_u;
+  lam_acc[i + 2 * P] = go_v + gc_v;
+  lam_k2[i] = dt * gc_h;
+  lam_k2[i + P] = dt * gc_u;
+  lam_k2[i + 2 * P] = dt * gc_v;
+}
+
+// predictor clean-up reversed (solver.py:639)
+template <class T>
+__global__ void adj_mid_kernel(SweConsts<T> k, Fields<T> f, const T* U, const T* k1, T half, int positivity,
+                               const T* lam_P, T* lam_k1, T* lam_acc, int64_t n) {
+  int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  int64_t b = t / f.plane, cell = t - b * f.plane;
+  int64_t i = b * 3 * f.plane + cell, P = f.plane;
+  T oh = U[i], ou = U[i + P], ov = U[i + 2 * P];
+  T ch = oh + half * k1[i], cu = ou + half * k1[i + P], cv = ov + half * k1[i + 2 * P];
+  T go_h = 0, go_u = 0, go_v = 0, gc_h, gc_u, gc_v;
+  clean_and_limit_bwd(oh, ou, ov, ch, cu, cv, k, positivity != 0, lam_P[i], lam_P[i + P], lam_P[i + 2 * P], go_h,
+                      go_u, go_v, gc_h, gc_u, gc_v);
+  lam_acc[i] += go_h + gc_h;
+  lam_acc[i + P] += go_u + gc_u;
+  lam_acc[i + 2 * P] += go_v + gc_v;
+  lam_k1[i] = half * gc_h;
+  lam_k1[i + P] = half * gc_u;
+  lam_k1[i + 2 * P] = half * gc_v;
+}
+
+}  // namespace swe
+
+// ===================================================================== host side
+using namespace swe;
+
+namespace {
+
+thread_local char g_msg[256] = {0};
+
+int afail(int code, const char* msg) {
+  strncpy(g_msg, msg, sizeof(g_msg) - 1);
+  return code;
+}
+
+inline size_t al256(size_t v) { return (v + 255) & ~(size_t)255; }
+
+template <class T> struct AdjWs {
+  T *lam_k2, *lam_acc, *lam_P, *lam_k1, *ghost;
+  int lmax;
+};
+
+template <class T> size_t adj_ws_bytes(const swe_grid* g) {
+  size_t state = (size_t)g->batch * 3 * (size_t)g->ny * g->nx * sizeof(T);
+  int lmax = g->ny > g->nx ? g->ny : g->nx;
+  size_t ghost = (size_t)g->batch * 4 * 2 * lmax * 3 * sizeof(T);
+  return 4 * al256(state) + al256(ghost);
+}
+
+template <class T> AdjWs<T> adj_ws(void* ws, const swe_grid* g) {
+  size_t state = al256((size_t)g->batch * 3 * (size_t)g->ny * g->nx * sizeof(T));
+  char* p = (char*)ws;
+  AdjWs<T> w;
+  w.lam_k2 = (T*)p;
+  w.lam_acc = (T*)(p + state);
+  w.lam_P = (T*)(p + 2 * state);
+  w.lam_k1 = (T*)(p + 3 * state);
+  w.ghost = (T*)(p + 4 * state);
+  w.lmax = g->ny > g->nx ? g->ny : g->nx;
+  return w;
+}
+
+template <class T> SweConsts<T> adj_consts(const swe_config* c, const swe_grid* g) {
+  const bool f32 = sizeof(T) == 4;
+  const double prec = f32 ? 1e-6 : 1e-10;
+  auto mx = [](double a, double b) { return a > b ? a : b; };
+  SweConsts<T> k;
+  k.g = (T)c->gravity;
+  k.half_g = (T)(0.5 * c->gravity);
+  k.dry = (T)c->dry_depth;
+  k.eps = (T)c->epsilon;
+  k.floor = (T)mx(mx(c->epsilon, c->dry_depth), prec);
+  k.hll_floor = (T)mx(c->epsilon, prec);
+  double ff = mx(c->epsilon, prec);
+  k.fric_floor = (T)ff;
+  k.fric_floor2 = (T)(ff * ff);
+  k.theta = (T)c->limiter_theta;
+  k.cfl = (T)c->cfl;
+  k.dx = (T)g->dx;
+  k.dy = (T)g->dy;
+  k.cval = (T)c->constant_value;
+  k.manning_g = (T)9.81;
+  k.third = (T)(1.0 / 3.0);
+  return k;
+}
+
+int adj_rows(const swe_grid* g, int hint) {
+  if (hint > 0) return hint;
+  int strips = (g->nx + 4 + kOutA - 1) / kOutA;
+  long long per = (long long)strips * g->batch;
+  int S = (int)((long long)(g->ny + 4) * per / 4096);
+  if (S < 8) S = 8;
+  if (S > 64) S = 64;
+  return S;
+}
+
+template <class T>
+int step_adjoint_impl(const swe_grid* g, const swe_config* c, const T* U, const T* k1, const T* k2, const T* bed,
+                      const T* rough, const T* level, int level_stride, double dt_in, const T* lam_out, T* lam_U,
+                      T* lam_level, void* ws, int rows_hint, void* stream) {
+  if (!g || !c || !U || !k1 || !k2 || !bed || !lam_out || !lam_U || !ws) return afail(SWE_ERR_ARG, "null argument");
+  if (g->batch <= 0 || g->ny < 2 || g->nx < 2) return afail(SWE_ERR_SHAPE, "bad grid");
+  if (c->apply_friction && !rough) return afail(SWE_ERR_ARG, "roughness required");
+  for (int s = 0; s < 4; ++s)
+    if ((c->boundary[s] == BC_WATER_LEVEL || c->boundary[s] == BC_ABSORBING) && !level)
+      return afail(SWE_ERR_ARG, "boundary level pointer is required");
+  hipStream_t s = (hipStream_t)stream;
+  AdjWs<T> w = adj_ws<T>(ws, g);
+  AdjArgs<T> a;
+  memset(&a, 0, sizeof(a));
+  a.k = adj_consts<T>(c, g);
+  a.f.B = g->batch;
+  a.f.ny = g->ny;
+  a.f.nx = g->nx;
+  a.f.plane = (int64_t)g->ny * g->nx;
+  a.f.bed_bstride = g->bed_batch == 1 ? 0 : a.f.plane;
+  a.f.rough_bstride = g->rough_batch == 1 ? 0 : a.f.plane;
+  for (int q = 0; q < 4; ++q) a.f.bc[q] = c->boundary[q];
+  a.f.bed = bed;
+  a.f.rough = rough;
+  a.f.level = level;
+  a.f.level_stride = level_stride;
+  a.S = adj_rows(g, rows_hint);
+  a.strips = (g->nx + 4 + kOutA - 1) / kOutA;
+  a.segs = (g->ny + 4 + a.S - 1) / a.S;
+  a.wsr = c->water_slope_reset;
+  a.positivity = c->positivity_limiter;
+  a.U = U;
+  a.k1 = k1;
+  T dt = (T)dt_in;
+  T half = T(0.5) * dt;
+  a.half_dt = half;
+  a.ghost = w.ghost;
+  a.lmax = w.lmax;
+  int64_t n = (int64_t)g->batch * a.f.plane;
+  dim3 eb(256), eg((unsigned)((n + 255) / 256));
+  long long waves = (long long)a.strips * a.segs;
+  dim3 lg((unsigned)((waves + kWavesPerBlock - 1) / kWavesPerBlock), (unsigned)g->batch);
+  dim3 lb(kWave * kWavesPerBlock);
+
+  hipLaunchKernelGGL((adj_final_kernel<T>), eg, eb, 0, s, a.k, a.f, U, k2, dt, c->positivity_limiter,
+                     c->apply_friction, c->friction_model, lam_out, w.lam_k2, w.lam_acc, n);
+  AdjArgs<T> a2 = a;
+  a2.lam = w.lam_k2;
+  a2.base = nullptr;
+  a2.out = w.lam_P;
+  hipLaunchKernelGGL((adj_L_kernel<T, 2>), lg, lb, 0, s, a2);
+  hipLaunchKernelGGL((adj_ghost_kernel<T>), dim3(g->batch), dim3(1024), 0, s, a2, 2, lam_level);
+  hipLaunchKernelGGL((adj_mid_kernel<T>), eg, eb, 0, s, a.k, a.f, U, k1, half, c->positivity_limiter, w.lam_P,
+                     w.lam_k1, w.lam_acc, n);
+  AdjArgs<T> a1 = a;
+  a1.lam = w.lam_k1;
+  a1.base = w.lam_acc;
+  a1.out = lam_U;
+  hipLaunchKernelGGL((adj_L_kernel<T, 1>), lg, lb, 0, s, a1);
+  hipLaunchKernelGGL((adj_ghost_kernel<T>), dim3(g->batch), dim3(1024), 0, s, a1, 1, lam_level);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return afail(SWE_ERR_HIP, hipGetErrorString(e));
+  return 0;
+}
+
+template <class T>
+int operator_adjoint_impl(const swe_grid* g, const swe_config* c, const T* U, const T* bed, const T* level,
+                          int level_stride, const T* lam_rhs, T* lam_U, T* lam_level, void* ws, int rows_hint,
+                          void* stream) {
+  if (!g || !c || !U || !bed || !lam_rhs || !lam_U || !ws) return afail(SWE_ERR_ARG, "null argument");
+  if (g->batch <= 0 || g->ny < 2 || g->nx < 2) return afail(SWE_ERR_SHAPE, "bad grid");
+  hipStream_t s = (hipStream_t)stream;
+  AdjWs<T> w = adj_ws<T>(ws, g);
+  AdjArgs<T> a;
+  memset(&a, 0, sizeof(a));
+  a.k = adj_consts<T>(c, g);
+  a.f.B = g->batch;
+  a.f.ny = g->ny;
+  a.f.nx = g->nx;
+  a.f.plane = (int64_t)g->ny * g->nx;
+  a.f.bed_bstride = g->bed_batch == 1 ? 0 : a.f.plane;
+  a.f.rough_bstride = 0;
+  for (int q = 0; q < 4; ++q) a.f.bc[q] = c->boundary[q];
+  a.f.bed = bed;
+  a.f.rough = bed;
+  a.f.level = level;
+  a.f.level_stride = level_stride;
+  a.S = adj_rows(g, rows_hint);
+  a.strips = (g->nx + 4 + kOutA - 1) / kOutA;
+  a.segs = (g->ny + 4 + a.S - 1) / a.S;
+  a.wsr = c->water_slope_reset;
+  a.positivity = c->positivity_limiter;
+  a.U = U;
+  a.k1 = U;
+  a.lam = lam_rhs;
+  a.out = lam_U;
+  a.ghost = w.ghost;
+  a.lmax = w.lmax;
+  long long waves = (long long)a.strips * a.segs;
+  dim3 lg((unsigned)((waves + kWavesPerBlock - 1) / kWavesPerBlock), (unsigned)g->batch);
+  hipLaunchKernelGGL((adj_L_kernel<T, 1>), lg, dim3(kWave * kWavesPerBlock), 0, s, a);
+  hipLaunchKernelGGL((adj_ghost_kernel<T>), dim3(g->batch), dim3(1024), 0, s, a, 1, lam_level);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return afail(SWE_ERR_HIP, hipGetErrorString(e));
+  return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+size_t swe_adjoint_workspace_bytes(const swe_grid* g, int dtype_bytes) {
+  if (!g || g->batch <= 0 || g->ny < 2 || g->nx < 2) return 0;
+  return dtype_bytes == 8 ? adj_ws_bytes<double>(g) : adj_ws_bytes<float>(g);
+}
+
+const char* swe_adjoint_last_error_message(void) { return g_msg; }
+
+int swe_operator_adjoint_f32(const swe_grid* g, const swe_config* c, const float* U, const float* bed,
+                             const float* level, int level_stride, const float* lam_rhs, float* lam_U,
+                             float* lam_level, void* ws, int rows_hint, void* stream) {
+  return operator_adjoint_impl<float>(g, c, U, bed, level, level_stride, lam_rhs, lam_U, lam_level, ws, rows_hint,
+                                      stream);
+}
+
+int swe_operator_adjoint_f64(const swe_grid* g, const swe_config* c, const double* U, const double* bed,
+                             const double* level, int level_stride, const double* lam_rhs, double* lam_U,
+                             double* lam_level, void* ws, int rows_hint, void* stream) {
+  return operator_adjoint_impl<double>(g, c, U, bed, level, level_stride, lam_rhs, lam_U, lam_level, ws, rows_hint,
+                                       stream);
+}
+
+int swe_step_adjoint_f32(const swe_grid* g, const swe_config* c, const float* U, const float* k1, const float* k2,
+                         const float* bed, const float* rough, const float* level, int level_stride, double dt,
+                         const float* lam_out, float* lam_U, float* lam_level, void* ws, int rows_hint,
+                         void* stream) {
+  return step_adjoint_impl<float>(g, c, U, k1, k2, bed, rough, level, level_stride, dt, lam_out, lam_U, lam_level,
+                                  ws, rows_hint, stream);
+}
+
+int swe_step_adjoint_f64(const swe_grid* g, const swe_config* c, const double* U, const double* k1,
+                         const double* k2, const double* bed, const double* rough, const double* level,
+                         int level_stride, double dt, const double* lam_out, double* lam_U, double* lam_level,
+                         void* ws, int rows_hint, void* stream) {
+  return step_adjoint_impl<double>(g, c, U, k1, k2, bed, rough, level, level_stride, dt, lam_out, lam_U, lam_level,
+                                   ws, rows_hint, stream);
+}
+
+}  // extern "C"

@@ -18,6 +18,7 @@
 #include <string.h>
 
 #include "swe_common.h"
+#include "swe_fetch.h"
 #include "../../include/swe2d_hip.h"
 
 namespace swe {
@@ -47,6 +48,7 @@ template <class T> struct StageArgs {
   const T* k1;           // stage 2: first-stage tendency
   T* out;                // stage 1: tendency; stage 2: new state
   T* hmax;               // stage 2, optional running max of depth [B,ny,nx]
+  T* k2_out;             // stage 2, optional record of the second-stage tendency
   const T* rain;         // optional rain rate field [B or 1, ny, nx]
   const T* infil;        // optional infiltration field
   int64_t rain_bstride, infil_bstride;
@@ -59,25 +61,6 @@ template <class T> struct StageArgs {
   double max_dt;         // host-driven steps: explicit max_dt
   T* dt_series;          // optional per-step dt record
   int record_wave;       // stage 1: accumulate wave speeds
-};
-
-template <class T> __device__ inline T shup(T v) { return __shfl_up(v, 1, kWave); }
-template <class T> __device__ inline T shdn(T v) { return __shfl_down(v, 1, kWave); }
-
-template <class T> __device__ inline Cell<T> shup(const Cell<T>& c) {
-  return Cell<T>{shup(c.h), shup(c.un), shup(c.ut), shup(c.eta), shup(c.z)};
-}
-template <class T> __device__ inline Cell<T> shdn(const Cell<T>& c) {
-  return Cell<T>{shdn(c.h), shdn(c.un), shdn(c.ut), shdn(c.eta), shdn(c.z)};
-}
-template <class T> __device__ inline Side<T> shdn(const Side<T>& s) {
-  return Side<T>{shdn(s.hlin), shdn(s.eta), shdn(s.z), shdn(s.un), shdn(s.ut)};
-}
-
-template <class T> struct Prim {
-  T h, u, v, eta, z;
-  __device__ Cell<T> x() const { return Cell<T>{h, u, v, eta, z}; }
-  __device__ Cell<T> y() const { return Cell<T>{h, v, u, eta, z}; }
 };
 
 // dt of the step from the accumulated maxima (solver.py:618-632):
@@ -96,50 +79,6 @@ template <class T> __device__ inline T step_dt(const typename Bits<T>::U* acc, c
 }
 
 template <class T> __device__ inline bool dt_ok(T dt) { return isfinite(dt) && dt > T(0); }
-
-// state of interior cell (r, c) as stage STAGE sees it
-template <class T, int STAGE>
-__device__ inline void load_state(const StageArgs<T>& a, int64_t base, int r, int c, T half_dt, T& h, T& hu,
-                                  T& hv) {
-  int64_t i = base + (int64_t)r * a.f.nx + c;
-  int64_t P = a.f.plane;
-  h = a.U[i];
-  hu = a.U[i + P];
-  hv = a.U[i + 2 * P];
-  if (STAGE == 2) {
-    T ch = h + half_dt * a.k1[i];
-    T cu = hu + half_dt * a.k1[i + P];
-    T cv = hv + half_dt * a.k1[i + 2 * P];
-    clean_and_limit(h, hu, hv, ch, cu, cv, a.k, a.positivity != 0);
-    h = ch;
-    hu = cu;
-    hv = cv;
-  }
-}
-
-template <class T, int STAGE>
-__device__ inline Prim<T> fetch(const StageArgs<T>& a, int b, int r, int c, T half_dt) {
-  const Fields<T>& f = a.f;
-  GhostMap g = ghost_map(r, c, f.ny, f.nx, f.bc);
-  int64_t base = (int64_t)b * 3 * f.plane;
-  T h, hu, hv;
-  load_state<T, STAGE>(a, base, g.r, g.c, half_dt, h, hu, hv);
-  T z = f.bed[(int64_t)b * f.bed_bstride + (int64_t)g.r * f.nx + g.c];
-  if (g.kind >= 0) {
-    T mirror = T(0);
-    if (g.kind == BC_WALL) {
-      T mh, mu, mv;
-      load_state<T, STAGE>(a, base, g.mr, g.mc, half_dt, mh, mu, mv);
-      mirror = (g.side == LEFT || g.side == RIGHT) ? mu : mv;
-    }
-    T level = (g.kind == BC_WATER_LEVEL || g.kind == BC_ABSORBING) ? f.level[b * f.level_stride] : T(0);
-    apply_ghost(g, h, hu, hv, mirror, z, level, a.k);
-  }
-  Prim<T> p;
-  primitive_cell(h, hu, hv, z, a.k, p.h, p.u, p.v, p.eta);
-  p.z = z;
-  return p;
-}
 
 template <class T> __device__ inline T wave_max(T v) {
   for (int off = 32; off > 0; off >>= 1) {
@@ -262,6 +201,11 @@ __global__ __launch_bounds__(256) void stage_kernel(StageArgs<T> a) {
               a.out[i + f.plane] = tu;
               a.out[i + 2 * f.plane] = tv;
             } else {
+              if (a.k2_out != nullptr) {
+                a.k2_out[i] = th;
+                a.k2_out[i + f.plane] = tu;
+                a.k2_out[i + 2 * f.plane] = tv;
+              }
               T oh = a.U[i], ou = a.U[i + f.plane], ov = a.U[i + 2 * f.plane];
               T nh = oh + dt * th, nu = ou + dt * tu, nv = ov + dt * tv;
               clean_and_limit(oh, ou, ov, nh, nu, nv, k, a.positivity != 0);
@@ -512,7 +456,7 @@ int spatial_operator_impl(const swe_grid* g, const swe_config* c, const T* U, co
 template <class T>
 int step_impl(const swe_grid* g, const swe_config* c, int stages, const T* U, const T* bed, const T* rough,
               const T* level, int level_stride, double max_dt, int parity, T* U_out, void* ws, int rows_hint,
-              void* stream) {
+              void* stream, T* k1_rec = nullptr, T* k2_rec = nullptr) {
   int e;
   if ((e = check_grid(g)) || (e = check_cfg(c)) || (e = check_level(level, level_stride, g, c))) return e;
   if (stages < 1 || stages > 3) return fail(SWE_ERR_ARG, "stages must be 1, 2 or 3");
@@ -523,8 +467,9 @@ int step_impl(const swe_grid* g, const swe_config* c, int stages, const T* U, co
   WsView<T> w = ws_view<T>(ws);
   int p = parity & 1;
   StageArgs<T> a = make_args<T>(g, c, bed, rough, level, level_stride, rows_hint);
+  T* k1buf = k1_rec != nullptr ? k1_rec : w.k1;
   a.U = U;
-  a.out = w.k1;
+  a.out = k1buf;
   a.acc = w.acc + p * 2 * kSlots;
   a.acc_next = w.acc + (1 - p) * 2 * kSlots;
   a.record_wave = 1;
@@ -534,7 +479,8 @@ int step_impl(const swe_grid* g, const swe_config* c, int stages, const T* U, co
   if ((stages & 1) && (e = launch_stage<T, 1>(a, s))) return e;
   if (stages & 2) {
     StageArgs<T> b2 = a;
-    b2.k1 = w.k1;
+    b2.k1 = k1buf;
+    b2.k2_out = k2_rec;
     b2.out = U_out;
     b2.record_wave = 0;
     if ((e = launch_stage<T, 2>(b2, s))) return e;
@@ -653,6 +599,22 @@ int swe_step_f64(const swe_grid* g, const swe_config* c, const double* U, const 
                  int rows_hint, void* stream) {
   return step_impl<double>(g, c, 3, U, bed, rough, level, level_stride, max_dt, parity, U_out, ws, rows_hint,
                            stream);
+}
+
+int swe_step_record_f32(const swe_grid* g, const swe_config* c, const float* U, const float* bed,
+                        const float* rough, const float* level, int level_stride, double max_dt, int parity,
+                        float* U_out, float* k1_out, float* k2_out, void* ws, int rows_hint, void* stream) {
+  if (!k1_out || !k2_out) return fail(SWE_ERR_ARG, "k1_out and k2_out are required");
+  return step_impl<float>(g, c, 3, U, bed, rough, level, level_stride, max_dt, parity, U_out, ws, rows_hint,
+                          stream, k1_out, k2_out);
+}
+
+int swe_step_record_f64(const swe_grid* g, const swe_config* c, const double* U, const double* bed,
+                        const double* rough, const double* level, int level_stride, double max_dt, int parity,
+                        double* U_out, double* k1_out, double* k2_out, void* ws, int rows_hint, void* stream) {
+  if (!k1_out || !k2_out) return fail(SWE_ERR_ARG, "k1_out and k2_out are required");
+  return step_impl<double>(g, c, 3, U, bed, rough, level, level_stride, max_dt, parity, U_out, ws, rows_hint,
+                           stream, k1_out, k2_out);
 }
 
 int swe_step_stage_f32(const swe_grid* g, const swe_config* c, int stage, const float* U, const float* bed,

@@ -85,6 +85,7 @@ class SWE2D(nn.Module):
         self._ws = {}
         self._field_cache = {}
         self.rows_hint = 0  # wavefront rows per strip; 0 = library heuristic / autotuned value
+        self.adjoint_checkpoint_every = 64  # K: forward checkpoints for the adjoint replay
 
     # ------------------------------------------------------------------ properties
     @property
@@ -233,10 +234,22 @@ class SWE2D(nn.Module):
                          boundary_level=None):
         """Tendency dU/dt (src/solver.py:448-531); wave speeds are maxima over the batch."""
         self._check_state(U)
-        U = U.contiguous()
+        level = self._prepare_boundary_level(
+            self.cfg.constant_level if boundary_level is None else boundary_level, U)
+        if (torch.is_grad_enabled() and (U.requires_grad or level.requires_grad) and rainfall_rate is None
+                and infiltration is None):
+            from .adjoint import operator_adjoint
+            rhs = operator_adjoint(self, U, level)
+            if return_wave_speed:
+                with torch.no_grad():
+                    _, ax, ay = self.spatial_operator(U.detach(), return_wave_speed=True,
+                                                      boundary_level=level.detach())
+                return rhs, ax, ay
+            return rhs
+        U = U.detach().contiguous()
         bed, rough = self._fields(U)
         ws = self._workspace(U, bed, rough)
-        lvl, stride = self._level_arg(self.cfg.constant_level if boundary_level is None else boundary_level, U)
+        lvl, stride = self._level_arg(level, U)
         lib = _lib.load()
         rhs = torch.empty_like(U)
         B = U.shape[0]

@@ -118,6 +118,15 @@ int swe_step_stage_f32(const swe_grid* grid, const swe_config* cfg, int stage, c
                        const float* roughness, const float* level, int level_stride, double max_dt, int parity,
                        float* U_out, void* ws, int rows_hint, void* stream);
 
+/* swe_step_*, additionally recording the two stage tendencies k1 and k2
+ * (each [batch, 3, ny, nx]) for the adjoint's checkpoint replay. */
+int swe_step_record_f32(const swe_grid* grid, const swe_config* cfg, const float* U, const float* bed,
+                        const float* roughness, const float* level, int level_stride, double max_dt, int parity,
+                        float* U_out, float* k1_out, float* k2_out, void* ws, int rows_hint, void* stream);
+int swe_step_record_f64(const swe_grid* grid, const swe_config* cfg, const double* U, const double* bed,
+                        const double* roughness, const double* level, int level_stride, double max_dt, int parity,
+                        double* U_out, double* k1_out, double* k2_out, void* ws, int rows_hint, void* stream);
+
 /* Device-driven integration of a duration -- replaces the SWE2D.forward loop
  * (solver.py:719-779) without per-step host synchronisation.  swe_run_begin
  * arms the device clock; each swe_advance call launches n_steps further steps
@@ -136,6 +145,35 @@ int swe_advance_f64(const swe_grid* grid, const swe_config* cfg, const double* b
                     int n_steps, double* hmax, double* dt_series, int dt_capacity, const int32_t* gauges,
                     int n_gauges, double* gauge_depth, double* gauge_level, int gauge_capacity, void* ws,
                     int rows_hint, void* stream);
+
+/* ---- adjoint (reverse mode) --------------------------------------------
+ * The reference differentiates step() with torch autograd (dt detached,
+ * solver.py:625-626; caller bc_inversion_train.py:332-452).  These entry
+ * points are its hand-written replacement.  lam_level receives, per member,
+ * the gradient with respect to the water-level boundary value and is
+ * ACCUMULATED into (+=), so a reverse sweep over many steps sums it. */
+size_t swe_adjoint_workspace_bytes(const swe_grid* grid, int dtype_bytes);
+const char* swe_adjoint_last_error_message(void);
+
+/* L^T: adjoint of swe_spatial_operator with respect to U (and the level). */
+int swe_operator_adjoint_f32(const swe_grid* grid, const swe_config* cfg, const float* U, const float* bed,
+                             const float* level, int level_stride, const float* lam_rhs, float* lam_U,
+                             float* lam_level, void* adj_ws, int rows_hint, void* stream);
+int swe_operator_adjoint_f64(const swe_grid* grid, const swe_config* cfg, const double* U, const double* bed,
+                             const double* level, int level_stride, const double* lam_rhs, double* lam_U,
+                             double* lam_level, void* adj_ws, int rows_hint, void* stream);
+
+/* Adjoint of one step taken with dt: given the step-start state U, the
+ * recorded k1, k2 and the adjoint lam_out of the step's output, writes the
+ * adjoint lam_U of U (lam_U must not alias the other arrays). */
+int swe_step_adjoint_f32(const swe_grid* grid, const swe_config* cfg, const float* U, const float* k1,
+                         const float* k2, const float* bed, const float* roughness, const float* level,
+                         int level_stride, double dt, const float* lam_out, float* lam_U, float* lam_level,
+                         void* adj_ws, int rows_hint, void* stream);
+int swe_step_adjoint_f64(const swe_grid* grid, const swe_config* cfg, const double* U, const double* k1,
+                         const double* k2, const double* bed, const double* roughness, const double* level,
+                         int level_stride, double dt, const double* lam_out, double* lam_U, double* lam_level,
+                         void* adj_ws, int rows_hint, void* stream);
 
 #ifdef __cplusplus
 }
