@@ -44,16 +44,36 @@ import torch.nn.functional as F
 NG = 2
 BC_TYPES = ("transmissive", "periodic", "constant", "water_level", "wall", "absorbing")
 
-# torch's CPU float32 sqrt is not correctly rounded (about 0.7 % of inputs land
-# one ulp away from the IEEE result on this build).  SQRT_MODE "torch" keeps the
-# reference's own rounding; "ieee" rounds a float64 square root once, which is
-# the correctly rounded float32 result a GPU (and IEEE hardware sqrt) gives.
+# torch's CPU sqrt is not correctly rounded on this build (about 0.7 % of
+# float32 and 0.9 % of float64 inputs land one ulp away from the IEEE result).
+# SQRT_MODE "torch" keeps the reference's own rounding; "ieee" gives the
+# correctly rounded square root that IEEE hardware (and the GPU) computes:
+# float32 via one rounding of the float64 root, float64 via numpy's hardware
+# sqrt, both with the sqrt derivative grad / (2 * result) for autograd.
 SQRT_MODE = "torch"
 
 
+class _IeeeSqrt(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, x):
+        if x.dtype == torch.float32:
+            y = torch.sqrt(x.double()).float()
+        else:
+            import numpy as np
+
+            y = torch.from_numpy(np.sqrt(x.detach().cpu().numpy())).to(x.device)
+        ctx.save_for_backward(y)
+        return y
+
+    @staticmethod
+    def backward(ctx, g):
+        (y,) = ctx.saved_tensors
+        return g / (2 * y)
+
+
 def _sqrt(x):
-    if SQRT_MODE == "ieee" and x.dtype == torch.float32:
-        return torch.sqrt(x.double()).float()
+    if SQRT_MODE == "ieee":
+        return _IeeeSqrt.apply(x)
     return torch.sqrt(x)
 
 
@@ -430,12 +450,20 @@ def clean_and_limit(old, cand, cfg):
 
 
 def step(U, bed, roughness, cfg, dx, dy, max_dt, level=None, rain_start=None, rain_half=None,
-         infiltration=None):
-    """One adaptive midpoint step (solver.py:572-663); returns (U_new, float dt)."""
+         infiltration=None, fixed_dt=None):
+    """One adaptive midpoint step (solver.py:572-663); returns (U_new, float dt).
+
+    ``fixed_dt`` (a test device, not in the reference) bypasses the CFL choice
+    and uses that dt, so finite differences can be taken with the dt series
+    frozen, matching the reference's detached dt.
+    """
     max_dt = max_dt.to(dtype=U.dtype) if torch.is_tensor(max_dt) else U.new_tensor(max_dt)
     lvl = as_level(cfg.constant_level if level is None else level, U)
     k1, ax, ay = spatial_operator(U, bed, cfg, dx, dy, lvl, rain_start, infiltration, wave_speed=True)
-    dt = torch.minimum(cfl_dt(ax, ay, cfg, dx, dy), max_dt).detach()
+    if fixed_dt is not None:
+        dt = U.new_tensor(fixed_dt)
+    else:
+        dt = torch.minimum(cfl_dt(ax, ay, cfg, dx, dy), max_dt).detach()
     if not torch.isfinite(dt):
         raise FloatingPointError("The calculated timestep is non-finite.")
     dt_value = float(dt)
