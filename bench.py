@@ -41,6 +41,8 @@ def parse():
     ap.add_argument("--cpu-side", type=int, default=2048, help="crop side of the CPU-baseline sample")
     ap.add_argument("--cpu-steps", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--adjoint-steps", type=int, default=10,
+                    help="steps of the adjoint leg (replay with k1/k2 record, then reverse); 0 to skip")
     ap.add_argument("--copy-probe", action="store_true",
                     help="also run a 1 GiB device copy (calibrates rocprof FETCH_SIZE / WRITE_SIZE)")
     return ap.parse_args()
@@ -97,6 +99,42 @@ def time_kernels(model, U, steps, rows):
     s1 = sum(e[0].elapsed_time(e[1]) for e in evs) / steps
     s2 = sum(e[1].elapsed_time(e[2]) for e in evs) / steps
     return s1, s2
+
+
+def time_adjoint(model, U, steps):
+    """Per-step cost of the reverse sweep: replay with k1/k2 recorded, then the step adjoint."""
+    import torch
+
+    from diffswe2d_amd import adjoint
+
+    lvl = U.new_tensor(0.0)
+    tape, dts = [], []
+    Ucur = U.clone()
+    for _ in range(2):  # warm-up of both kernels
+        o, k1, k2, dt = adjoint.record_step(model, Ucur, lvl, 1e30)
+        lam_level = torch.zeros(U.shape[0], dtype=U.dtype, device=U.device)
+        adjoint.step_adjoint(model, Ucur, k1, k2, lvl, dt, torch.ones_like(U), lam_level)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        o, k1, k2, dt = adjoint.record_step(model, Ucur, lvl, 1e30)
+        tape.append((Ucur, k1, k2))
+        dts.append(dt)
+        Ucur = o
+    torch.cuda.synchronize()
+    t_rec = (time.perf_counter() - t0) / steps
+    lam = torch.ones_like(U)
+    lam_level = torch.zeros(U.shape[0], dtype=U.dtype, device=U.device)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for k in range(steps - 1, -1, -1):
+        Uk, k1, k2 = tape[k]
+        lam = adjoint.step_adjoint(model, Uk, k1, k2, lvl, dts[k], lam, lam_level)
+    torch.cuda.synchronize()
+    t_adj = (time.perf_counter() - t0) / steps
+    if not bool(torch.isfinite(lam).all()):
+        raise RuntimeError("adjoint produced non-finite values")
+    return t_rec * 1e3, t_adj * 1e3
 
 
 def cpu_baseline(sc, cfg, side, steps):
@@ -192,6 +230,17 @@ def main():
         torch.cuda.synchronize()
         print("# copy probe: 3 x 1 GiB device copies", file=sys.stderr, flush=True)
 
+    adj = None
+    if a.adjoint_steps > 0:
+        rec_ms, adj_ms = time_adjoint(model, U, a.adjoint_steps)
+        back_ms = rec_ms + adj_ms  # backward with K-step checkpoints = replay (with record) + reverse sweep
+        adj = {"value": round(cells / (adj_ms * 1e-3) / 1e9, 4), "unit": "Gcell-updates/s",
+               "reverse_ms_per_step": round(adj_ms, 4), "replay_record_ms_per_step": round(rec_ms, 4),
+               "backward_ms_per_step": round(back_ms, 4),
+               "backward_over_forward": round(back_ms / ms, 3), "steps": a.adjoint_steps}
+        print(f"# adjoint: replay {rec_ms:.3f} ms + reverse {adj_ms:.3f} ms per step "
+              f"({back_ms / ms:.2f}x forward)", file=sys.stderr, flush=True)
+
     # HBM bytes per step from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE runs of this workload
     traffic, traffic_src = None, None
     tfile = ROOT / "profiles" / "basin8192_fp32_traffic.json"
@@ -219,6 +268,7 @@ def main():
                      "kernel": "step (stage1 + stage2)", "algorithmic_bytes_per_cell_update": 32,
                      "kernel_ms": round(step_kernel_ms, 4)},
         "kernel_ms": {"stage1": round(s1, 4), "stage2": round(s2, 4)},
+        "adjoint": adj,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

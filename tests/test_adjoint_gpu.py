@@ -198,8 +198,10 @@ def test_spline_gradient_central_differences(golden, cuda_device, ieee_oracle):
         obs.append(s[0, 0, prob._iy, prob._ix] + prob._bed_g)
     loss = prob.misfit(torch.stack(obs), target)
     loss.backward()
+    # 64 controls over 10 samples: only the controls whose basis functions reach a sample move the loss
+    active = np.flatnonzero(basis.abs().sum(0).cpu().numpy() > 0)
     rng = np.random.default_rng(4)
-    idx = rng.choice(len(ctrl0), size=8, replace=False)
+    idx = rng.choice(active, size=min(8, len(active)), replace=False)
     h = 1e-6
     basis_c, target_c, c0 = basis.cpu(), target.cpu(), ctrl0.cpu()
     for i in idx:
@@ -209,5 +211,5 @@ def test_spline_gradient_central_differences(golden, cuda_device, ieee_oracle):
         fd = (oracle_gauge_loss(d, basis_c, target_c, cp, dts_per_chunk)
               - oracle_gauge_loss(d, basis_c, target_c, cm, dts_per_chunk)) / (2 * h)
         ad = float(ctrl.grad[i])
-        print(f"control {i}: adjoint {ad:.8e}, central FD (dt frozen) {fd:.8e}, rel {abs(ad - fd) / abs(fd):.2e}")
+        print(f"control {i}: adjoint {ad:.8e}, central FD (dt frozen) {fd:.8e}, rel {abs(ad - fd) / max(abs(fd), 1e-300):.2e}")
         assert abs(ad - fd) <= 1e-4 * max(abs(fd), 1e-3 * float(ctrl.grad.abs().max()))
