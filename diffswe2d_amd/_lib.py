@@ -36,6 +36,14 @@ class SweConfigC(ctypes.Structure):
                 ("apply_friction", ctypes.c_int32), ("friction_model", ctypes.c_int32)]
 
 
+class SweScenario(ctypes.Structure):
+    _fields_ = [("ny", ctypes.c_int32), ("nx", ctypes.c_int32), ("dx", ctypes.c_double), ("dy", ctypes.c_double),
+                ("U_a", ctypes.c_void_p), ("U_b", ctypes.c_void_p), ("k1", ctypes.c_void_p),
+                ("bed", ctypes.c_void_p), ("roughness", ctypes.c_void_p), ("levels", ctypes.c_void_p),
+                ("n_chunks", ctypes.c_int32), ("chunk_seconds", ctypes.c_double), ("gauges", ctypes.c_void_p),
+                ("n_gauges", ctypes.c_int32), ("gauge_out", ctypes.c_void_p), ("rows_hint", ctypes.c_int32)]
+
+
 class SweControl(ctypes.Structure):
     _fields_ = [("elapsed", ctypes.c_double * 2), ("duration", ctypes.c_double), ("tol", ctypes.c_double),
                 ("cap", ctypes.c_double), ("bad_dt", ctypes.c_double), ("done", ctypes.c_int32),
@@ -72,6 +80,12 @@ _SIGNATURES = {
     "swe_operator_adjoint_f64": ([_G, _C, _P, _P, _P, _I, _P, _P, _P, _P, _I, _P], _I),
     "swe_step_adjoint_f32": ([_G, _C, _P, _P, _P, _P, _P, _P, _I, _D, _P, _P, _P, _P, _I, _P], _I),
     "swe_step_adjoint_f64": ([_G, _C, _P, _P, _P, _P, _P, _P, _I, _D, _P, _P, _P, _P, _I, _P], _I),
+    "swe_ensemble_workspace_bytes": ([_I], ctypes.c_size_t),
+    "swe_ensemble_last_error_message": ([], ctypes.c_char_p),
+    "swe_ensemble_init": ([_C, _I, _P, _P, ctypes.c_size_t, ctypes.POINTER(ctypes.c_longlong), _P], _I),
+    "swe_ensemble_advance": ([_P, _I, ctypes.c_longlong, _I, _I, _P], _I),
+    "swe_ensemble_clock_offset": ([_I], ctypes.c_size_t),
+    "swe_ensemble_clock_bytes": ([], ctypes.c_size_t),
 }
 
 _lib = None

@@ -22,6 +22,7 @@
 
 #include "swe_adjoint_math.h"
 #include "swe_common.h"
+#include "swe_host.h"
 #include "swe_fetch.h"
 #include "../../include/swe2d_hip.h"
 
@@ -372,29 +373,6 @@ template <class T> AdjWs<T> adj_ws(void* ws, const swe_grid* g) {
   return w;
 }
 
-template <class T> SweConsts<T> adj_consts(const swe_config* c, const swe_grid* g) {
-  const bool f32 = sizeof(T) == 4;
-  const double prec = f32 ? 1e-6 : 1e-10;
-  auto mx = [](double a, double b) { return a > b ? a : b; };
-  SweConsts<T> k;
-  k.g = (T)c->gravity;
-  k.half_g = (T)(0.5 * c->gravity);
-  k.dry = (T)c->dry_depth;
-  k.eps = (T)c->epsilon;
-  k.floor = (T)mx(mx(c->epsilon, c->dry_depth), prec);
-  k.hll_floor = (T)mx(c->epsilon, prec);
-  double ff = mx(c->epsilon, prec);
-  k.fric_floor = (T)ff;
-  k.fric_floor2 = (T)(ff * ff);
-  k.theta = (T)c->limiter_theta;
-  k.cfl = (T)c->cfl;
-  k.dx = (T)g->dx;
-  k.dy = (T)g->dy;
-  k.cval = (T)c->constant_value;
-  k.manning_g = (T)9.81;
-  k.third = (T)(1.0 / 3.0);
-  return k;
-}
 
 int adj_rows(const swe_grid* g, int hint) {
   if (hint > 0) return hint;
@@ -420,7 +398,7 @@ int step_adjoint_impl(const swe_grid* g, const swe_config* c, const T* U, const 
   AdjWs<T> w = adj_ws<T>(ws, g);
   AdjArgs<T> a;
   memset(&a, 0, sizeof(a));
-  a.k = adj_consts<T>(c, g);
+  a.k = host_consts<T>(c, g->dx, g->dy);
   a.f.B = g->batch;
   a.f.ny = g->ny;
   a.f.nx = g->nx;
@@ -481,7 +459,7 @@ int operator_adjoint_impl(const swe_grid* g, const swe_config* c, const T* U, co
   AdjWs<T> w = adj_ws<T>(ws, g);
   AdjArgs<T> a;
   memset(&a, 0, sizeof(a));
-  a.k = adj_consts<T>(c, g);
+  a.k = host_consts<T>(c, g->dx, g->dy);
   a.f.B = g->batch;
   a.f.ny = g->ny;
   a.f.nx = g->nx;

@@ -18,75 +18,12 @@
 #include <string.h>
 
 #include "swe_common.h"
+#include "swe_host.h"
 #include "swe_fetch.h"
+#include "swe_stage.h"
 #include "../../include/swe2d_hip.h"
 
 namespace swe {
-
-// accumulator bank: [3 banks][2 (x, y)][kSlots] raw bits of non-negative speeds
-template <class T> struct Bits;
-template <> struct Bits<float> { using U = unsigned int; };
-template <> struct Bits<double> { using U = unsigned long long; };
-
-template <class T> __device__ inline typename Bits<T>::U to_bits(T v);
-template <> __device__ inline unsigned int to_bits<float>(float v) { return __float_as_uint(v); }
-template <> __device__ inline unsigned long long to_bits<double>(double v) {
-  return (unsigned long long)__double_as_longlong(v);
-}
-template <class T> __device__ inline T from_bits(typename Bits<T>::U b);
-template <> __device__ inline float from_bits<float>(unsigned int b) { return __uint_as_float(b); }
-template <> __device__ inline double from_bits<double>(unsigned long long b) {
-  return __longlong_as_double((long long)b);
-}
-
-template <class T> struct StageArgs {
-  SweConsts<T> k;
-  Fields<T> f;
-  int strips, segs, S;
-  int wsr, positivity, friction, friction_model;
-  const T* U;            // [B,3,ny,nx] state at step start
-  const T* k1;           // stage 2: first-stage tendency
-  T* out;                // stage 1: tendency; stage 2: new state
-  T* hmax;               // stage 2, optional running max of depth [B,ny,nx]
-  T* k2_out;             // stage 2, optional record of the second-stage tendency
-  const T* rain;         // optional rain rate field [B or 1, ny, nx]
-  const T* infil;        // optional infiltration field
-  int64_t rain_bstride, infil_bstride;
-  typename Bits<T>::U* acc;       // stage 1 writes, stage 2 reads: [2][kSlots]
-  typename Bits<T>::U* acc_next;  // stage 2 zeroes for the following step
-  Control* ctl;          // control block at the head of the workspace
-  int clock;             // 1: device-driven run (duration / elapsed kept on the device)
-  int parity;            // step parity inside a device-driven run
-  int step_index;        // step index inside a device-driven run
-  double max_dt;         // host-driven steps: explicit max_dt
-  T* dt_series;          // optional per-step dt record
-  int record_wave;       // stage 1: accumulate wave speeds
-};
-
-// dt of the step from the accumulated maxima (solver.py:618-632):
-// min(cfl * reciprocal(max(ax/dx + ay/dy, eps)), max_dt)
-template <class T> __device__ inline T step_dt(const typename Bits<T>::U* acc, const SweConsts<T>& k, T max_dt) {
-  typename Bits<T>::U bx = 0, by = 0;
-  for (int i = 0; i < kSlots; ++i) {
-    typename Bits<T>::U a = acc[i], b = acc[kSlots + i];
-    bx = a > bx ? a : bx;
-    by = b > by ? b : by;
-  }
-  T ax = from_bits<T>(bx), ay = from_bits<T>(by);
-  T den = clamp_lo(ax / k.dx + ay / k.dy, k.eps);
-  T stable = (T(1) / den) * k.cfl;
-  return tmin(stable, max_dt);
-}
-
-template <class T> __device__ inline bool dt_ok(T dt) { return isfinite(dt) && dt > T(0); }
-
-template <class T> __device__ inline T wave_max(T v) {
-  for (int off = 32; off > 0; off >>= 1) {
-    T o = __shfl_xor(v, off, kWave);
-    v = tmax(v, o);
-  }
-  return v;
-}
 
 template <class T, int STAGE>
 __global__ __launch_bounds__(256) void stage_kernel(StageArgs<T> a) {
@@ -96,7 +33,6 @@ __global__ __launch_bounds__(256) void stage_kernel(StageArgs<T> a) {
   const int strip = wid % a.strips;
   const int seg = wid / a.strips;
   if (seg >= a.segs) return;  // uniform per wavefront
-  const Fields<T>& f = a.f;
   const SweConsts<T>& k = a.k;
 
   // ---- step control
@@ -145,94 +81,8 @@ __global__ __launch_bounds__(256) void stage_kernel(StageArgs<T> a) {
     half_dt = T(0.5) * dt;
   }
 
-  const int x0 = strip * kOut;
-  const int y0 = seg * a.S;
-  const int col = x0 - kHalo + lane;
-  const int rows = min(a.S, f.ny - y0);
-  const int64_t mbase = (int64_t)b * 3 * f.plane;
-
-  Prim<T> P2{}, P1{};
-  Side<T> carryL{};       // row r-1 as the top state of face (r-1 | r)
-  T gfh = 0, gft = 0, gfn = 0;  // flux of face (o-1 | o) as seen by row o
-  T ax = T(0), ay = T(0);
-  const bool lane_out = lane >= kHalo && lane < kWave - kHalo && col < f.nx;
-  const bool xface_ok = lane >= 1 && lane <= kWave - 3 && col >= -1 && col <= f.nx - 1;
-  const bool ycol_ok = col >= 0 && col < f.nx;
-
-  for (int it = 0; it < rows + 4; ++it) {
-    const int r = y0 - kHalo + it;
-    Prim<T> Pn = fetch<T, STAGE>(a, b, r, col, half_dt);
-    if (it >= 2) {
-      // y increments of row r-1 and its two face states
-      Cell<T> c1 = P1.y();
-      Incr<T> yi = increments(P2.y(), c1, Pn.y(), k, a.wsr != 0);
-      Side<T> below = side_minus(c1, yi, k);
-      if (it >= 3) {
-        // face (r-2 | r-1)
-        FaceFlux<T> G = face_flux(carryL, P2.h, P2.z, below, P1.h, P1.z, k);
-        if (ycol_ok && r - 2 >= -1 && r - 2 <= f.ny - 1) ay = tmax(ay, G.speed);
-        if (it >= 4) {
-          // output row o = r - 2 (its state is P2)
-          const int o = r - 2;
-          T ry_h = (gfh - G.fh) / k.dy;
-          T ry_u = (gft - G.ft) / k.dy;
-          T ry_v = (gfn - (G.fn - G.sL)) / k.dy;
-          // x sweep of row o
-          Cell<T> cx = P2.x();
-          Cell<T> cm = shup(cx), cp = shdn(cx);
-          Incr<T> xi = increments(cm, cx, cp, k, a.wsr != 0);
-          Side<T> right_of_face = side_minus(cx, xi, k);
-          Side<T> left_of_face = side_plus(cx, xi, k);
-          Side<T> nxt = shdn(right_of_face);
-          FaceFlux<T> F = face_flux(left_of_face, cx.h, cx.z, nxt, cp.h, cp.z, k);
-          if (xface_ok) ax = tmax(ax, F.speed);
-          T pfh = shup(F.fh), pft = shup(F.ft), pfn = shup(F.fn - F.sR);
-          T rx_h = (pfh - F.fh) / k.dx;
-          T rx_u = (pfn - (F.fn - F.sL)) / k.dx;
-          T rx_v = (pft - F.ft) / k.dx;
-          T th = rx_h + ry_h, tu = rx_u + ry_u, tv = rx_v + ry_v;
-          if (lane_out) {
-            int64_t i = mbase + (int64_t)o * f.nx + col;
-            int64_t cell = (int64_t)o * f.nx + col;
-            if (a.rain != nullptr) th = th + a.rain[(int64_t)b * a.rain_bstride + cell];
-            if (a.infil != nullptr) th = th - clamp_lo(a.infil[(int64_t)b * a.infil_bstride + cell], T(0));
-            if (STAGE == 1) {
-              a.out[i] = th;
-              a.out[i + f.plane] = tu;
-              a.out[i + 2 * f.plane] = tv;
-            } else {
-              if (a.k2_out != nullptr) {
-                a.k2_out[i] = th;
-                a.k2_out[i + f.plane] = tu;
-                a.k2_out[i + 2 * f.plane] = tv;
-              }
-              T oh = a.U[i], ou = a.U[i + f.plane], ov = a.U[i + 2 * f.plane];
-              T nh = oh + dt * th, nu = ou + dt * tu, nv = ov + dt * tv;
-              clean_and_limit(oh, ou, ov, nh, nu, nv, k, a.positivity != 0);
-              if (a.friction) {
-                T coef = f.rough[(int64_t)b * f.rough_bstride + cell];
-                swe::friction(nh, nu, nv, coef, dt, k, a.friction_model);
-              }
-              a.out[i] = nh;
-              a.out[i + f.plane] = nu;
-              a.out[i + 2 * f.plane] = nv;
-              if (a.hmax != nullptr) {
-                int64_t hi = (int64_t)b * f.plane + cell;
-                a.hmax[hi] = tmax(a.hmax[hi], nh);
-              }
-            }
-          }
-        }
-        gfh = G.fh;
-        gft = G.ft;
-        gfn = G.fn - G.sR;
-      }
-      carryL = side_plus(c1, yi, k);
-    }
-    P2 = P1;
-    P1 = Pn;
-  }
-
+  T ax, ay;
+  stage_strip<T, STAGE>(a, strip, seg, b, dt, half_dt, ax, ay);
   if (STAGE == 1 && a.record_wave) {
     ax = wave_max(ax);
     ay = wave_max(ay);
@@ -346,29 +196,6 @@ int check_cfg(const swe_config* c) {
   return 0;
 }
 
-template <class T> SweConsts<T> make_consts(const swe_config* c, const swe_grid* g) {
-  const bool f32 = sizeof(T) == 4;
-  const double prec = f32 ? 1e-6 : 1e-10;
-  auto mx = [](double a, double b) { return a > b ? a : b; };
-  SweConsts<T> k;
-  k.g = (T)c->gravity;
-  k.half_g = (T)(0.5 * c->gravity);
-  k.dry = (T)c->dry_depth;
-  k.eps = (T)c->epsilon;
-  k.floor = (T)mx(mx(c->epsilon, c->dry_depth), prec);
-  k.hll_floor = (T)mx(c->epsilon, prec);
-  double ff = mx(c->epsilon, prec);
-  k.fric_floor = (T)ff;
-  k.fric_floor2 = (T)(ff * ff);
-  k.theta = (T)c->limiter_theta;
-  k.cfl = (T)c->cfl;
-  k.dx = (T)g->dx;
-  k.dy = (T)g->dy;
-  k.cval = (T)c->constant_value;
-  k.manning_g = (T)9.81;
-  k.third = (T)(1.0 / 3.0);
-  return k;
-}
 
 int pick_rows(const swe_grid* g, int rows_hint) {
   if (rows_hint > 0) return rows_hint;
@@ -387,7 +214,7 @@ StageArgs<T> make_args(const swe_grid* g, const swe_config* c, const T* bed, con
                        int level_stride, int rows_hint) {
   StageArgs<T> a;
   memset(&a, 0, sizeof(a));
-  a.k = make_consts<T>(c, g);
+  a.k = host_consts<T>(c, g->dx, g->dy);
   a.f.B = g->batch;
   a.f.ny = g->ny;
   a.f.nx = g->nx;

@@ -134,3 +134,76 @@ def tsunami_basin(seed: int = 3, n: int = 8192, dx: float = 100.0, land_fraction
 def cfl_estimate(h_max: float, dx: float, cfl: float = 0.5, g: float = 9.81) -> float:
     """dt of a fluid at rest with depth h_max on a square grid."""
     return cfl * dx / (2.0 * math.sqrt(g * h_max))
+
+
+# ---------------------------------------------------------------------------
+# boundary-control inversion and ensemble workloads
+# ---------------------------------------------------------------------------
+RUNUP_GRIDS = ((172, 272, 0.02), (344, 544, 0.01), (688, 1088, 0.005))
+
+
+def physical_gauges(gauges, dx_from, dx_to):
+    """Map gauge cells between resolutions of the same basin by their centre position."""
+    out = []
+    for iy, ix in gauges:
+        y, x = (iy + 0.5) * dx_from, (ix + 0.5) * dx_from
+        out.append((int(y // dx_to), int(x // dx_to)))
+    return out
+
+
+def spline_times(n_chunks=450, sample_dt=0.05):
+    return np.arange(n_chunks) * sample_dt
+
+
+def greville_points(t0, t1, n_controls, degree=3):
+    """Greville abscissae of the clamped B-spline basis on [t0, t1]."""
+    n_inner = n_controls - degree - 1
+    inner = np.linspace(t0, t1, n_inner + 2)[1:-1] if n_inner > 0 else np.empty(0)
+    knots = np.concatenate([[t0] * (degree + 1), inner, [t1] * (degree + 1)])
+    return np.array([knots[i + 1:i + degree + 1].mean() for i in range(n_controls)])
+
+
+def random_waveform_controls(rng, n_controls, n_chunks=450, sample_dt=0.05, pulses=3):
+    """Seeded spline controls: a random Gaussian-pulse train sampled at the Greville points."""
+    t_end = (n_chunks - 1) * sample_dt
+    p = gaussian_pulses(rng, count=pulses, centre=(0.1 * t_end, 0.6 * t_end))
+    return pulse_level(p, greville_points(0.0, t_end, n_controls)), p
+
+
+def inversion_problem(seed=4, nx=544, ny=344, dx=0.01, n_controls=64, n_chunks=450, sample_dt=0.05):
+    """Boundary-control inversion on the coastal run-up domain: truth waveform and setup."""
+    base = coastal_runup(nx=nx, ny=ny, dx=dx)
+    rng = np.random.default_rng(seed)
+    truth_pulses = gaussian_pulses(rng)
+    times = spline_times(n_chunks, sample_dt)
+    return dict(base=base, truth_pulses=truth_pulses, truth_levels=pulse_level(truth_pulses, times),
+                times=times, n_controls=n_controls, n_chunks=n_chunks, sample_dt=sample_dt)
+
+
+def ensemble_scenarios(n=256, seed=5, n_controls=64, n_chunks=450, sample_dt=0.05, grids=RUNUP_GRIDS):
+    """Seeded scenarios cycling through the run-up grids, each with its own spline waveform.
+
+    Returns (scenarios, controls [n, n_controls]); the basis that maps controls
+    to the per-chunk levels is ``inversion.clamped_bspline_basis(times, n_controls)``.
+    """
+    import torch
+
+    from .inversion import clamped_bspline_basis
+
+    rng = np.random.default_rng(seed)
+    ref = coastal_runup()  # 544 x 344 gauges define the physical gauge points
+    times = torch.tensor(spline_times(n_chunks, sample_dt))
+    basis = clamped_bspline_basis(times, n_controls).numpy()
+    bases = {}
+    out, ctrls = [], []
+    for i in range(n):
+        ny, nx, dx = grids[i % len(grids)]
+        if (ny, nx) not in bases:
+            b = coastal_runup(nx=nx, ny=ny, dx=dx)
+            bases[(ny, nx)] = (b, physical_gauges(ref["gauges"], 0.01, dx))
+        b, g = bases[(ny, nx)]
+        c, _ = random_waveform_controls(rng, n_controls, n_chunks, sample_dt)
+        ctrls.append(c)
+        out.append(dict(bed=b["bed"][0, 0], rough=b["rough"][0, 0], dx=dx, dy=dx, U0=b["U"][0],
+                        levels=basis @ c, chunk=sample_dt, gauges=g))
+    return out, np.stack(ctrls)

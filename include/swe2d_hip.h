@@ -175,6 +175,43 @@ int swe_step_adjoint_f64(const swe_grid* grid, const swe_config* cfg, const doub
                          int level_stride, double dt, const double* lam_out, double* lam_U, double* lam_level,
                          void* adj_ws, int rows_hint, void* stream);
 
+/* ---- ensemble of forward runs (FP32) -------------------------------------
+ * Uneven grids, each with its own adaptive dt, advanced in lockstep by one
+ * launch sequence (the reference batches members on one grid with one shared
+ * dt, solver.py:528-529; here every scenario keeps its own).  A scenario runs
+ * n_chunks calls of SWE2D.forward(U, t_end=chunk_seconds) (solver.py:719-779)
+ * with boundary level levels[k] during chunk k, and records h + z at its gauges
+ * at the end of every chunk (bc_inversion_train.py:332-410). */
+typedef struct swe_scenario {
+  int32_t ny, nx;
+  double dx, dy;
+  float* U_a;                 /* [3, ny, nx]: initial state in, ping-pong buffer */
+  float* U_b;                 /* [3, ny, nx] */
+  float* k1;                  /* [3, ny, nx] scratch */
+  const float* bed;           /* [ny, nx] */
+  const float* roughness;     /* [ny, nx] */
+  const float* levels;        /* [n_chunks] */
+  int32_t n_chunks;
+  double chunk_seconds;
+  const int32_t* gauges;      /* [n_gauges][2] (iy, ix) */
+  int32_t n_gauges;
+  float* gauge_out;           /* [n_chunks][n_gauges] */
+  int32_t rows_hint;          /* rows per wavefront strip; 0 = heuristic */
+} swe_scenario;
+
+size_t swe_ensemble_workspace_bytes(int n_scenarios);
+const char* swe_ensemble_last_error_message(void);
+/* Copies the scenario table to the device workspace (synchronously) and resets
+ * every scenario's clock; returns the number of wavefronts one launch covers. */
+int swe_ensemble_init(const swe_config* cfg, int n_scenarios, const swe_scenario* scenarios, void* ws,
+                      size_t ws_bytes, long long* total_waves, void* stream);
+/* Launches n_steps global steps (3 launches each); finished scenarios exit at
+ * once.  Capturable in a HIP graph (parity comes from first_step + i). */
+int swe_ensemble_advance(void* ws, int n_scenarios, long long total_waves, int first_step, int n_steps,
+                         void* stream);
+size_t swe_ensemble_clock_offset(int n_scenarios);
+size_t swe_ensemble_clock_bytes(void);
+
 #ifdef __cplusplus
 }
 #endif

@@ -11,35 +11,13 @@
 
 #include "swe_adjoint_math.h"
 #include "swe_common.h"
+#include "swe_host.h"
 #include "swe2d_hip.h"
 
 using namespace swe;
 
 namespace {
 
-template <class T> SweConsts<T> consts(const swe_config* c, double dx, double dy) {
-  const bool f32 = sizeof(T) == 4;
-  const double prec = f32 ? 1e-6 : 1e-10;
-  auto mx = [](double a, double b) { return a > b ? a : b; };
-  SweConsts<T> k;
-  k.g = (T)c->gravity;
-  k.half_g = (T)(0.5 * c->gravity);
-  k.dry = (T)c->dry_depth;
-  k.eps = (T)c->epsilon;
-  k.floor = (T)mx(mx(c->epsilon, c->dry_depth), prec);
-  k.hll_floor = (T)mx(c->epsilon, prec);
-  double ff = mx(c->epsilon, prec);
-  k.fric_floor = (T)ff;
-  k.fric_floor2 = (T)(ff * ff);
-  k.theta = (T)c->limiter_theta;
-  k.cfl = (T)c->cfl;
-  k.dx = (T)dx;
-  k.dy = (T)dy;
-  k.cval = (T)c->constant_value;
-  k.manning_g = (T)9.81;
-  k.third = (T)(1.0 / 3.0);
-  return k;
-}
 
 // One member on a padded (ny+4) x (nx+4) grid.
 template <class T> struct Member {
@@ -359,7 +337,7 @@ template <class T> Run<T> make_run(const swe_config* cfg, int B, int ny, int nx,
   R.dx = dx;
   R.dy = dy;
   R.plane = (int64_t)ny * nx;
-  R.k = consts<T>(cfg, dx, dy);
+  R.k = host_consts<T>(cfg, dx, dy);
   return R;
 }
 

@@ -1,0 +1,49 @@
+"""Multi-scenario ensemble against batch-1 SWE2D.forward runs of each scenario."""
+import numpy as np
+import pytest
+import torch
+
+import diffswe2d_amd as swe
+from diffswe2d_amd import scenarios
+from diffswe2d_amd.ensemble import Ensemble
+
+pytestmark = pytest.mark.gpu
+
+GRIDS = ((22, 34, 0.16), (43, 68, 0.08), (86, 136, 0.04))
+
+
+def make(n=6, n_chunks=12, chunk=0.05):
+    sc, ctrl = scenarios.ensemble_scenarios(n=n, seed=7, n_controls=8, n_chunks=n_chunks, sample_dt=chunk,
+                                           grids=GRIDS)
+    # scale the waveforms up so the short run sees the boundary move
+    for s in sc:
+        s["levels"] = 5.0 * s["levels"] + 0.01
+    return sc
+
+
+@pytest.mark.parametrize("graph", [False, True])
+def test_ensemble_matches_single_runs(cuda_device, graph):
+    cfg = swe.SWEConfig(boundary_left="water_level", boundary_right="wall", boundary_top="wall",
+                        boundary_bottom="wall")
+    sc = make()
+    ens = Ensemble(cfg, sc, cuda_device, graph=graph, steps_per_graph=16)
+    ens.run()
+    res = ens.results()
+    steps = [r["steps"] for r in res]
+    print("per-scenario steps", steps)
+    assert all(r["done"] for r in res)
+    assert len(set(steps)) > 1  # scenarios really took their own dt
+    for i, s in enumerate(sc):
+        m = swe.SWE2D(torch.tensor(s["bed"], dtype=torch.float32)[None, None],
+                      torch.tensor(s["rough"], dtype=torch.float32)[None, None], s["dx"], s["dy"],
+                      config=cfg, train_dem=False).to(cuda_device)
+        U = torch.tensor(s["U0"], dtype=torch.float32, device=cuda_device)[None]
+        bed = torch.tensor(s["bed"], dtype=torch.float32)
+        obs, n_steps = [], 0
+        for k, lvl in enumerate(s["levels"]):
+            U, _ = m(U, t_end=s["chunk"], start_time=k * s["chunk"], boundary_level=float(np.float32(lvl)))
+            n_steps += len(m.last_dt_series)
+            obs.append([float(U[0, 0, iy, ix]) + float(bed[iy, ix]) for iy, ix in s["gauges"]])
+        assert n_steps == res[i]["steps"], (i, n_steps, res[i]["steps"])
+        assert torch.equal(U[0], res[i]["U"]), f"scenario {i}: final state differs"
+        np.testing.assert_array_equal(np.float32(obs), res[i]["gauges"].cpu().numpy())
