@@ -178,8 +178,15 @@ def main():
         sys.exit(1)
     dev = torch.device("cuda:0")
     model, U, sc, cfg = basin_model(a.n, dev)
+    tuned = None
     if a.rows:
         model.rows_hint = a.rows
+    else:
+        from diffswe2d_amd import autotune
+
+        best, rep = autotune.tune_rows(model, U, candidates=(16, 32, 64, 128), reps=6, save=False)
+        tuned = {str(k): round(v, 4) for k, v in rep.items()}
+        print(f"# autotuned rows per strip: {best} ({tuned})", file=sys.stderr, flush=True)
     n = a.n
     cells = n * n
 
@@ -261,7 +268,7 @@ def main():
         "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
         "config": {"workload": f"tsunami_basin_{n}x{n}_dx100m_manning0.025_transmissive_cfl0.5",
                    "cells": cells, "steps_of_run": a.steps, "launches_per_step": 2,
-                   "rows_per_strip": int(model.rows_hint)},
+                   "rows_per_strip": int(model.rows_hint), "autotune_ms_per_step": tuned},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "traffic_source": traffic_src,

@@ -197,6 +197,13 @@ class SWE2D(nn.Module):
         self._field_cache["fields"] = (key, bed, rough)
         return bed, rough
 
+    def _rows(self, U):
+        if self.rows_hint:
+            return int(self.rows_hint)
+        from . import autotune
+
+        return autotune.lookup(U.device, U.shape[-2], U.shape[-1], U.shape[0], U.dtype)
+
     def _workspace(self, U, bed, rough):
         B, _, ny, nx = U.shape
         key = (U.device, U.dtype, B, ny, nx, bed.shape[0], rough.shape[0])
@@ -266,7 +273,7 @@ class SWE2D(nn.Module):
         rain, infil = field(rainfall_rate), field(infiltration)
         fn = lib.swe_spatial_operator_f32 if U.dtype == torch.float32 else lib.swe_spatial_operator_f64
         _lib.check(fn(ws["grid"], ws["cfg"], _lib.ptr(U), _lib.ptr(bed), _lib.ptr(lvl), stride, _lib.ptr(rain),
-                      _lib.ptr(infil), _lib.ptr(rhs), _lib.ptr(ws["buf"]), int(self.rows_hint),
+                      _lib.ptr(infil), _lib.ptr(rhs), _lib.ptr(ws["buf"]), self._rows(U),
                       _lib.stream_ptr(U.device)), "spatial_operator")
         if return_wave_speed:
             ax, ay = self._bank_speeds(ws, U)
@@ -318,7 +325,7 @@ class SWE2D(nn.Module):
         out = torch.empty_like(U)
         fn = lib.swe_step_f32 if U.dtype == torch.float32 else lib.swe_step_f64
         _lib.check(fn(ws["grid"], ws["cfg"], _lib.ptr(U), _lib.ptr(bed), _lib.ptr(rough), _lib.ptr(lvl), stride,
-                      float(max_dt_value), ws["parity"], _lib.ptr(out), _lib.ptr(ws["buf"]), int(self.rows_hint),
+                      float(max_dt_value), ws["parity"], _lib.ptr(out), _lib.ptr(ws["buf"]), self._rows(U),
                       _lib.stream_ptr(U.device)), "step")
         ws["parity"] ^= 1
         dt = _lib.read_control(ws["buf"]).last_dt
@@ -428,7 +435,7 @@ class SWE2D(nn.Module):
             _lib.check(fn(ws["grid"], ws["cfg"], _lib.ptr(bed), _lib.ptr(rough), _lib.ptr(lvl), stride,
                           _lib.ptr(Ua), _lib.ptr(Ub), launched, n, _lib.ptr(hmax), _lib.ptr(dts), capacity,
                           _lib.ptr(gauges), len(names), _lib.ptr(gd) if names else None,
-                          _lib.ptr(gl) if names else None, capacity, _lib.ptr(ws["buf"]), int(self.rows_hint),
+                          _lib.ptr(gl) if names else None, capacity, _lib.ptr(ws["buf"]), self._rows(U0),
                           stream), "advance")
             launched += n
             ctl = _lib.read_control(ws["buf"])
