@@ -86,6 +86,16 @@ _SIGNATURES = {
     "swe_ensemble_advance": ([_P, _I, ctypes.c_longlong, _I, _I, _P], _I),
     "swe_ensemble_clock_offset": ([_I], ctypes.c_size_t),
     "swe_ensemble_clock_bytes": ([], ctypes.c_size_t),
+    "sur_last_error_message": ([], ctypes.c_char_p),
+    "sur_gemm_nt_bf16": ([_I, _I, _I, _P, ctypes.c_long, _P, ctypes.c_long, _P, ctypes.c_long, _I, _P, _P], _I),
+    "sur_im2col": ([_P, _I, _I, _I, _I, _I, _I, _I, _P, _P], _I),
+    "sur_col2im": ([_P, _I, _I, _I, _I, _I, _P, _P], _I),
+    "sur_bias_act": ([_P, _P, _I, ctypes.c_long, _I, _P, _P, _P, _P], _I),
+    "sur_act_bwd": ([_P, _P, _I, ctypes.c_long, _I, _P, _P, _P, _I, _P], _I),
+    "sur_bias_grad": ([_P, _I, ctypes.c_long, _P, _P], _I),
+    "sur_cast_bf16": ([_P, ctypes.c_long, _P, _P], _I),
+    "sur_cast_pad_rows": ([_P, _I, _I, _I, _P, _P], _I),
+    "sur_transpose_bf16": ([_P, _I, _I, _I, _P, _P], _I),
 }
 
 _lib = None

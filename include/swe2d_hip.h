@@ -212,6 +212,26 @@ int swe_ensemble_advance(void* ws, int n_scenarios, long long total_waves, int f
 size_t swe_ensemble_clock_offset(int n_scenarios);
 size_t swe_ensemble_clock_bytes(void);
 
+/* ---- BF16 MFMA kernels of the 1-D convolution surrogate --------------------
+ * No reference counterpart (the surrogate warm start is new).  bf16 operands
+ * are raw 16-bit words; accumulation is FP32. */
+const char* sur_last_error_message(void);
+/* C[m][n] = sum_k A[m*lda + k] * B[n*ldb + k]; K % 16 == 0; split-K into FP32
+ * partials [splits][M][N] reduced in split order when splits > 1. */
+int sur_gemm_nt_bf16(int M, int N, int K, const void* A, long lda, const void* B, long ldb, float* C, long ldc,
+                     int splits, float* partials, void* stream);
+int sur_im2col(const void* X, int Cin, int Bn, int L, int Kw, int dil, int Kpad, int transpose, void* col,
+               void* stream);
+int sur_col2im(const float* dcol, int Cin, int Bn, int L, int Kw, int dil, float* dX, void* stream);
+int sur_bias_act(const float* C, const float* bias, int M, long N, int relu, float* pre, void* out_bf16,
+                 float* out_f32, void* stream);
+int sur_act_bwd(const float* g_out, const float* pre, int M, long N, int relu, float* g_pre, void* g_bf16,
+                void* g_bf16_t, int ldt, void* stream);
+int sur_bias_grad(const float* g, int M, long N, float* db, void* stream);
+int sur_cast_bf16(const float* x, long count, void* y, void* stream);
+int sur_cast_pad_rows(const float* x, int R, int Cc, int Cpad, void* y, void* stream);
+int sur_transpose_bf16(const float* x, int R, int Cc, int Rpad, void* y, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

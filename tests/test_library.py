@@ -10,7 +10,7 @@ ROOT = Path(__file__).resolve().parent.parent
 
 def declared_symbols():
     text = (ROOT / "include" / "swe2d_hip.h").read_text()
-    return sorted(set(re.findall(r"^\S.*?\b(swe_\w+)\(", text, flags=re.M)))
+    return sorted(set(re.findall(r"^\S.*?\b((?:swe|sur)_\w+)\(", text, flags=re.M)))
 
 
 def test_header_and_binding_agree():
