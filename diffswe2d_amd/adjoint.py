@@ -42,8 +42,12 @@ def _fn(lib, name, dtype):
     return getattr(lib, f"{name}_{'f32' if dtype == torch.float32 else 'f64'}")
 
 
-def record_step(model, U, level, max_dt):
-    """One forward step that also returns k1, k2 and dt (for replay)."""
+def record_step(model, U, level, max_dt, read_dt=True):
+    """One forward step that also returns k1, k2 and dt (for replay).
+
+    With read_dt False the dt is not read back (no host synchronisation) and
+    max_dt is returned: a replay with the recorded dt reproduces it exactly,
+    because the recorded dt never exceeds the CFL value it is the minimum of."""
     lib = _lib.load()
     bed, rough = model._fields(U)
     ws = model._workspace(U, bed, rough)
@@ -54,7 +58,7 @@ def record_step(model, U, level, max_dt):
         ws["parity"], _lib.ptr(out), _lib.ptr(k1), _lib.ptr(k2), _lib.ptr(ws["buf"]), int(model.rows_hint),
         _lib.stream_ptr(U.device)), "step_record")
     ws["parity"] ^= 1
-    dt = _lib.read_control(ws["buf"]).last_dt
+    dt = _lib.read_control(ws["buf"]).last_dt if read_dt else float(max_dt)
     return out, k1, k2, dt
 
 
@@ -111,31 +115,59 @@ def differentiable_step(model, U, level, max_dt):
     return out, float(dt)
 
 
+def _checkpointed_run(model, U0, level, duration, max_steps, cap, K):
+    """Device-driven forward (no per-step host sync) keeping the state every K steps.
+
+    Returns (U, hmax, dts, checkpoints, done, elapsed)."""
+    lib = _lib.load()
+    bed, rough = model._fields(U0)
+    ws = model._workspace(U0, bed, rough)
+    lvl, stride = model._level_arg(level, U0)
+    stream = _lib.stream_ptr(U0.device)
+    Ua, Ub = U0.clone(), torch.empty_like(U0)
+    hmax = U0[:, 0].clone()
+    cap_arg = -1.0 if cap is None else float(cap)
+    _lib.check(lib.swe_run_begin(_lib.ptr(ws["buf"]), float(duration), cap_arg, U0.element_size(), stream),
+               "run_begin")
+    fn = _fn(lib, "swe_advance", U0.dtype)
+    capacity = max(K, 64)
+    dts = torch.empty(capacity, dtype=U0.dtype, device=U0.device)
+    checkpoints = []
+    launched, steps = 0, 0
+    ctl = None
+    while launched < max_steps:
+        cur = Ua if steps % 2 == 0 else Ub
+        checkpoints.append(cur.clone())
+        n = min(K, max_steps - launched)
+        if launched + n > capacity:
+            capacity = 2 * (launched + n)
+            dts = torch.cat((dts, torch.empty(capacity - dts.numel(), dtype=dts.dtype, device=dts.device)))
+        _lib.check(fn(ws["grid"], ws["cfg"], _lib.ptr(bed), _lib.ptr(rough), _lib.ptr(lvl), stride, _lib.ptr(Ua),
+                      _lib.ptr(Ub), launched, n, _lib.ptr(hmax), _lib.ptr(dts), capacity, None, 0, None, None, 0,
+                      _lib.ptr(ws["buf"]), model._rows(U0), stream), "advance")
+        launched += n
+        ctl = _lib.read_control(ws["buf"])
+        steps = ctl.steps
+        if ctl.done:
+            break
+    if ctl.status != 0:
+        if ctl.status == 1:
+            raise FloatingPointError("The calculated timestep is non-finite.")
+        raise FloatingPointError(f"Non-positive timestep: {ctl.bad_dt}.")
+    U = Ua if steps % 2 == 0 else Ub
+    dt_list = dts[:steps].cpu().tolist()
+    # a checkpoint taken after the last step is not needed
+    checkpoints = checkpoints[:(steps + K - 1) // K] if steps > 0 else checkpoints[:1]
+    return U, hmax, dt_list, checkpoints, bool(ctl.done), ctl.elapsed[steps & 1]
+
+
 class _ForwardFn(torch.autograd.Function):
     @staticmethod
     def forward(ctx, U0, level, model, duration, max_steps, cap):
-        U = U0.detach().contiguous()
+        U0_ = U0.detach().contiguous()
         lvl = level.detach()
         K = int(getattr(model, "adjoint_checkpoint_every", 64))
-        checkpoints, dts = [], []
-        hmax = U[:, 0].clone()
-        elapsed = 0.0
-        tol = max(1e-12, 1e-12 * abs(duration))
-        done = False
-        for it in range(max_steps):
-            remaining = duration - elapsed
-            if remaining <= tol:
-                done = True
-                break
-            if it % K == 0:
-                checkpoints.append(U)
-            step_max = remaining if cap is None else min(remaining, cap)
-            U, dt = model._step_raw(U, lvl, step_max)
-            hmax = torch.maximum(hmax, U[:, 0])
-            dts.append(dt)
-            elapsed += dt
-            if elapsed > duration and elapsed - duration <= tol:
-                elapsed = duration
+        U, hmax, dts, checkpoints, done, elapsed = _checkpointed_run(model, U0_, lvl, duration, max_steps, cap, K)
         if not done:
             raise RuntimeError(
                 f"max_steps={max_steps} reached after {elapsed:.8f} s; requested duration={duration:.8f} s.")
@@ -149,16 +181,17 @@ class _ForwardFn(torch.autograd.Function):
     def backward(ctx, g_U, _g_hmax):
         lvl, *checkpoints = ctx.saved_tensors
         model, dts, K = ctx.model, ctx.dts, ctx.K
+        verify = bool(getattr(model, "adjoint_verify_replay", False))
         lam = g_U.contiguous()
         lam_level = torch.zeros(lam.shape[0], dtype=lam.dtype, device=lam.device)
         n = len(dts)
-        for seg in range(len(checkpoints) - 1, -1, -1):
+        for seg in range((n + K - 1) // K - 1, -1, -1):
             first, last = seg * K, min(n, (seg + 1) * K)
             U = checkpoints[seg]
             tape = []
             for k in range(first, last):
-                U_next, k1, k2, dt = record_step(model, U, lvl, dts[k])
-                if dt != dts[k]:
+                U_next, k1, k2, dt = record_step(model, U, lvl, dts[k], read_dt=verify)
+                if verify and dt != dts[k]:
                     raise RuntimeError(f"adjoint replay diverged at step {k}: dt {dt} != {dts[k]}")
                 tape.append((U, k1, k2))
                 U = U_next

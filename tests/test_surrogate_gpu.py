@@ -57,7 +57,9 @@ def test_network_forward_and_gradients(cuda_device):
     lr.backward()
     ef = float((y.detach().cpu() - yr.detach()).abs().max() / yr.abs().max())
     print(f"forward max rel err {ef:.2e}; loss {float(loss):.6f} vs {float(lr):.6f}")
-    assert ef < 1e-4
+    # bf16 activations: a last-bit FP32 difference before a bf16 rounding flips that
+    # activation by one bf16 ulp (0.4 %), so outputs agree to ~1e-3, not FP32 precision
+    assert ef < 5e-3
     for i, (p, q) in enumerate(zip(net.weights, params)):
         e = float((p.grad.cpu() - q.grad).norm() / max(float(q.grad.norm()), 1e-30))
         print(f"param {i}: grad rel err {e:.2e}")
