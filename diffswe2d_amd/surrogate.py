@@ -133,16 +133,20 @@ def _conv_forward(K, spec, x, W, b, Bn, L):
     return y, (xb, pre)
 
 
-def _conv_backward(K, spec, saved, W, g_y, Bn, L, need_dx, splits):
+def _conv_backward(K, spec, saved, W, g_y, Bn, L, need_dx, splits, need_w=True):
     xb, pre = saved
     BL = Bn * L
     cpad = _ceil16(spec.cout)
     g_pre, g_bf, g_t = K.act_bwd(g_y.contiguous(), pre, spec.relu, cpad)
     kpad = _ceil16(spec.cin * spec.kw)
-    colT = K.im2col(xb, spec.cin, Bn, L, spec.kw, spec.dil, kpad, True)
     nk = spec.cin * spec.kw
-    dW = K.gemm(spec.cout, nk, BL, g_bf, BL, colT, BL, splits=splits).reshape(spec.cout, spec.cin, spec.kw)
-    db = K.bias_grad(g_pre)
+    dW = db = None
+    if need_w:
+        if BL % 16:
+            raise ValueError(f"weight gradients need batch * length divisible by 16, got {BL}")
+        colT = K.im2col(xb, spec.cin, Bn, L, spec.kw, spec.dil, kpad, True)
+        dW = K.gemm(spec.cout, nk, BL, g_bf, BL, colT, BL, splits=splits).reshape(spec.cout, spec.cin, spec.kw)
+        db = K.bias_grad(g_pre)
     dx = None
     if need_dx:
         Wt = K.transpose(W.detach().reshape(spec.cout, nk).contiguous(), cpad)
@@ -179,7 +183,9 @@ class _NetFn(torch.autograd.Function):
         for i in range(len(net.specs) - 1, -1, -1):
             spec = net.specs[i]
             need_dx = i > 0 or need_x
-            gy, dW, db = _conv_backward(net.K, spec, ctx.saved[i], ctx.params[2 * i], gy, Bn, L, need_dx, splits)
+            need_w = ctx.needs_input_grad[2 + 2 * i] or ctx.needs_input_grad[3 + 2 * i]
+            gy, dW, db = _conv_backward(net.K, spec, ctx.saved[i], ctx.params[2 * i], gy, Bn, L, need_dx, splits,
+                                        need_w)
             grads[2 * i], grads[2 * i + 1] = dW, db
         gx = gy.reshape(Bn, L) if need_x else None
         return (gx, None, *grads)

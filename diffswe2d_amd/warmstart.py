@@ -57,13 +57,20 @@ def surrogate_start(net, basis, target, raw0, bounds=(-0.02, 0.05), iters=300, l
     opt = torch.optim.Adam([raw], lr=lr)
     b = basis.float()
     tgt = target.float()
-    for _ in range(iters):
-        opt.zero_grad()
-        series = b @ inversion.bounded_controls(raw, *bounds)
-        pred = net(series[None])[0].T  # [n_chunks, 6]
-        loss = ((pred - tgt) / net.out_scale).square().mean()
-        loss.backward()
-        opt.step()
+    flags = [p.requires_grad for p in net.weights]
+    for p in net.weights:
+        p.requires_grad_(False)  # only the input gradient is needed
+    try:
+        for _ in range(iters):
+            opt.zero_grad()
+            series = b @ inversion.bounded_controls(raw, *bounds)
+            pred = net(series[None])[0].T  # [n_chunks, 6]
+            loss = ((pred - tgt) / net.out_scale).square().mean()
+            loss.backward()
+            opt.step()
+    finally:
+        for p, f in zip(net.weights, flags):
+            p.requires_grad_(f)
     return raw.detach().to(raw0.dtype)
 
 
