@@ -80,6 +80,9 @@ _SIGNATURES = {
     "swe_operator_adjoint_f64": ([_G, _C, _P, _P, _P, _I, _P, _P, _P, _P, _I, _P], _I),
     "swe_step_adjoint_f32": ([_G, _C, _P, _P, _P, _P, _P, _P, _I, _D, _P, _P, _P, _P, _I, _P], _I),
     "swe_step_adjoint_f64": ([_G, _C, _P, _P, _P, _P, _P, _P, _I, _D, _P, _P, _P, _P, _I, _P], _I),
+    "swe_segment_tape_bytes": ([_G, _I, _I], ctypes.c_size_t),
+    "swe_segment_adjoint_f32": ([_G, _C, _P, _P, _P, _P, _I, _P, _I, _I, _P, _P, _P, _P, _P, _P, _I, _P], _I),
+    "swe_segment_adjoint_f64": ([_G, _C, _P, _P, _P, _P, _I, _P, _I, _I, _P, _P, _P, _P, _P, _P, _I, _P], _I),
     "swe_ensemble_workspace_bytes": ([_I], ctypes.c_size_t),
     "swe_ensemble_last_error_message": ([], ctypes.c_char_p),
     "swe_ensemble_init": ([_C, _I, _P, _P, ctypes.c_size_t, ctypes.POINTER(ctypes.c_longlong), _P], _I),
@@ -130,10 +133,11 @@ def load(path: Path | None = None):
     return lib
 
 
-def check(code: int, what: str):
+def check(code: int, what: str, source: str = "swe_last_error_message"):
+    """Raise RuntimeError for a non-zero status; source names the message function of the entry point's file."""
     if code != 0:
         lib = load()
-        msg = lib.swe_last_error_message().decode(errors="replace")
+        msg = getattr(lib, source)().decode(errors="replace")
         raise RuntimeError(f"{what} failed ({code}: {lib.swe_strerror(code).decode()}): {msg}")
 
 

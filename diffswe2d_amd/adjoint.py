@@ -18,9 +18,13 @@ correction) and roughness are treated as constants.
 """
 from __future__ import annotations
 
+import ctypes
+
 import torch
 
 from . import _lib
+
+_ADJ_MSG = "swe_adjoint_last_error_message"
 
 
 def _adj_ws(model, U):
@@ -73,7 +77,7 @@ def step_adjoint(model, U, k1, k2, level, dt, lam_out, lam_level):
     _lib.check(_fn(lib, "swe_step_adjoint", U.dtype)(
         ws["grid"], ws["cfg"], _lib.ptr(U), _lib.ptr(k1), _lib.ptr(k2), _lib.ptr(bed), _lib.ptr(rough),
         _lib.ptr(lvl), stride, float(dt), _lib.ptr(lam_out.contiguous()), _lib.ptr(lam_U), _lib.ptr(lam_level),
-        _lib.ptr(aws["buf"]), int(model.rows_hint), _lib.stream_ptr(U.device)), "step_adjoint")
+        _lib.ptr(aws["buf"]), int(model.rows_hint), _lib.stream_ptr(U.device)), "step_adjoint", _ADJ_MSG)
     return lam_U
 
 
@@ -181,26 +185,65 @@ class _ForwardFn(torch.autograd.Function):
     def backward(ctx, g_U, _g_hmax):
         lvl, *checkpoints = ctx.saved_tensors
         model, dts, K = ctx.model, ctx.dts, ctx.K
-        verify = bool(getattr(model, "adjoint_verify_replay", False))
-        lam = g_U.contiguous()
+        lam = g_U.contiguous().clone()
         lam_level = torch.zeros(lam.shape[0], dtype=lam.dtype, device=lam.device)
         n = len(dts)
-        for seg in range((n + K - 1) // K - 1, -1, -1):
-            first, last = seg * K, min(n, (seg + 1) * K)
-            U = checkpoints[seg]
-            tape = []
-            for k in range(first, last):
-                U_next, k1, k2, dt = record_step(model, U, lvl, dts[k], read_dt=verify)
-                if verify and dt != dts[k]:
-                    raise RuntimeError(f"adjoint replay diverged at step {k}: dt {dt} != {dts[k]}")
-                tape.append((U, k1, k2))
-                U = U_next
-            for k in range(last - 1, first - 1, -1):
-                Uk, k1, k2 = tape[k - first]
-                lam = step_adjoint(model, Uk, k1, k2, lvl, dts[k], lam, lam_level)
-            del tape
+        if n == 0:
+            g_level = _level_grad(lvl, lam_level) if ctx.needs_input_grad[1] else None
+            return lam, g_level, None, None, None, None
+        if getattr(model, "adjoint_verify_replay", False):
+            lam = _replay_checked(model, checkpoints, lvl, dts, K, lam, lam_level)
+        else:
+            lam = _replay_segments(model, checkpoints, lvl, dts, K, lam, lam_level)
         g_level = _level_grad(lvl, lam_level) if ctx.needs_input_grad[1] else None
         return lam, g_level, None, None, None, None
+
+
+def _replay_segments(model, checkpoints, lvl, dts, K, lam, lam_level):
+    """Each segment: one library call replays it onto a tape and sweeps it in reverse."""
+    lib = _lib.load()
+    U0 = checkpoints[0]
+    bed, rough = model._fields(U0)
+    ws = model._workspace(U0, bed, rough)
+    aws = _adj_ws(model, U0)
+    l, stride = model._level_arg(lvl, U0)
+    n = len(dts)
+    seg_len = min(K, n)
+    tape = torch.empty(lib.swe_segment_tape_bytes(ws["grid"], seg_len, U0.element_size()) // U0.element_size(),
+                       dtype=U0.dtype, device=U0.device)
+    lam_tmp = torch.empty_like(lam)
+    host_dts = (ctypes.c_double * n)(*dts)
+    fn = _fn(lib, "swe_segment_adjoint", U0.dtype)
+    stream = _lib.stream_ptr(U0.device)
+    rows = int(model.rows_hint)
+    for seg in range((n + K - 1) // K - 1, -1, -1):
+        first, last = seg * K, min(n, (seg + 1) * K)
+        _lib.check(fn(ws["grid"], ws["cfg"], _lib.ptr(checkpoints[seg]), _lib.ptr(bed), _lib.ptr(rough), _lib.ptr(l),
+                      stride, ctypes.addressof(host_dts) + 8 * first, last - first, ws["parity"], _lib.ptr(lam),
+                      _lib.ptr(lam_tmp), _lib.ptr(lam_level), _lib.ptr(tape), _lib.ptr(ws["buf"]),
+                      _lib.ptr(aws["buf"]), rows, stream), "segment_adjoint", _ADJ_MSG)
+        ws["parity"] ^= (last - first) & 1
+    return lam
+
+
+def _replay_checked(model, checkpoints, lvl, dts, K, lam, lam_level):
+    """Step-by-step replay that reads every dt back and checks it against the recorded one."""
+    n = len(dts)
+    for seg in range((n + K - 1) // K - 1, -1, -1):
+        first, last = seg * K, min(n, (seg + 1) * K)
+        U = checkpoints[seg]
+        tape = []
+        for k in range(first, last):
+            U_next, k1, k2, dt = record_step(model, U, lvl, dts[k], read_dt=True)
+            if dt != dts[k]:
+                raise RuntimeError(f"adjoint replay diverged at step {k}: dt {dt} != {dts[k]}")
+            tape.append((U, k1, k2))
+            U = U_next
+        for k in range(last - 1, first - 1, -1):
+            Uk, k1, k2 = tape[k - first]
+            lam = step_adjoint(model, Uk, k1, k2, lvl, dts[k], lam, lam_level)
+        del tape
+    return lam
 
 
 def differentiable_forward(model, U0, duration, absolute_start, max_steps, boundary_level, cap):
@@ -237,7 +280,7 @@ class _OperatorFn(torch.autograd.Function):
         _lib.check(_fn(lib, "swe_operator_adjoint", U.dtype)(
             ws["grid"], ws["cfg"], _lib.ptr(U), _lib.ptr(bed), _lib.ptr(l), stride, _lib.ptr(g.contiguous()),
             _lib.ptr(lam_U), _lib.ptr(lam_level), _lib.ptr(aws["buf"]), int(model.rows_hint),
-            _lib.stream_ptr(U.device)), "operator_adjoint")
+            _lib.stream_ptr(U.device)), "operator_adjoint", _ADJ_MSG)
         g_level = _level_grad(lvl, lam_level) if ctx.needs_input_grad[1] else None
         return lam_U, g_level, None
 

@@ -493,7 +493,86 @@ int operator_adjoint_impl(const swe_grid* g, const swe_config* c, const T* U, co
 
 }  // namespace
 
+extern "C" int swe_step_record_f32(const swe_grid*, const swe_config*, const float*, const float*, const float*,
+                                   const float*, int, double, int, float*, float*, float*, void*, int, void*);
+extern "C" const char* swe_last_error_message(void);
+extern "C" int swe_step_record_f64(const swe_grid*, const swe_config*, const double*, const double*, const double*,
+                                   const double*, int, double, int, double*, double*, double*, void*, int, void*);
+
+namespace {
+
+inline int record_step_t(const swe_grid* g, const swe_config* c, const float* U, const float* bed, const float* rough,
+                         const float* level, int ls, double dt, int parity, float* out, float* k1, float* k2, void* ws,
+                         int rows, void* stream) {
+  return swe_step_record_f32(g, c, U, bed, rough, level, ls, dt, parity, out, k1, k2, ws, rows, stream);
+}
+inline int record_step_t(const swe_grid* g, const swe_config* c, const double* U, const double* bed,
+                         const double* rough, const double* level, int ls, double dt, int parity, double* out,
+                         double* k1, double* k2, void* ws, int rows, void* stream) {
+  return swe_step_record_f64(g, c, U, bed, rough, level, ls, dt, parity, out, k1, k2, ws, rows, stream);
+}
+
+// Replay one checkpoint segment (recording U_k, k1_k, k2_k on the tape) and
+// sweep it in reverse.  tape holds 3 * n_steps + 1 states of B*3*ny*nx values.
+template <class T>
+int segment_adjoint_impl(const swe_grid* g, const swe_config* c, const T* U_ckpt, const T* bed, const T* rough,
+                         const T* level, int ls, const double* dts, int n_steps, int parity, T* lam, T* lam_tmp,
+                         T* lam_level, T* tape, void* fwd_ws, void* adj_ws, int rows, void* stream) {
+  if (!g || !c || !U_ckpt || !dts || !lam || !lam_tmp || !tape || !fwd_ws || !adj_ws || n_steps <= 0)
+    return afail(SWE_ERR_ARG, "bad segment arguments");
+  const size_t S = (size_t)g->batch * 3 * (size_t)g->ny * g->nx;
+  T* Ustates = tape;                            // n_steps + 1 states
+  T* K1 = tape + (size_t)(n_steps + 1) * S;     // n_steps
+  T* K2 = K1 + (size_t)n_steps * S;             // n_steps
+  hipError_t e = hipMemcpyAsync(Ustates, U_ckpt, S * sizeof(T), hipMemcpyDeviceToDevice, (hipStream_t)stream);
+  if (e != hipSuccess) return afail(SWE_ERR_HIP, hipGetErrorString(e));
+  int rc;
+  for (int k = 0; k < n_steps; ++k) {
+    rc = record_step_t(g, c, Ustates + k * S, bed, rough, level, ls, dts[k], (parity + k) & 1, Ustates + (k + 1) * S,
+                       K1 + k * S, K2 + k * S, fwd_ws, rows, stream);
+    if (rc) return afail(rc, swe_last_error_message());
+  }
+  T* cur = lam;
+  T* nxt = lam_tmp;
+  for (int k = n_steps - 1; k >= 0; --k) {
+    rc = step_adjoint_impl<T>(g, c, Ustates + k * S, K1 + k * S, K2 + k * S, bed, rough, level, ls, dts[k], cur, nxt,
+                              lam_level, adj_ws, rows, stream);
+    if (rc) return rc;
+    T* t = cur;
+    cur = nxt;
+    nxt = t;
+  }
+  if (cur != lam) {
+    e = hipMemcpyAsync(lam, cur, S * sizeof(T), hipMemcpyDeviceToDevice, (hipStream_t)stream);
+    if (e != hipSuccess) return afail(SWE_ERR_HIP, hipGetErrorString(e));
+  }
+  return 0;
+}
+
+}  // namespace
+
 extern "C" {
+
+size_t swe_segment_tape_bytes(const swe_grid* g, int n_steps, int dtype_bytes) {
+  if (!g || n_steps <= 0) return 0;
+  return (size_t)(3 * n_steps + 1) * (size_t)g->batch * 3 * (size_t)g->ny * g->nx * (size_t)dtype_bytes;
+}
+
+int swe_segment_adjoint_f32(const swe_grid* g, const swe_config* c, const float* U_ckpt, const float* bed,
+                            const float* rough, const float* level, int level_stride, const double* dts, int n_steps,
+                            int parity, float* lam, float* lam_tmp, float* lam_level, float* tape, void* fwd_ws,
+                            void* adj_ws, int rows_hint, void* stream) {
+  return segment_adjoint_impl<float>(g, c, U_ckpt, bed, rough, level, level_stride, dts, n_steps, parity, lam,
+                                     lam_tmp, lam_level, tape, fwd_ws, adj_ws, rows_hint, stream);
+}
+
+int swe_segment_adjoint_f64(const swe_grid* g, const swe_config* c, const double* U_ckpt, const double* bed,
+                            const double* rough, const double* level, int level_stride, const double* dts,
+                            int n_steps, int parity, double* lam, double* lam_tmp, double* lam_level, double* tape,
+                            void* fwd_ws, void* adj_ws, int rows_hint, void* stream) {
+  return segment_adjoint_impl<double>(g, c, U_ckpt, bed, rough, level, level_stride, dts, n_steps, parity, lam,
+                                      lam_tmp, lam_level, tape, fwd_ws, adj_ws, rows_hint, stream);
+}
 
 size_t swe_adjoint_workspace_bytes(const swe_grid* g, int dtype_bytes) {
   if (!g || g->batch <= 0 || g->ny < 2 || g->nx < 2) return 0;

@@ -78,8 +78,10 @@ class Ensemble:
     def n_done(self) -> int:
         return int(self.ws[4:8].view(torch.int32).item())
 
-    def run(self, max_steps=10_000_000):
-        """Advance until every scenario has finished its chunks; returns global steps taken."""
+    def run(self, max_steps=10_000_000, log=None, log_every=2048):
+        """Advance until every scenario has finished its chunks; returns global steps taken.
+
+        ``log`` (optional) receives a progress line every ``log_every`` global steps."""
         stream = _lib.stream_ptr(self.device)
         if self.graph_enabled and self._graph is None and self.steps % 2 == 0:
             g = torch.cuda.CUDAGraph()
@@ -93,8 +95,11 @@ class Ensemble:
             else:
                 self._advance(self.steps, self.spg, stream)
                 self.steps += self.spg
-            if self.n_done() >= self.n:
+            done = self.n_done()
+            if done >= self.n:
                 break
+            if log and self.steps % log_every < self.spg:
+                log(f"ensemble: {self.steps} global steps, {done}/{self.n} scenarios done")
         torch.cuda.synchronize(self.device)
         return self.steps
 

@@ -173,6 +173,9 @@ def random_waveform_controls(rng, n_controls, n_chunks=450, sample_dt=0.05, puls
 def inversion_problem(seed=4, nx=544, ny=344, dx=0.01, n_controls=64, n_chunks=450, sample_dt=0.05):
     """Boundary-control inversion on the coastal run-up domain: truth waveform and setup."""
     base = coastal_runup(nx=nx, ny=ny, dx=dx)
+    # the gauges are physical points (the 544 x 344 grid's picks), the same
+    # points the ensemble scenarios sample at every resolution
+    base["gauges"] = physical_gauges(coastal_runup()["gauges"], 0.01, dx)
     rng = np.random.default_rng(seed)
     t_end = (n_chunks - 1) * sample_dt
     truth_pulses = gaussian_pulses(rng, centre=(0.1 * t_end, 0.6 * t_end))

@@ -113,8 +113,9 @@ class _Kernels:
         return y
 
 
-def default_specs(channels=160, dilations=(1, 2, 4, 8, 16, 32, 64, 1), n_out=6):
-    specs = [ConvSpec(1, channels, 7, 1, True, False)]
+def default_specs(channels=160, dilations=(1, 2, 4, 8, 16, 32, 64, 1), n_out=6, n_cond=0):
+    """Layer list; the first layer reads the series plus n_cond constant condition channels."""
+    specs = [ConvSpec(1 + n_cond, channels, 7, 1, True, False)]
     specs += [ConvSpec(channels, channels, 5, d, True, True) for d in dilations]
     specs.append(ConvSpec(channels, n_out, 1, 1, False, False))
     return specs
@@ -160,21 +161,21 @@ def _conv_backward(K, spec, saved, W, g_y, Bn, L, need_dx, splits, need_w=True):
 class _NetFn(torch.autograd.Function):
     @staticmethod
     def forward(ctx, x, net, *params):
-        Bn, L = x.shape
+        Bn, cin, L = x.shape
         K = net.K
-        h = x.detach().reshape(1, Bn * L).contiguous()
+        h = x.detach().permute(1, 0, 2).reshape(cin, Bn * L).contiguous()
         saved = []
         for i, spec in enumerate(net.specs):
             h, s = _conv_forward(K, spec, h, params[2 * i], params[2 * i + 1], Bn, L)
             saved.append(s)
-        ctx.net, ctx.saved, ctx.shape = net, saved, (Bn, L)
+        ctx.net, ctx.saved, ctx.shape = net, saved, (Bn, cin, L)
         ctx.params = params
         n_out = net.specs[-1].cout
         return h.reshape(n_out, Bn, L).permute(1, 0, 2).contiguous()
 
     @staticmethod
     def backward(ctx, g):
-        net, (Bn, L) = ctx.net, ctx.shape
+        net, (Bn, cin, L) = ctx.net, ctx.shape
         n_out = net.specs[-1].cout
         gy = g.permute(1, 0, 2).reshape(n_out, Bn * L).contiguous().float()
         grads = [None] * (2 * len(net.specs))
@@ -187,12 +188,16 @@ class _NetFn(torch.autograd.Function):
             gy, dW, db = _conv_backward(net.K, spec, ctx.saved[i], ctx.params[2 * i], gy, Bn, L, need_dx, splits,
                                         need_w)
             grads[2 * i], grads[2 * i + 1] = dW, db
-        gx = gy.reshape(Bn, L) if need_x else None
+        gx = gy.reshape(cin, Bn, L).permute(1, 0, 2) if need_x else None
         return (gx, None, *grads)
 
 
 class Surrogate(torch.nn.Module):
-    """Boundary series [B, L] -> gauge series [B, n_out, L] on BF16 MFMA kernels."""
+    """Boundary series [B, L] (+ per-sample conditions [B, n_cond]) -> gauge series [B, n_out, L].
+
+    Conditions (e.g. the grid resolution of the run that produced a pair) enter
+    as constant input channels, so one network serves every resolution of an
+    ensemble."""
 
     def __init__(self, specs=None, device="cuda", seed=0, in_scale=0.02, out_scale=0.02):
         super().__init__()
@@ -207,18 +212,25 @@ class Surrogate(torch.nn.Module):
             self.weights.append(torch.nn.Parameter(W.to(device)))
             self.weights.append(torch.nn.Parameter(torch.zeros(spec.cout, device=device)))
         self.in_scale, self.out_scale = in_scale, out_scale
+        self.n_cond = self.specs[0].cin - 1
 
     def n_params(self):
         return sum(p.numel() for p in self.weights)
 
-    def forward(self, series):
+    def forward(self, series, cond=None):
         """series: boundary levels [B, L] (metres) -> gauge h + z [B, n_out, L] (metres)."""
-        x = (series / self.in_scale).float().contiguous()
-        y = _NetFn.apply(x, self, *self.weights)
+        x = (series / self.in_scale).float()[:, None, :]
+        if self.n_cond:
+            if cond is None or cond.shape != (series.shape[0], self.n_cond):
+                raise ValueError(f"this surrogate needs conditions of shape ({series.shape[0]}, {self.n_cond})")
+            x = torch.cat((x, cond.float()[:, :, None].expand(-1, -1, series.shape[1])), 1)
+        elif cond is not None:
+            raise ValueError("this surrogate takes no conditions")
+        y = _NetFn.apply(x.contiguous(), self, *self.weights)
         return y * self.out_scale
 
 
-def train(net, series, targets, epochs=60, batch=64, lr=2e-3, log=None, time_budget=None, seed=0):
+def train(net, series, targets, epochs=60, batch=64, lr=2e-3, log=None, time_budget=None, seed=0, cond=None):
     """Adam on FP32 master weights; the MFMA kernels compute every forward and backward."""
     opt = torch.optim.Adam(net.weights, lr=lr)
     n = series.shape[0]
@@ -233,12 +245,12 @@ def train(net, series, targets, epochs=60, batch=64, lr=2e-3, log=None, time_bud
         for s in range(steps_per_epoch):
             idx = perm[s * batch:(s + 1) * batch]
             opt.zero_grad()
-            pred = net(series[idx])
+            pred = net(series[idx], None if cond is None else cond[idx])
             loss = ((pred - targets[idx]) / net.out_scale).square().mean()
             loss.backward()
             opt.step()
             sched.step()
-            tot += float(loss)
+            tot += float(loss.detach())
         hist.append((ep, time.perf_counter() - t0, tot / steps_per_epoch))
         if log:
             log(f"surrogate epoch {ep}: loss {tot / steps_per_epoch:.4e}")

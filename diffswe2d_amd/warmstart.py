@@ -4,8 +4,10 @@ Pipeline (no reference counterpart beyond the plain inversion it accelerates,
 Examples/inverse/.../bc_inversion_train.py):
 
 1. ``generate_pairs``: an ensemble of seeded scenarios (uneven grids, own dt,
-   csrc/swe_ensemble.hip) produces (boundary series, 6 gauge series) pairs.
-2. ``surrogate.train``: the BF16 MFMA 1-D conv surrogate learns series -> gauges.
+   csrc/swe_ensemble.hip) produces (boundary series, 6 gauge series) pairs,
+   sampled at the same physical gauge points at every resolution.
+2. ``surrogate.train``: the BF16 MFMA 1-D conv surrogate learns series -> gauges,
+   conditioned on the resolution (log2 of dx / 1 cm) of the run.
 3. ``surrogate_start``: the controls that make the surrogate match the observed
    gauges (Adam through the surrogate's own backward kernels).
 4. The plain solver judges that start: if its gauge misfit is not better than
@@ -23,37 +25,52 @@ from . import inversion, scenarios
 from .config import SWEConfig
 from .ensemble import Ensemble
 from .solver import SWE2D
-from .surrogate import Surrogate, train
+from .surrogate import Surrogate, default_specs, train
 
 
 def runup_config():
     return SWEConfig(boundary_left="water_level", boundary_right="wall", boundary_top="wall", boundary_bottom="wall")
 
 
+def resolution_condition(dx):
+    """Surrogate condition channel of a run on a grid of spacing dx (metres)."""
+    return float(np.log2(dx / 0.01))
+
+
 def generate_pairs(n, device, seed=5, n_controls=64, n_chunks=450, sample_dt=0.05, grids=scenarios.RUNUP_GRIDS,
                    log=None):
-    """Run the ensemble; returns (series [n, n_chunks], gauges [n, 6, n_chunks], controls, stats)."""
+    """Run the ensemble; returns (series [n, n_chunks], gauges [n, 6, n_chunks], controls, stats).
+
+    stats["cond"] holds each pair's resolution condition [n, 1]."""
     sc, ctrl = scenarios.ensemble_scenarios(n=n, seed=seed, n_controls=n_controls, n_chunks=n_chunks,
                                             sample_dt=sample_dt, grids=grids)
     t0 = time.perf_counter()
     ens = Ensemble(runup_config(), sc, device)
-    steps = ens.run()
+    steps = ens.run(log=log)
     sec = time.perf_counter() - t0
     res = ens.results()
     gauges = torch.stack([r["gauges"].T for r in res])  # [n, 6, n_chunks]
     series = torch.tensor(np.stack([s["levels"] for s in sc]), dtype=torch.float32, device=device)
     cells = sum(s["bed"].size * r["steps"] for s, r in zip(sc, res))
+    cond = torch.tensor([[resolution_condition(s["dx"])] for s in sc], dtype=torch.float32, device=device)
     stats = dict(scenarios=n, seconds=sec, global_steps=steps, cell_updates=cells,
-                 gcell_per_s=cells / sec / 1e9, per_scenario_steps=[r["steps"] for r in res])
+                 gcell_per_s=cells / sec / 1e9, per_scenario_steps=[r["steps"] for r in res], cond=cond)
     if log:
         log(f"ensemble: {n} scenarios, {cells / 1e9:.2f} G cell-updates in {sec:.2f} s "
             f"({stats['gcell_per_s']:.2f} Gcell/s)")
     return series, gauges, ctrl, stats
 
 
-def surrogate_start(net, basis, target, raw0, bounds=(-0.02, 0.05), iters=300, lr=0.05):
-    """Controls (raw) minimising the surrogate's gauge misfit; gradients from the MFMA backward."""
+def surrogate_start(net, basis, target, raw0, bounds=(-0.02, 0.05), iters=300, lr=0.05, dx=None):
+    """Controls (raw) minimising the surrogate's gauge misfit; gradients from the MFMA backward.
+
+    dx: grid spacing of the inverted model (needed when the surrogate is conditioned)."""
     raw = raw0.detach().clone().float().requires_grad_(True)
+    cond = None
+    if net.n_cond:
+        if dx is None:
+            raise ValueError("a conditioned surrogate needs the grid spacing dx")
+        cond = torch.tensor([[resolution_condition(dx)]], dtype=torch.float32, device=raw.device)
     opt = torch.optim.Adam([raw], lr=lr)
     b = basis.float()
     tgt = target.float()
@@ -64,7 +81,7 @@ def surrogate_start(net, basis, target, raw0, bounds=(-0.02, 0.05), iters=300, l
         for _ in range(iters):
             opt.zero_grad()
             series = b @ inversion.bounded_controls(raw, *bounds)
-            pred = net(series[None])[0].T  # [n_chunks, 6]
+            pred = net(series[None], cond)[0].T  # [n_chunks, 6]
             loss = ((pred - tgt) / net.out_scale).square().mean()
             loss.backward()
             opt.step()
@@ -82,7 +99,7 @@ class InversionSetup:
         p = scenarios.inversion_problem(seed=seed, nx=nx, ny=ny, dx=dx, n_controls=n_controls, n_chunks=n_chunks,
                                         sample_dt=sample_dt)
         base = p["base"]
-        self.device, self.dtype = device, dtype
+        self.device, self.dtype, self.dx = device, dtype, dx
         self.model = SWE2D(torch.tensor(base["bed"], dtype=dtype), torch.tensor(base["rough"], dtype=dtype), dx, dx,
                            config=runup_config(), train_dem=False).to(device)
         U0 = torch.tensor(base["U"], dtype=dtype, device=device)
@@ -124,15 +141,14 @@ def run(device, n_scenarios=256, n_controls=64, n_chunks=450, grids=scenarios.RU
                            steps_per_forward=setup.problem.stats.get("steps"))
     series, gauges, ctrl, stats = generate_pairs(n_scenarios, device, seed=seed, n_controls=n_controls,
                                                  n_chunks=n_chunks, grids=grids, log=log)
-    report["ensemble"] = {k: v for k, v in stats.items() if k != "per_scenario_steps"}
-    net = Surrogate(__import__("diffswe2d_amd.surrogate", fromlist=["default_specs"]).default_specs(channels=channels),
-                    device=device, seed=0)
+    report["ensemble"] = {k: v for k, v in stats.items() if k not in ("per_scenario_steps", "cond")}
+    net = Surrogate(default_specs(channels=channels, n_cond=1), device=device, seed=0)
     t1 = time.perf_counter()
-    hist_s = train(net, series, gauges, epochs=surrogate_epochs, batch=32, lr=2e-3,
+    hist_s = train(net, series, gauges, epochs=surrogate_epochs, batch=32, lr=2e-3, cond=stats["cond"],
                    log=(lambda m: log(m)) if log else None)
     report["surrogate"] = dict(params=net.n_params(), seconds=time.perf_counter() - t1, loss=hist_s[-1][2])
     t2 = time.perf_counter()
-    raw_ws = surrogate_start(net, setup.basis, setup.target, setup.prior_raw)
+    raw_ws = surrogate_start(net, setup.basis, setup.target, setup.prior_raw, dx=setup.dx)
     m_ws = setup.misfit(raw_ws)
     m_prior = hist_p[0][2]
     fallback = not (m_ws < accept * m_prior)

@@ -175,6 +175,22 @@ int swe_step_adjoint_f64(const swe_grid* grid, const swe_config* cfg, const doub
                          int level_stride, double dt, const double* lam_out, double* lam_U, double* lam_level,
                          void* adj_ws, int rows_hint, void* stream);
 
+/* Reverse sweep over one checkpoint segment in a single call: replays n_steps
+ * steps from U_ckpt with the recorded dts (host array; every replayed dt equals
+ * the recorded one) onto a caller tape of swe_segment_tape_bytes(), then applies
+ * the step adjoints last to first.  lam holds the adjoint of the segment's
+ * final state on entry and of U_ckpt on exit; lam_tmp is scratch of the same
+ * size; lam_level accumulates.  parity is the forward workspace's step parity. */
+size_t swe_segment_tape_bytes(const swe_grid* grid, int n_steps, int dtype_bytes);
+int swe_segment_adjoint_f32(const swe_grid* grid, const swe_config* cfg, const float* U_ckpt, const float* bed,
+                            const float* roughness, const float* level, int level_stride, const double* dts,
+                            int n_steps, int parity, float* lam, float* lam_tmp, float* lam_level, float* tape,
+                            void* fwd_ws, void* adj_ws, int rows_hint, void* stream);
+int swe_segment_adjoint_f64(const swe_grid* grid, const swe_config* cfg, const double* U_ckpt, const double* bed,
+                            const double* roughness, const double* level, int level_stride, const double* dts,
+                            int n_steps, int parity, double* lam, double* lam_tmp, double* lam_level, double* tape,
+                            void* fwd_ws, void* adj_ws, int rows_hint, void* stream);
+
 /* ---- ensemble of forward runs (FP32) -------------------------------------
  * Uneven grids, each with its own adaptive dt, advanced in lockstep by one
  * launch sequence (the reference batches members on one grid with one shared

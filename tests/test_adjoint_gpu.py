@@ -89,7 +89,8 @@ def test_step_adjoint(golden, cuda_device, ieee_oracle, case, dtype):
         assert abs(float(gl) - float(gl_ref)) <= tol * max(1.0, abs(float(gl_ref))) * 10
 
 
-def test_forward_adjoint_with_checkpoints_fp64(cuda_device, ieee_oracle):
+@pytest.mark.parametrize("verify", [False, True])
+def test_forward_adjoint_with_checkpoints_fp64(cuda_device, ieee_oracle, verify):
     """A 0.3 s forward call (many steps, K = 5) against oracle autograd through the same steps.
 
     The initial surface carries a smooth ripple: a lake exactly at rest has
@@ -104,6 +105,7 @@ def test_forward_adjoint_with_checkpoints_fp64(cuda_device, ieee_oracle):
     rough = torch.tensor(sc["rough"])
     m = swe.SWE2D(bed, rough, 0.136, 0.136, config=cfg, train_dem=False).to(cuda_device)
     m.adjoint_checkpoint_every = 5
+    m.adjoint_verify_replay = verify  # True: per-step replay with dt read-back; False: one call per segment
     U0 = torch.tensor(sc["U"])
     yy, xx = torch.meshgrid(torch.arange(26, dtype=torch.float64), torch.arange(40, dtype=torch.float64),
                             indexing="ij")

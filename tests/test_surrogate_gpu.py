@@ -29,7 +29,7 @@ def test_gemm_nt_matches_fp32_matmul(cuda_device, M, N, K, splits):
 
 def torch_net(specs, params, x):
     """CPU reference: each layer on bf16-rounded input and weights, FP32 math."""
-    h = x[:, None, :]
+    h = x[:, None, :] if x.dim() == 2 else x
     for i, sp in enumerate(specs):
         W, b = params[2 * i], params[2 * i + 1]
         y = F.conv1d(bf(h), bf(W), b, padding=sp.dil * (sp.kw - 1) // 2, dilation=sp.dil)
@@ -39,20 +39,23 @@ def torch_net(specs, params, x):
     return h
 
 
-def test_network_forward_and_gradients(cuda_device):
-    specs = S.default_specs(channels=32, dilations=(1, 4, 16), n_out=6)
+@pytest.mark.parametrize("n_cond", [0, 1])
+def test_network_forward_and_gradients(cuda_device, n_cond):
+    specs = S.default_specs(channels=32, dilations=(1, 4, 16), n_out=6, n_cond=n_cond)
     net = S.Surrogate(specs, device=cuda_device, seed=3)
     Bn, L = 8, 450
     g = torch.Generator().manual_seed(1)
     x = torch.randn(Bn, L, generator=g) * 0.02
     tgt = torch.randn(Bn, 6, L, generator=g) * 0.02
+    cond = torch.arange(Bn, dtype=torch.float32)[:, None] * 0.5 - 1.0 if n_cond else None
     xg = x.to(cuda_device).requires_grad_(True)
-    y = net(xg)
+    y = net(xg, None if cond is None else cond.to(cuda_device))
     loss = ((y - tgt.to(cuda_device)) / 0.02).square().mean()
     loss.backward()
     params = [p.detach().cpu().clone().requires_grad_(True) for p in net.weights]
     xr = (x / 0.02).clone().requires_grad_(True)
-    yr = torch_net(specs, params, xr) * 0.02
+    xin = xr if cond is None else torch.cat((xr[:, None, :], cond[:, :, None].expand(-1, -1, L)), 1)
+    yr = torch_net(specs, params, xin) * 0.02
     lr = ((yr - tgt) / 0.02).square().mean()
     lr.backward()
     ef = float((y.detach().cpu() - yr.detach()).abs().max() / yr.abs().max())
@@ -79,6 +82,15 @@ def test_training_reduces_loss(cuda_device):
     hist = S.train(net, x.to(cuda_device), y.to(cuda_device), epochs=40, batch=16, lr=3e-3)
     print(f"loss {hist[0][2]:.4e} -> {hist[-1][2]:.4e}")
     assert hist[-1][2] < 0.2 * hist[0][2]
+
+
+def test_condition_shape_is_checked(cuda_device):
+    net = S.Surrogate(S.default_specs(channels=16, dilations=(1,), n_cond=1), device=cuda_device)
+    with pytest.raises(ValueError):
+        net(torch.zeros(2, 32, device=cuda_device))
+    plain = S.Surrogate(S.default_specs(channels=16, dilations=(1,)), device=cuda_device)
+    with pytest.raises(ValueError):
+        plain(torch.zeros(2, 32, device=cuda_device), torch.zeros(2, 1, device=cuda_device))
 
 
 def test_parameter_count_default():

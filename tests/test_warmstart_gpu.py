@@ -19,12 +19,12 @@ def test_surrogate_warm_start_beats_prior(cuda_device):
     series, gauges, ctrl, stats = warmstart.generate_pairs(48, cuda_device, seed=11, n_controls=n_ctrl,
                                                           n_chunks=n_chunks, grids=grids, log=print)
     assert torch.isfinite(gauges).all()
-    net = Surrogate(default_specs(channels=64, dilations=(1, 2, 4, 8, 16, 32)), device=cuda_device, seed=0)
-    hist = train(net, series, gauges, epochs=80, batch=16, lr=3e-3)
+    net = Surrogate(default_specs(channels=64, dilations=(1, 2, 4, 8, 16, 32), n_cond=1), device=cuda_device, seed=0)
+    hist = train(net, series, gauges, epochs=80, batch=16, lr=3e-3, cond=stats["cond"])
     print(f"surrogate: {net.n_params()} params, loss {hist[0][2]:.3e} -> {hist[-1][2]:.3e}")
     setup = warmstart.InversionSetup(cuda_device, n_controls=n_ctrl, n_chunks=n_chunks, nx=136, ny=86, dx=0.04)
     m_prior = setup.misfit(setup.prior_raw)
-    raw_ws = warmstart.surrogate_start(net, setup.basis, setup.target, setup.prior_raw, iters=200)
+    raw_ws = warmstart.surrogate_start(net, setup.basis, setup.target, setup.prior_raw, iters=200, dx=setup.dx)
     m_ws = setup.misfit(raw_ws)
     print(f"plain-solver misfit: prior {m_prior:.4e}, surrogate start {m_ws:.4e}")
     assert m_ws < 0.7 * m_prior
