@@ -73,6 +73,12 @@ def inverse_bounded(value: float, lo: float, hi: float, like: torch.Tensor) -> t
     return torch.logit(frac.clamp(eps, 1.0 - eps))
 
 
+def raw_from_controls(values: torch.Tensor, lo: float, hi: float) -> torch.Tensor:
+    """Raw parameters whose bounded_controls equal values (clamped into the open interval)."""
+    eps = torch.finfo(values.dtype).eps
+    return torch.logit(((values - lo) / (hi - lo)).clamp(eps, 1.0 - eps))
+
+
 def smoothness(series: torch.Tensor):
     d1 = series[1:] - series[:-1]
     d2 = series[2:] - 2.0 * series[1:-1] + series[:-2]
@@ -146,10 +152,11 @@ def adam_invert(problem: GaugeProblem, basis: torch.Tensor, target: torch.Tensor
         opt.step()
         torch.cuda.synchronize() if raw.is_cuda else None
         el = time.perf_counter() - t0
-        hist.append((ep, el, float(misfit)))
+        m = float(misfit.detach())
+        hist.append((ep, el, m))
         if log:
-            log(f"epoch {ep}: misfit {float(misfit):.4e} ({el:.2f} s)")
-        if target_misfit is not None and float(misfit) <= target_misfit:
+            log(f"epoch {ep}: misfit {m:.4e} ({el:.2f} s)")
+        if target_misfit is not None and m <= target_misfit:
             break
         if time_budget is not None and el > time_budget:
             break

@@ -61,34 +61,57 @@ def generate_pairs(n, device, seed=5, n_controls=64, n_chunks=450, sample_dt=0.0
     return series, gauges, ctrl, stats
 
 
-def surrogate_start(net, basis, target, raw0, bounds=(-0.02, 0.05), iters=300, lr=0.05, dx=None):
+def surrogate_start(net, basis, target, raw0, bounds=(-0.02, 0.05), iters=300, lr=0.05, dx=None, info=None):
     """Controls (raw) minimising the surrogate's gauge misfit; gradients from the MFMA backward.
 
-    dx: grid spacing of the inverted model (needed when the surrogate is conditioned)."""
-    raw = raw0.detach().clone().float().requires_grad_(True)
+    raw0 is one start [n_controls] or several [k, n_controls]; several starts
+    are refined together as one batch through the surrogate and the one with
+    the lowest surrogate misfit is returned.  dx: grid spacing of the inverted
+    model (needed when the surrogate is conditioned).  ``info`` (a dict, if
+    given) receives the per-start surrogate misfits before and after."""
+    starts = raw0.detach().float().reshape(-1, raw0.shape[-1]).clone().requires_grad_(True)
+    k = starts.shape[0]
     cond = None
     if net.n_cond:
         if dx is None:
             raise ValueError("a conditioned surrogate needs the grid spacing dx")
-        cond = torch.tensor([[resolution_condition(dx)]], dtype=torch.float32, device=raw.device)
-    opt = torch.optim.Adam([raw], lr=lr)
-    b = basis.float()
-    tgt = target.float()
+        cond = torch.full((k, net.n_cond), resolution_condition(dx), dtype=torch.float32, device=starts.device)
+    opt = torch.optim.Adam([starts], lr=lr)
+    bT = basis.float().T.contiguous()  # [n_controls, n_chunks]
+    tgt = target.float().T[None]  # [1, 6, n_chunks]
+
+    def per_start():
+        pred = net(inversion.bounded_controls(starts, *bounds) @ bT, cond)  # [k, 6, n_chunks]
+        return ((pred - tgt) / net.out_scale).square().mean(dim=(1, 2))
+
     flags = [p.requires_grad for p in net.weights]
     for p in net.weights:
         p.requires_grad_(False)  # only the input gradient is needed
     try:
+        with torch.no_grad():
+            first = per_start()
         for _ in range(iters):
             opt.zero_grad()
-            series = b @ inversion.bounded_controls(raw, *bounds)
-            pred = net(series[None], cond)[0].T  # [n_chunks, 6]
-            loss = ((pred - tgt) / net.out_scale).square().mean()
-            loss.backward()
+            per_start().sum().backward()  # the starts are independent: the sum's gradient is each one's own
             opt.step()
+        with torch.no_grad():
+            last = per_start()
     finally:
         for p, f in zip(net.weights, flags):
             p.requires_grad_(f)
-    return raw.detach().to(raw0.dtype)
+    if info is not None:
+        info["surrogate_misfit_before"] = first.tolist()
+        info["surrogate_misfit_after"] = last.tolist()
+    return starts[int(torch.argmin(last))].detach().to(raw0.dtype)
+
+
+def nearest_pairs(gauges, target, k=8, mask=None):
+    """Indices of the k ensemble pairs whose gauge series [n, 6, n_chunks] best match target [n_chunks, 6]."""
+    mis = (gauges - target.T[None].to(gauges.dtype)).square().mean(dim=(1, 2))
+    if mask is not None:
+        mis = torch.where(mask, mis, torch.full_like(mis, float("inf")))
+    order = torch.argsort(mis)
+    return order[:min(k, int(torch.isfinite(mis).sum()))], mis
 
 
 class InversionSetup:
@@ -120,7 +143,7 @@ class InversionSetup:
 
 
 def run(device, n_scenarios=256, n_controls=64, n_chunks=450, grids=scenarios.RUNUP_GRIDS, epochs_plain=60,
-        surrogate_epochs=40, channels=160, accept=0.5, log=print, seed=5):
+        surrogate_epochs=40, channels=160, accept=0.5, log=print, seed=5, n_starts=8):
     """Full pipeline; returns a report with the time-to-misfit of both inversions.
 
     The plain inversion starts from the prior (level 0) and runs ``epochs_plain``
@@ -148,7 +171,14 @@ def run(device, n_scenarios=256, n_controls=64, n_chunks=450, grids=scenarios.RU
                    log=(lambda m: log(m)) if log else None)
     report["surrogate"] = dict(params=net.n_params(), seconds=time.perf_counter() - t1, loss=hist_s[-1][2])
     t2 = time.perf_counter()
-    raw_ws = surrogate_start(net, setup.basis, setup.target, setup.prior_raw, dx=setup.dx)
+    # starts: the prior and the ensemble pairs whose gauges already resemble the
+    # observations (same resolution), all refined through the surrogate at once
+    same_grid = (stats["cond"][:, 0] == resolution_condition(setup.dx))
+    idx, pair_mis = nearest_pairs(gauges, setup.target, k=n_starts, mask=same_grid)
+    ctrl_t = torch.as_tensor(ctrl, dtype=setup.dtype, device=device)[idx]
+    starts = torch.cat((setup.prior_raw[None], inversion.raw_from_controls(ctrl_t, -0.02, 0.05)))
+    info = {}
+    raw_ws = surrogate_start(net, setup.basis, setup.target, starts, dx=setup.dx, info=info)
     m_ws = setup.misfit(raw_ws)
     m_prior = hist_p[0][2]
     fallback = not (m_ws < accept * m_prior)
@@ -161,6 +191,7 @@ def run(device, n_scenarios=256, n_controls=64, n_chunks=450, grids=scenarios.RU
     reached = min(h[2] for h in hist_w) <= bar
     # time at which the plain run first reached the warm run's misfit levels, for the curve
     report["warm"] = dict(seconds_to_bar=t_warm if reached else None, start_seconds=t_start,
+                          nearest_pair_gauge_misfit=float(pair_mis[idx[0]]) if len(idx) else None, **info,
                           start_misfit=m_ws, prior_misfit=m_prior, fallback=fallback, epochs=len(hist_w),
                           history=hist_w, reached=reached)
     report["ratio_warm_over_plain"] = (t_warm / t_plain) if reached else None

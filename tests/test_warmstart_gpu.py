@@ -28,6 +28,19 @@ def test_surrogate_warm_start_beats_prior(cuda_device):
     m_ws = setup.misfit(raw_ws)
     print(f"plain-solver misfit: prior {m_prior:.4e}, surrogate start {m_ws:.4e}")
     assert m_ws < 0.7 * m_prior
+    # several starts (prior + nearest ensemble pairs on the same grid) refined as one batch
+    same = stats["cond"][:, 0] == warmstart.resolution_condition(setup.dx)
+    idx, mis = warmstart.nearest_pairs(gauges, setup.target, k=4, mask=same)
+    assert len(idx) == 4 and bool(same[idx].all())
+    starts = torch.cat((setup.prior_raw[None],
+                        inversion.raw_from_controls(torch.as_tensor(ctrl, device=cuda_device)[idx].float(), -0.02, 0.05)))
+    info = {}
+    raw_multi = warmstart.surrogate_start(net, setup.basis, setup.target, starts, iters=200, dx=setup.dx, info=info)
+    assert len(info["surrogate_misfit_after"]) == 5
+    assert min(info["surrogate_misfit_after"]) <= info["surrogate_misfit_after"][0]
+    m_multi = setup.misfit(raw_multi)
+    print(f"multi-start surrogate misfits {info['surrogate_misfit_after']}; plain-solver misfit {m_multi:.4e}")
+    assert m_multi < 0.7 * m_prior
     # and the adjoint improves on it further
     raw, hist = inversion.adam_invert(setup.problem, setup.basis, setup.target, raw_ws, epochs=3, lr=0.02)
     assert hist[-1][2] <= m_ws * 1.01
