@@ -1,12 +1,15 @@
 // Hand-written adjoint of one time step on gfx950.
 //
 // The step U -> U' (solver.py:572-663) with dt held fixed (the reference
-// detaches dt, :625-626) is reversed by six launches:
-//   adj_final   friction and corrector clean-up reversed     -> lam_k2, lam_acc
-//   adj_L<2>    L^T at the predictor P = clean(U, U + dt/2 k1) -> lam_P (+ ghosts)
+// detaches dt, :625-626) is reversed by two L^T sweeps, each followed by the
+// ghost fold (four per-side launches and a level reduction):
+//   adj_L<2>    friction and corrector clean-up reversed on load (-> lam_k2 in
+//               registers, lam_acc written by the owning lane), then L^T at the
+//               predictor P = clean(U, U + dt/2 k1)             -> lam_P (+ ghosts)
 //   adj_ghost   ghost-cell adjoints folded back onto the edge cells; level
-//   adj_mid     predictor clean-up reversed                  -> lam_k1, lam_acc
-//   adj_L<1>    L^T at U                                     -> lam_U = lam_acc + ...
+//   adj_L<1>    predictor clean-up reversed on load (-> lam_k1 in registers,
+//               lam_acc updated by the owning lane), then L^T at U
+//                                                              -> lam_U = lam_acc + ...
 //   adj_ghost   as above
 // L^T mirrors the forward stage's decomposition: one wavefront streams down a
 // strip of 64 columns, now with a 3-cell halo (58 outputs): a cell's adjoint
@@ -44,6 +47,15 @@ template <class T> struct AdjArgs {
   T* out;                // adjoint of the stage input state [B,3,ny,nx]
   T* ghost;              // [B][4 sides][2 layers][lmax][3]
   int lmax;
+  // fused elementwise reverse on load (FUSE 1: corrector + friction, FUSE 2: predictor)
+  const T* k2;           // second-stage tendency (FUSE 1)
+  const T* lam_out;      // adjoint of the step output (FUSE 1)
+  const T* lam_P;        // adjoint of the predictor state (FUSE 2)
+  T dt;
+  int friction, friction_model;
+  T* acc;                // adjoint reaching U directly: written (FUSE 1) / updated (FUSE 2) by the owning lane
+  T* part;               // per-block level-gradient partials of the ghost fold
+  int part_blocks;       // blocks per side in the ghost fold
 };
 
 template <class T> struct Lam3 {
@@ -100,6 +112,70 @@ __device__ inline int64_t ghost_slot(int b, int r, int c, int ny, int nx, int lm
   return ((((int64_t)b * 4 + side) * 2 + layer) * lmax + i) * 3;
 }
 
+// friction and the corrector clean-up reversed at interior cell i (solver.py:646-661):
+// lam = adjoint of k2, acc = adjoint reaching U directly
+template <class T>
+__device__ inline void corrector_bwd(const AdjArgs<T>& a, int b, int64_t cell, int64_t i, Lam3<T>& lam, Lam3<T>& acc) {
+  const SweConsts<T>& k = a.k;
+  const int64_t P = a.f.plane;
+  const T dt = a.dt;
+  T oh = a.U[i], ou = a.U[i + P], ov = a.U[i + 2 * P];
+  T ch = oh + dt * a.k2[i], cu = ou + dt * a.k2[i + P], cv = ov + dt * a.k2[i + 2 * P];
+  T gh = a.lam_out[i], gu = a.lam_out[i + P], gv = a.lam_out[i + 2 * P];
+  if (a.friction) {
+    T vh = ch, vu = cu, vv = cv;
+    clean_and_limit(oh, ou, ov, vh, vu, vv, k, a.positivity != 0);
+    T coef = a.f.rough[b * a.f.rough_bstride + cell];
+    friction_bwd(vh, vu, vv, coef, dt, k, a.friction_model, gh, gu, gv, gh, gu, gv);
+  }
+  T go_h = 0, go_u = 0, go_v = 0, gc_h, gc_u, gc_v;
+  clean_and_limit_bwd(oh, ou, ov, ch, cu, cv, k, a.positivity != 0, gh, gu, gv, go_h, go_u, go_v, gc_h, gc_u, gc_v);
+  acc = Lam3<T>{go_h + gc_h, go_u + gc_u, go_v + gc_v};
+  lam = Lam3<T>{dt * gc_h, dt * gc_u, dt * gc_v};
+}
+
+// predictor clean-up reversed at interior cell i (solver.py:639)
+template <class T>
+__device__ inline void predictor_bwd(const AdjArgs<T>& a, int64_t i, Lam3<T>& lam, Lam3<T>& acc) {
+  const int64_t P = a.f.plane;
+  const T half = a.half_dt;
+  T oh = a.U[i], ou = a.U[i + P], ov = a.U[i + 2 * P];
+  T ch = oh + half * a.k1[i], cu = ou + half * a.k1[i + P], cv = ov + half * a.k1[i + 2 * P];
+  T go_h = 0, go_u = 0, go_v = 0, gc_h, gc_u, gc_v;
+  clean_and_limit_bwd(oh, ou, ov, ch, cu, cv, a.k, a.positivity != 0, a.lam_P[i], a.lam_P[i + P], a.lam_P[i + 2 * P],
+                      go_h, go_u, go_v, gc_h, gc_u, gc_v);
+  acc = Lam3<T>{go_h + gc_h, go_u + gc_u, go_v + gc_v};
+  lam = Lam3<T>{half * gc_h, half * gc_u, half * gc_v};
+}
+
+// adjoint of the stage tendency at padded cell (n, col); with FUSE it is built from
+// the elementwise reverse, whose direct part the owning lane stores in a.acc
+template <class T, int FUSE>
+__device__ inline Lam3<T> stage_lam(const AdjArgs<T>& a, int b, int n, int col, bool own) {
+  if (FUSE == 0) return load_lam(a, b, n, col);
+  if (n < 0 || n >= a.f.ny || col < 0 || col >= a.f.nx) return Lam3<T>{T(0), T(0), T(0)};
+  const int64_t cell = (int64_t)n * a.f.nx + col;
+  const int64_t i = (int64_t)b * 3 * a.f.plane + cell;
+  const int64_t P = a.f.plane;
+  Lam3<T> lam, acc;
+  if (FUSE == 1) {
+    corrector_bwd(a, b, cell, i, lam, acc);
+    if (own) {
+      a.acc[i] = acc.h;
+      a.acc[i + P] = acc.u;
+      a.acc[i + 2 * P] = acc.v;
+    }
+  } else {
+    predictor_bwd(a, i, lam, acc);
+    if (own) {
+      a.acc[i] += acc.h;
+      a.acc[i + P] += acc.u;
+      a.acc[i + 2 * P] += acc.v;
+    }
+  }
+  return lam;
+}
+
 // x sweep of one row (all lanes take part): adjoint of the primitives of each
 // lane's cell from the x-faces of the row.
 template <class T>
@@ -132,8 +208,14 @@ __device__ inline PrimGrad<T> x_sweep_bwd(const AdjArgs<T>& a, const Prim<T>& P,
   return own;
 }
 
-template <class T, int STAGE>
-__global__ __launch_bounds__(256) void adj_L_kernel(AdjArgs<T> a) {
+// FP32: 3 waves per SIMD (at most 168 VGPRs) instead of the 2 the unconstrained
+// allocation (174) leaves; the few values this spills cost far less than the
+// latency the third wave hides.  FP64 keeps its natural allocation.
+template <class T> constexpr int adj_waves_per_eu() { return sizeof(T) == 4 ? 3 : 1; }
+
+template <class T, int STAGE, int FUSE>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(adj_waves_per_eu<T>())))
+void adj_L_kernel(AdjArgs<T> a) {
   const int lane = threadIdx.x & (kWave - 1);
   const int wid = blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6);
   const int b = blockIdx.y;
@@ -160,7 +242,8 @@ __global__ __launch_bounds__(256) void adj_L_kernel(AdjArgs<T> a) {
   for (int it = 0; it < rows + 6; ++it) {
     const int n = y0 - kHaloA + it;
     Prim<T> Pn = fetch<T, STAGE, AdjArgs<T>>(a, b, n, col, half);
-    Lam3<T> Ln = load_lam(a, b, n, col);
+    // the lane that emits row n (three iterations later) owns its elementwise reverse
+    Lam3<T> Ln = stage_lam<T, FUSE>(a, b, n, col, lane_out && n >= y0 && n < y0 + rows);
     Incr<T> yi_cur{};
     bool reset_cur = false;
     PrimGrad<T> An{T(0), T(0), T(0), T(0)};
@@ -232,104 +315,73 @@ __global__ __launch_bounds__(256) void adj_L_kernel(AdjArgs<T> a) {
   }
 }
 
-// Fold the ghost-cell adjoints back onto the cells they were copied from and
-// accumulate the boundary-level gradient (one block per member; sides in turn,
-// so no two threads touch the same cell).
+// Fold the ghost-cell adjoints of one side back onto the cells they were
+// copied from and form per-block partials of the boundary-level gradient.
+// One thread per position along the side (both ghost layers, in order); the
+// four sides run as four launches in turn, so a cell reached from two sides
+// (near a corner) is updated in a fixed order and no two threads of a launch
+// touch the same cell.
 template <class T>
-__global__ __launch_bounds__(1024) void adj_ghost_kernel(AdjArgs<T> a, int stage, T* lam_level) {
-  __shared__ T part[1024];
+__global__ __launch_bounds__(256) void adj_ghost_side_kernel(AdjArgs<T> a, int stage, int side) {
+  __shared__ T part[256];
   const Fields<T>& f = a.f;
-  const int b = blockIdx.x;
+  const int b = blockIdx.y;
   const int64_t base = (int64_t)b * 3 * f.plane;
+  const int len = (side == LEFT || side == RIGHT) ? f.ny : f.nx;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
   T glev_sum = T(0);
-  for (int side = 0; side < 4; ++side) {
-    int len = (side == LEFT || side == RIGHT) ? f.ny : f.nx;
-    for (int i = threadIdx.x; i < len; i += blockDim.x) {
-      for (int layer = 0; layer < 2; ++layer) {
-        int r, c;
-        if (side == LEFT) { r = i; c = -1 - layer; }
-        else if (side == RIGHT) { r = i; c = f.nx + layer; }
-        else if (side == TOP) { r = -1 - layer; c = i; }
-        else { r = f.ny + layer; c = i; }
-        GhostMap g = ghost_map(r, c, f.ny, f.nx, f.bc);
-        int64_t gs = ((((int64_t)b * 4 + side) * 2 + layer) * a.lmax + i) * 3;
-        T gh = a.ghost[gs], gu = a.ghost[gs + 1], gv = a.ghost[gs + 2];
-        T eh, eu, ev;
-        if (stage == 2) load_state<T, 2, AdjArgs<T>>(a, base, g.r, g.c, a.half_dt, eh, eu, ev);
-        else load_state<T, 1, AdjArgs<T>>(a, base, g.r, g.c, a.half_dt, eh, eu, ev);
-        T z = f.bed[(int64_t)b * f.bed_bstride + (int64_t)g.r * f.nx + g.c];
-        T level = (g.kind == BC_WATER_LEVEL || g.kind == BC_ABSORBING) ? f.level[b * f.level_stride] : T(0);
-        T ge_h, ge_u, ge_v, gmir, glev;
-        ghost_bwd(g, eh, z, level, gh, gu, gv, a.k, ge_h, ge_u, ge_v, gmir, glev);
-        int64_t e = base + (int64_t)g.r * f.nx + g.c;
-        a.out[e] += ge_h;
-        a.out[e + f.plane] += ge_u;
-        a.out[e + 2 * f.plane] += ge_v;
-        if (gmir != T(0)) {
-          int64_t m = base + (int64_t)g.mr * f.nx + g.mc;
-          if (side == LEFT || side == RIGHT) a.out[m + f.plane] += gmir;
-          else a.out[m + 2 * f.plane] += gmir;
-        }
-        glev_sum += glev;
+  if (i < len) {
+    for (int layer = 0; layer < 2; ++layer) {
+      int r, c;
+      if (side == LEFT) { r = i; c = -1 - layer; }
+      else if (side == RIGHT) { r = i; c = f.nx + layer; }
+      else if (side == TOP) { r = -1 - layer; c = i; }
+      else { r = f.ny + layer; c = i; }
+      GhostMap g = ghost_map(r, c, f.ny, f.nx, f.bc);
+      int64_t gs = ((((int64_t)b * 4 + side) * 2 + layer) * a.lmax + i) * 3;
+      T gh = a.ghost[gs], gu = a.ghost[gs + 1], gv = a.ghost[gs + 2];
+      T eh, eu, ev;
+      if (stage == 2) load_state<T, 2, AdjArgs<T>>(a, base, g.r, g.c, a.half_dt, eh, eu, ev);
+      else load_state<T, 1, AdjArgs<T>>(a, base, g.r, g.c, a.half_dt, eh, eu, ev);
+      T z = f.bed[(int64_t)b * f.bed_bstride + (int64_t)g.r * f.nx + g.c];
+      T level = (g.kind == BC_WATER_LEVEL || g.kind == BC_ABSORBING) ? f.level[b * f.level_stride] : T(0);
+      T ge_h, ge_u, ge_v, gmir, glev;
+      ghost_bwd(g, eh, z, level, gh, gu, gv, a.k, ge_h, ge_u, ge_v, gmir, glev);
+      int64_t e = base + (int64_t)g.r * f.nx + g.c;
+      a.out[e] += ge_h;
+      a.out[e + f.plane] += ge_u;
+      a.out[e + 2 * f.plane] += ge_v;
+      if (gmir != T(0)) {
+        int64_t m = base + (int64_t)g.mr * f.nx + g.mc;
+        if (side == LEFT || side == RIGHT) a.out[m + f.plane] += gmir;
+        else a.out[m + 2 * f.plane] += gmir;
       }
+      glev_sum += glev;
     }
-    __syncthreads();
   }
   part[threadIdx.x] = glev_sum;
   __syncthreads();
-  for (int s = blockDim.x / 2; s > 0; s >>= 1) {
-    if (threadIdx.x < s) part[threadIdx.x] += part[threadIdx.x + s];
+  for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+    if (threadIdx.x < w) part[threadIdx.x] += part[threadIdx.x + w];
     __syncthreads();
   }
-  if (threadIdx.x == 0 && lam_level != nullptr) lam_level[b] += part[0];
+  if (threadIdx.x == 0) a.part[((int64_t)b * 4 + side) * a.part_blocks + blockIdx.x] = part[0];
 }
 
-// friction and the corrector clean-up reversed (solver.py:646-661)
+// lam_level[b] += sum of the member's ghost-fold partials, in a fixed order
 template <class T>
-__global__ void adj_final_kernel(SweConsts<T> k, Fields<T> f, const T* U, const T* k2, T dt, int positivity,
-                                 int friction, int model, const T* lam_out, T* lam_k2, T* lam_acc, int64_t n) {
-  int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  if (t >= n) return;
-  int64_t b = t / f.plane, cell = t - b * f.plane;
-  int64_t i = b * 3 * f.plane + cell, P = f.plane;
-  T oh = U[i], ou = U[i + P], ov = U[i + 2 * P];
-  T ch = oh + dt * k2[i], cu = ou + dt * k2[i + P], cv = ov + dt * k2[i + 2 * P];
-  T gh = lam_out[i], gu = lam_out[i + P], gv = lam_out[i + 2 * P];
-  if (friction) {
-    T vh = ch, vu = cu, vv = cv;
-    clean_and_limit(oh, ou, ov, vh, vu, vv, k, positivity != 0);
-    T coef = f.rough[b * f.rough_bstride + cell];
-    friction_bwd(vh, vu, vv, coef, dt, k, model, gh, gu, gv, gh, gu, gv);
+__global__ __launch_bounds__(256) void adj_level_reduce_kernel(const T* part, int n_part, T* lam_level) {
+  __shared__ T acc[256];
+  const int b = blockIdx.x;
+  T s = T(0);
+  for (int j = threadIdx.x; j < n_part; j += blockDim.x) s += part[(int64_t)b * n_part + j];
+  acc[threadIdx.x] = s;
+  __syncthreads();
+  for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+    if (threadIdx.x < w) acc[threadIdx.x] += acc[threadIdx.x + w];
+    __syncthreads();
   }
-  T go_h = 0, go_u = 0, go_v = 0, gc_h, gc_u, gc_v;
-  clean_and_limit_bwd(oh, ou, ov, ch, cu, cv, k, positivity != 0, gh, gu, gv, go_h, go_u, go_v, gc_h, gc_u, gc_v);
-  lam_acc[i] = go_h + gc_h;
-  lam_acc[i + P] = go_u + gc_u;
-  lam_acc[i + 2 * P] = go_v + gc_v;
-  lam_k2[i] = dt * gc_h;
-  lam_k2[i + P] = dt * gc_u;
-  lam_k2[i + 2 * P] = dt * gc_v;
-}
-
-// predictor clean-up reversed (solver.py:639)
-template <class T>
-__global__ void adj_mid_kernel(SweConsts<T> k, Fields<T> f, const T* U, const T* k1, T half, int positivity,
-                               const T* lam_P, T* lam_k1, T* lam_acc, int64_t n) {
-  int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  if (t >= n) return;
-  int64_t b = t / f.plane, cell = t - b * f.plane;
-  int64_t i = b * 3 * f.plane + cell, P = f.plane;
-  T oh = U[i], ou = U[i + P], ov = U[i + 2 * P];
-  T ch = oh + half * k1[i], cu = ou + half * k1[i + P], cv = ov + half * k1[i + 2 * P];
-  T go_h = 0, go_u = 0, go_v = 0, gc_h, gc_u, gc_v;
-  clean_and_limit_bwd(oh, ou, ov, ch, cu, cv, k, positivity != 0, lam_P[i], lam_P[i + P], lam_P[i + 2 * P], go_h,
-                      go_u, go_v, gc_h, gc_u, gc_v);
-  lam_acc[i] += go_h + gc_h;
-  lam_acc[i + P] += go_u + gc_u;
-  lam_acc[i + 2 * P] += go_v + gc_v;
-  lam_k1[i] = half * gc_h;
-  lam_k1[i + P] = half * gc_u;
-  lam_k1[i + 2 * P] = half * gc_v;
+  if (threadIdx.x == 0) lam_level[b] += acc[0];
 }
 
 }  // namespace swe
@@ -348,29 +400,51 @@ int afail(int code, const char* msg) {
 
 inline size_t al256(size_t v) { return (v + 255) & ~(size_t)255; }
 
+constexpr int kGhostThreads = 256;
+
 template <class T> struct AdjWs {
-  T *lam_k2, *lam_acc, *lam_P, *lam_k1, *ghost;
-  int lmax;
+  T *lam_acc, *lam_P, *ghost, *part;
+  int lmax, part_blocks;
 };
 
+inline int ghost_blocks(const swe_grid* g) {
+  int lmax = g->ny > g->nx ? g->ny : g->nx;
+  return (lmax + kGhostThreads - 1) / kGhostThreads;
+}
+
+// [lam_acc | lam_P | ghost adjoints | level partials], each 256-byte aligned
 template <class T> size_t adj_ws_bytes(const swe_grid* g) {
   size_t state = (size_t)g->batch * 3 * (size_t)g->ny * g->nx * sizeof(T);
   int lmax = g->ny > g->nx ? g->ny : g->nx;
   size_t ghost = (size_t)g->batch * 4 * 2 * lmax * 3 * sizeof(T);
-  return 4 * al256(state) + al256(ghost);
+  size_t part = (size_t)g->batch * 4 * ghost_blocks(g) * sizeof(T);
+  return 2 * al256(state) + al256(ghost) + al256(part);
 }
 
 template <class T> AdjWs<T> adj_ws(void* ws, const swe_grid* g) {
   size_t state = al256((size_t)g->batch * 3 * (size_t)g->ny * g->nx * sizeof(T));
+  int lmax = g->ny > g->nx ? g->ny : g->nx;
+  size_t ghost = al256((size_t)g->batch * 4 * 2 * lmax * 3 * sizeof(T));
   char* p = (char*)ws;
   AdjWs<T> w;
-  w.lam_k2 = (T*)p;
-  w.lam_acc = (T*)(p + state);
-  w.lam_P = (T*)(p + 2 * state);
-  w.lam_k1 = (T*)(p + 3 * state);
-  w.ghost = (T*)(p + 4 * state);
-  w.lmax = g->ny > g->nx ? g->ny : g->nx;
+  w.lam_acc = (T*)p;
+  w.lam_P = (T*)(p + state);
+  w.ghost = (T*)(p + 2 * state);
+  w.part = (T*)(p + 2 * state + ghost);
+  w.lmax = lmax;
+  w.part_blocks = ghost_blocks(g);
   return w;
+}
+
+// the ghost fold of one L^T sweep: four side launches, then the level reduction
+template <class T>
+void ghost_fold(const AdjArgs<T>& a, int stage, int batch, T* lam_level, hipStream_t s) {
+  dim3 grid((unsigned)a.part_blocks, (unsigned)batch);
+  for (int side = 0; side < 4; ++side)
+    hipLaunchKernelGGL((adj_ghost_side_kernel<T>), grid, dim3(kGhostThreads), 0, s, a, stage, side);
+  if (lam_level != nullptr)
+    hipLaunchKernelGGL((adj_level_reduce_kernel<T>), dim3(batch), dim3(256), 0, s, a.part, 4 * a.part_blocks,
+                       lam_level);
 }
 
 
@@ -422,28 +496,32 @@ int step_adjoint_impl(const swe_grid* g, const swe_config* c, const T* U, const 
   a.half_dt = half;
   a.ghost = w.ghost;
   a.lmax = w.lmax;
-  int64_t n = (int64_t)g->batch * a.f.plane;
-  dim3 eb(256), eg((unsigned)((n + 255) / 256));
+  a.part = w.part;
+  a.part_blocks = w.part_blocks;
   long long waves = (long long)a.strips * a.segs;
   dim3 lg((unsigned)((waves + kWavesPerBlock - 1) / kWavesPerBlock), (unsigned)g->batch);
   dim3 lb(kWave * kWavesPerBlock);
 
-  hipLaunchKernelGGL((adj_final_kernel<T>), eg, eb, 0, s, a.k, a.f, U, k2, dt, c->positivity_limiter,
-                     c->apply_friction, c->friction_model, lam_out, w.lam_k2, w.lam_acc, n);
+  // stage 2: corrector + friction reversed on load, L^T at the predictor
   AdjArgs<T> a2 = a;
-  a2.lam = w.lam_k2;
+  a2.k2 = k2;
+  a2.lam_out = lam_out;
+  a2.dt = dt;
+  a2.friction = c->apply_friction;
+  a2.friction_model = c->friction_model;
+  a2.acc = w.lam_acc;
   a2.base = nullptr;
   a2.out = w.lam_P;
-  hipLaunchKernelGGL((adj_L_kernel<T, 2>), lg, lb, 0, s, a2);
-  hipLaunchKernelGGL((adj_ghost_kernel<T>), dim3(g->batch), dim3(1024), 0, s, a2, 2, lam_level);
-  hipLaunchKernelGGL((adj_mid_kernel<T>), eg, eb, 0, s, a.k, a.f, U, k1, half, c->positivity_limiter, w.lam_P,
-                     w.lam_k1, w.lam_acc, n);
+  hipLaunchKernelGGL((adj_L_kernel<T, 2, 1>), lg, lb, 0, s, a2);
+  ghost_fold(a2, 2, g->batch, lam_level, s);
+  // stage 1: predictor clean-up reversed on load, L^T at U
   AdjArgs<T> a1 = a;
-  a1.lam = w.lam_k1;
+  a1.lam_P = w.lam_P;
+  a1.acc = w.lam_acc;
   a1.base = w.lam_acc;
   a1.out = lam_U;
-  hipLaunchKernelGGL((adj_L_kernel<T, 1>), lg, lb, 0, s, a1);
-  hipLaunchKernelGGL((adj_ghost_kernel<T>), dim3(g->batch), dim3(1024), 0, s, a1, 1, lam_level);
+  hipLaunchKernelGGL((adj_L_kernel<T, 1, 2>), lg, lb, 0, s, a1);
+  ghost_fold(a1, 1, g->batch, lam_level, s);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return afail(SWE_ERR_HIP, hipGetErrorString(e));
   return 0;
@@ -482,10 +560,12 @@ int operator_adjoint_impl(const swe_grid* g, const swe_config* c, const T* U, co
   a.out = lam_U;
   a.ghost = w.ghost;
   a.lmax = w.lmax;
+  a.part = w.part;
+  a.part_blocks = w.part_blocks;
   long long waves = (long long)a.strips * a.segs;
   dim3 lg((unsigned)((waves + kWavesPerBlock - 1) / kWavesPerBlock), (unsigned)g->batch);
-  hipLaunchKernelGGL((adj_L_kernel<T, 1>), lg, dim3(kWave * kWavesPerBlock), 0, s, a);
-  hipLaunchKernelGGL((adj_ghost_kernel<T>), dim3(g->batch), dim3(1024), 0, s, a, 1, lam_level);
+  hipLaunchKernelGGL((adj_L_kernel<T, 1, 0>), lg, dim3(kWave * kWavesPerBlock), 0, s, a);
+  ghost_fold(a, 1, g->batch, lam_level, s);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return afail(SWE_ERR_HIP, hipGetErrorString(e));
   return 0;
