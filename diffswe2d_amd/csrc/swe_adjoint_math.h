@@ -15,6 +15,18 @@
 
 namespace swe {
 
+// a / b inside a gradient expression.  FP32 device code multiplies by the
+// hardware reciprocal (v_rcp_f32, about 1 ulp) instead of the correctly rounded
+// division sequence; divisions that recompute forward values keep IEEE
+// division, so every branch of the reverse is taken exactly as in the forward.
+// FP64 and host code divide.
+template <class T> SWE_HD inline T gdiv(T a, T b) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  if constexpr (sizeof(T) == 4) return a * __builtin_amdgcn_rcpf(b);
+#endif
+  return a / b;
+}
+
 // weight of a in minimum(a, b) / maximum(a, b)
 template <class T> SWE_HD T wmin(T a, T b) { return a < b ? T(1) : (a == b ? T(0.5) : T(0)); }
 template <class T> SWE_HD T wmax(T a, T b) { return a > b ? T(1) : (a == b ? T(0.5) : T(0)); }
@@ -80,8 +92,8 @@ SWE_HD CellGrad<T> sides_bwd(const Cell<T>& c, const Incr<T>& d, bool reset, con
   if (wet && facR_in >= T(-10) && facR_in <= T(10)) gq += gfacR;
   // q = (0.5 dh) / max(h, floor)
   T s = clamp_lo(c.h, k.floor);
-  gdh += T(0.5) * gq / s;
-  T gs = -gq * (T(0.5) * d.dh) / (s * s);
+  gdh += gdiv(T(0.5) * gq, s);
+  T gs = gdiv(-gq * (T(0.5) * d.dh), s * s);
   if (c.h >= k.floor) r.h += gs;
   // water-slope reset: deta' = reset ? dh + dz : deta
   if (reset) {
@@ -141,10 +153,10 @@ SWE_HD void primitive_bwd(T h_raw, T hu, T hv, const SweConsts<T>& k, T gh, T gu
   bool wet = h > k.dry;
   T safe = clamp_lo(h, k.floor);
   gh += geta;  // eta = h + z
-  g_hu = wet ? gu / safe : T(0);
-  g_hv = wet ? gv / safe : T(0);
+  g_hu = wet ? gdiv(gu, safe) : T(0);
+  g_hv = wet ? gdiv(gv, safe) : T(0);
   if (wet) {
-    T gs = -(gu * hu + gv * hv) / (safe * safe);
+    T gs = gdiv(-(gu * hu + gv * hv), safe * safe);
     if (h >= k.floor) gh += gs;
   }
   g_hraw = h_raw >= T(0) ? gh : T(0);
@@ -216,8 +228,8 @@ SWE_HD void face_flux_bwd(const Side<T>& L, T hL_cell, T zL_cell, const Side<T>&
     }
     T gSL = 0, gSR = 0;
     if (gms != T(0) || gps != T(0)) {
-      T gnum_m = gms / den, gnum_p = gps / den;
-      T gden = -gms * num_m / (den * den) - gps * num_p / (den * den);
+      T gnum_m = gdiv(gms, den), gnum_p = gdiv(gps, den);
+      T gden = gdiv(-gms * num_m, den * den) - gdiv(gps * num_p, den * den);
       if (dS >= k.hll_floor) {
         gSR += gden;
         gSL -= gden;
@@ -278,16 +290,16 @@ SWE_HD void face_flux_bwd(const Side<T>& L, T hL_cell, T zL_cell, const Side<T>&
     T gsfL = 0, gsfR = 0;
     T gqnL = 0, gqtL = 0, gqnR = 0, gqtR = 0;
     if (wetL) {
-      gsfL += gcL * k.g / (T(2) * cL);
-      gqnL = gnL / sfL;
-      gqtL = gtL / sfL;
-      gsfL -= gnL * qnL / (sfL * sfL) + gtL * qtL / (sfL * sfL);
+      gsfL += gdiv(gcL * k.g, T(2) * cL);
+      gqnL = gdiv(gnL, sfL);
+      gqtL = gdiv(gtL, sfL);
+      gsfL -= gdiv(gnL * qnL, sfL * sfL) + gdiv(gtL * qtL, sfL * sfL);
     }
     if (wetR) {
-      gsfR += gcR * k.g / (T(2) * cR);
-      gqnR = gnR / sfR;
-      gqtR = gtR / sfR;
-      gsfR -= gnR * qnR / (sfR * sfR) + gtR * qtR / (sfR * sfR);
+      gsfR += gdiv(gcR * k.g, T(2) * cR);
+      gqnR = gdiv(gnR, sfR);
+      gqtR = gdiv(gtR, sfR);
+      gsfR -= gdiv(gnR * qnR, sfR * sfR) + gdiv(gtR * qtR, sfR * sfR);
     }
     if (hL >= k.floor) ghL += gsfL;
     if (hR >= k.floor) ghR += gsfR;
@@ -367,8 +379,8 @@ SWE_HD void clean_and_limit_bwd(T oh, T ou, T ov, T ch, T cu, T cv, const SweCon
   if (dh < T(0)) {
     T gtheta = g2h * dh + g2u * du + g2v * dv;
     T gq = q <= T(1) ? gtheta : T(0);
-    g_oh += gq / s;
-    T gs = -gq * oh / (s * s);
+    g_oh += gdiv(gq, s);
+    T gs = gdiv(-gq * oh, s * s);
     if (-dh >= k.eps) gdh -= gs;  // s = max(-dh, eps)
   }
   // delta = cand - old
@@ -436,21 +448,21 @@ SWE_HD void friction_bwd(T h_in, T hu, T hv, T coef, T dt, const SweConsts<T>& k
   T ghv = wet ? gv_out * f : T(0);
   T gh = gh_out;
   T gden = -gf * f * f;
-  T gb = gden / safe;
-  T gsafe = -gden * b / (safe * safe);
+  T gb = gdiv(gden, safe);
+  T gsafe = gdiv(-gden * b, safe * safe);
   T ga = gb * dt;
   T gcf = ga * speed;
   T gspeed = ga * cf;
   gh += gcf * gcf_dh;
   if (moving) {
-    T gss2 = gspeed / (T(2) * speed);
+    T gss2 = gdiv(gspeed, T(2) * speed);
     T gs2 = s2 >= k.fric_floor2 ? gss2 : T(0);
     T gu = T(2) * u * gs2, gv = T(2) * v * gs2;
     if (wet) {
-      ghu += gu / safe;
-      ghv += gv / safe;
+      ghu += gdiv(gu, safe);
+      ghv += gdiv(gv, safe);
     }
-    gsafe -= (gu * um + gv * vm) / (safe * safe);
+    gsafe -= gdiv(gu * um + gv * vm, safe * safe);
   }
   if (h >= k.floor) gh += gsafe;
   g_h = h_in >= T(0) ? gh : T(0);
