@@ -42,7 +42,7 @@ def parse():
     ap.add_argument("--cpu-steps", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--adjoint-steps", type=int, default=10,
-                    help="steps of the adjoint leg (replay with k1/k2 record, then reverse); 0 to skip")
+                    help="steps of the adjoint leg (taped forward, reverse sweep, checkpointed backward); 0 to skip")
     ap.add_argument("--copy-probe", action="store_true",
                     help="also run a 1 GiB device copy (calibrates rocprof FETCH_SIZE / WRITE_SIZE)")
     return ap.parse_args()
@@ -102,39 +102,84 @@ def time_kernels(model, U, steps, rows):
 
 
 def time_adjoint(model, U, steps):
-    """Per-step cost of the reverse sweep: replay with k1/k2 recorded, then the step adjoint."""
+    """Per-step costs of the differentiable run's pieces on `steps` steps.
+
+    forward_tape: device-driven forward that records the segment tape (states,
+    k1, k2); reverse: reverse sweep of that tape; replay_reverse: a checkpointed
+    segment's backward (replay from its first state onto a scratch tape, then
+    the reverse sweep), all through the library's segment entry points."""
+    import ctypes
+
     import torch
 
-    from diffswe2d_amd import adjoint
+    from diffswe2d_amd import _lib, adjoint
 
-    lvl = U.new_tensor(0.0)
-    tape, dts = [], []
-    Ucur = U.clone()
-    for _ in range(2):  # warm-up of both kernels
-        o, k1, k2, dt = adjoint.record_step(model, Ucur, lvl, 1e30)
-        lam_level = torch.zeros(U.shape[0], dtype=U.dtype, device=U.device)
-        adjoint.step_adjoint(model, Ucur, k1, k2, lvl, dt, torch.ones_like(U), lam_level)
+    lib = _lib.load()
+    bed, rough = model._fields(U)
+    ws = model._workspace(U, bed, rough)
+    aws = adjoint._adj_ws(model, U)
+    lvl, stride = model._level_arg(0.0, U)
+    stream = _lib.stream_ptr(U.device)
+    rows = int(model.rows_hint)
+    n = steps
+    tape = torch.empty((3 * n + 1, *U.shape), dtype=U.dtype, device=U.device)
+    dts = torch.empty(n, dtype=U.dtype, device=U.device)
+    hmax = U[:, 0].clone()
+
+    def forward_tape():
+        tape[0].copy_(U)
+        _lib.check(lib.swe_run_begin(_lib.ptr(ws["buf"]), 1e30, -1.0, U.element_size(), stream), "run_begin")
+        _lib.check(lib.swe_advance_tape_f32(ws["grid"], ws["cfg"], _lib.ptr(bed), _lib.ptr(rough), _lib.ptr(lvl),
+                                            stride, _lib.ptr(tape[0]), _lib.ptr(tape[n + 1]),
+                                            _lib.ptr(tape[2 * n + 1]), 0, 0, n, _lib.ptr(hmax), _lib.ptr(dts), n,
+                                            _lib.ptr(ws["buf"]), rows, stream), "advance_tape")
+
+    forward_tape()  # warm-up
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    for _ in range(steps):
-        o, k1, k2, dt = adjoint.record_step(model, Ucur, lvl, 1e30)
-        tape.append((Ucur, k1, k2))
-        dts.append(dt)
-        Ucur = o
+    forward_tape()
     torch.cuda.synchronize()
-    t_rec = (time.perf_counter() - t0) / steps
-    lam = torch.ones_like(U)
+    t_fwd = (time.perf_counter() - t0) / n
+    ctl = _lib.read_control(ws["buf"])
+    if ctl.steps != n or ctl.status != 0:
+        raise RuntimeError(f"taped forward failed: steps {ctl.steps}, status {ctl.status}")
+    from diffswe2d_amd.solver import sync_parity
+
+    sync_parity(ws, ctl)
+    host_dts = (ctypes.c_double * n)(*dts.cpu().tolist())
+    lam0 = torch.ones_like(U)
+    lam, lam_tmp = torch.empty_like(U), torch.empty_like(U)
     lam_level = torch.zeros(U.shape[0], dtype=U.dtype, device=U.device)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for k in range(steps - 1, -1, -1):
-        Uk, k1, k2 = tape[k]
-        lam = adjoint.step_adjoint(model, Uk, k1, k2, lvl, dts[k], lam, lam_level)
-    torch.cuda.synchronize()
-    t_adj = (time.perf_counter() - t0) / steps
-    if not bool(torch.isfinite(lam).all()):
-        raise RuntimeError("adjoint produced non-finite values")
-    return t_rec * 1e3, t_adj * 1e3
+    scratch = torch.empty(lib.swe_segment_tape_bytes(ws["grid"], n, 4) // 4, dtype=U.dtype, device=U.device)
+
+    def reverse():
+        lam.copy_(lam0)
+        _lib.check(lib.swe_segment_reverse_f32(
+            ws["grid"], ws["cfg"], _lib.ptr(tape[0]), _lib.ptr(tape[n + 1]), _lib.ptr(tape[2 * n + 1]),
+            _lib.ptr(bed), _lib.ptr(rough), _lib.ptr(lvl), stride, ctypes.addressof(host_dts), n, _lib.ptr(lam),
+            _lib.ptr(lam_tmp), _lib.ptr(lam_level), _lib.ptr(aws["buf"]), rows, stream),
+            "segment_reverse", "swe_adjoint_last_error_message")
+
+    def replay_reverse():
+        lam.copy_(lam0)
+        _lib.check(lib.swe_segment_adjoint_f32(
+            ws["grid"], ws["cfg"], _lib.ptr(tape[0]), _lib.ptr(bed), _lib.ptr(rough), _lib.ptr(lvl), stride,
+            ctypes.addressof(host_dts), n, ws["parity"], _lib.ptr(lam), _lib.ptr(lam_tmp), _lib.ptr(lam_level),
+            _lib.ptr(scratch), _lib.ptr(ws["buf"]), _lib.ptr(aws["buf"]), rows, stream),
+            "segment_adjoint", "swe_adjoint_last_error_message")
+        ws["parity"] ^= n & 1
+
+    out = {}
+    for name, fn in (("reverse", reverse), ("replay_reverse", replay_reverse)):
+        fn()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        fn()
+        torch.cuda.synchronize()
+        out[name] = (time.perf_counter() - t0) / n * 1e3
+        if not bool(torch.isfinite(lam).all()):
+            raise RuntimeError("adjoint produced non-finite values")
+    return t_fwd * 1e3, out["reverse"], out["replay_reverse"]
 
 
 def cpu_baseline(sc, cfg, side, steps):
@@ -239,14 +284,20 @@ def main():
 
     adj = None
     if a.adjoint_steps > 0:
-        rec_ms, adj_ms = time_adjoint(model, U, a.adjoint_steps)
-        back_ms = rec_ms + adj_ms  # backward with K-step checkpoints = replay (with record) + reverse sweep
-        adj = {"value": round(cells / (adj_ms * 1e-3) / 1e9, 4), "unit": "Gcell-updates/s",
-               "reverse_ms_per_step": round(adj_ms, 4), "replay_record_ms_per_step": round(rec_ms, 4),
-               "backward_ms_per_step": round(back_ms, 4),
-               "backward_over_forward": round(back_ms / ms, 3), "steps": a.adjoint_steps}
-        print(f"# adjoint: replay {rec_ms:.3f} ms + reverse {adj_ms:.3f} ms per step "
-              f"({back_ms / ms:.2f}x forward)", file=sys.stderr, flush=True)
+        fwd_tape_ms, rev_ms, ckpt_ms = time_adjoint(model, U, a.adjoint_steps)
+        # checkpointed backward (K-step segments): replay onto a scratch tape + reverse sweep.
+        # taped backward: the forward kept the tape (forward_tape_ms per step), the backward only reverses.
+        adj = {"value": round(cells / (rev_ms * 1e-3) / 1e9, 4), "unit": "Gcell-updates/s",
+               "reverse_ms_per_step": round(rev_ms, 4),
+               "checkpointed_backward_ms_per_step": round(ckpt_ms, 4),
+               "checkpointed_backward_over_forward": round(ckpt_ms / ms, 3),
+               "taped_forward_ms_per_step": round(fwd_tape_ms, 4),
+               "taped_backward_over_forward": round(rev_ms / ms, 3),
+               "taped_gradient_over_forward": round((fwd_tape_ms + rev_ms) / ms, 3),
+               "steps": a.adjoint_steps}
+        print(f"# adjoint: taped forward {fwd_tape_ms:.3f} ms, reverse {rev_ms:.3f} ms, checkpointed backward "
+              f"{ckpt_ms:.3f} ms per step ({rev_ms / ms:.2f}x / {ckpt_ms / ms:.2f}x forward)",
+              file=sys.stderr, flush=True)
 
     # HBM bytes per step from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE runs of this workload
     traffic, traffic_src = None, None

@@ -74,6 +74,8 @@ _SIGNATURES = {
     "swe_run_begin": ([_P, _D, _D, _I, _P], _I),
     "swe_advance_f32": ([_G, _C, _P, _P, _P, _I, _P, _P, _I, _I, _P, _P, _I, _P, _I, _P, _P, _I, _P, _I, _P], _I),
     "swe_advance_f64": ([_G, _C, _P, _P, _P, _I, _P, _P, _I, _I, _P, _P, _I, _P, _I, _P, _P, _I, _P, _I, _P], _I),
+    "swe_advance_tape_f32": ([_G, _C, _P, _P, _P, _I, _P, _P, _P, _I, _I, _I, _P, _P, _I, _P, _I, _P], _I),
+    "swe_advance_tape_f64": ([_G, _C, _P, _P, _P, _I, _P, _P, _P, _I, _I, _I, _P, _P, _I, _P, _I, _P], _I),
     "swe_adjoint_workspace_bytes": ([_G, _I], ctypes.c_size_t),
     "swe_adjoint_last_error_message": ([], ctypes.c_char_p),
     "swe_operator_adjoint_f32": ([_G, _C, _P, _P, _P, _I, _P, _P, _P, _P, _I, _P], _I),
@@ -83,6 +85,8 @@ _SIGNATURES = {
     "swe_segment_tape_bytes": ([_G, _I, _I], ctypes.c_size_t),
     "swe_segment_adjoint_f32": ([_G, _C, _P, _P, _P, _P, _I, _P, _I, _I, _P, _P, _P, _P, _P, _P, _I, _P], _I),
     "swe_segment_adjoint_f64": ([_G, _C, _P, _P, _P, _P, _I, _P, _I, _I, _P, _P, _P, _P, _P, _P, _I, _P], _I),
+    "swe_segment_reverse_f32": ([_G, _C, _P, _P, _P, _P, _P, _P, _I, _P, _I, _P, _P, _P, _P, _I, _P], _I),
+    "swe_segment_reverse_f64": ([_G, _C, _P, _P, _P, _P, _P, _P, _I, _P, _I, _P, _P, _P, _P, _I, _P], _I),
     "swe_ensemble_workspace_bytes": ([_I], ctypes.c_size_t),
     "swe_ensemble_last_error_message": ([], ctypes.c_char_p),
     "swe_ensemble_init": ([_C, _I, _P, _P, ctypes.c_size_t, ctypes.POINTER(ctypes.c_longlong), _P], _I),

@@ -23,6 +23,7 @@ import ctypes
 import torch
 
 from . import _lib
+from .solver import sync_parity
 
 _ADJ_MSG = "swe_adjoint_last_error_message"
 
@@ -119,50 +120,102 @@ def differentiable_step(model, U, level, max_dt):
     return out, float(dt)
 
 
-def _checkpointed_run(model, U0, level, duration, max_steps, cap, K):
-    """Device-driven forward (no per-step host sync) keeping the state every K steps.
+def tape_budget(model, U0):
+    """Bytes the forward may spend on segment tapes (k1, k2 and every state).
 
-    Returns (U, hmax, dts, checkpoints, done, elapsed)."""
+    ``model.adjoint_tape``: "auto" (default) spends up to half of the device's
+    free memory, True spends whatever it needs, False keeps checkpoints only,
+    a number is a budget in bytes."""
+    mode = getattr(model, "adjoint_tape", "auto")
+    if mode is False or mode == "off":
+        return 0
+    if mode is True or mode == "on":
+        return float("inf")
+    if isinstance(mode, (int, float)):
+        return float(mode)
+    if mode != "auto":
+        raise ValueError(f"adjoint_tape must be 'auto', True or False, got {mode!r}")
+    free, _ = torch.cuda.mem_get_info(U0.device)
+    return 0.5 * free
+
+
+def _checkpointed_run(model, U0, level, duration, max_steps, cap, K, budget=0):
+    """Device-driven forward (no per-step host sync) in segments of K steps.
+
+    A segment records its full tape (states, k1, k2; 3K + 1 states) while the
+    tapes fit in ``budget`` bytes; otherwise it keeps only its starting state,
+    and the backward replays it.  Returns (U, hmax, dts, segments, done,
+    elapsed); a segment is (kind, first, n_steps, data) with kind "tape"
+    (data = (U slots, k1 slots, k2 slots)) or "ckpt" (data = starting state)."""
     lib = _lib.load()
     bed, rough = model._fields(U0)
     ws = model._workspace(U0, bed, rough)
     lvl, stride = model._level_arg(level, U0)
     stream = _lib.stream_ptr(U0.device)
-    Ua, Ub = U0.clone(), torch.empty_like(U0)
+    rows = model._rows(U0)
+    cur = U0.clone()
+    Ua, Ub = None, None
     hmax = U0[:, 0].clone()
     cap_arg = -1.0 if cap is None else float(cap)
     _lib.check(lib.swe_run_begin(_lib.ptr(ws["buf"]), float(duration), cap_arg, U0.element_size(), stream),
                "run_begin")
-    fn = _fn(lib, "swe_advance", U0.dtype)
+    advance = _fn(lib, "swe_advance", U0.dtype)
+    advance_tape = _fn(lib, "swe_advance_tape", U0.dtype)
+    state_bytes = U0.numel() * U0.element_size()
     capacity = max(K, 64)
     dts = torch.empty(capacity, dtype=U0.dtype, device=U0.device)
-    checkpoints = []
+    segments = []
+    used = 0
     launched, steps = 0, 0
     ctl = None
     while launched < max_steps:
-        cur = Ua if steps % 2 == 0 else Ub
-        checkpoints.append(cur.clone())
         n = min(K, max_steps - launched)
         if launched + n > capacity:
             capacity = 2 * (launched + n)
             dts = torch.cat((dts, torch.empty(capacity - dts.numel(), dtype=dts.dtype, device=dts.device)))
-        _lib.check(fn(ws["grid"], ws["cfg"], _lib.ptr(bed), _lib.ptr(rough), _lib.ptr(lvl), stride, _lib.ptr(Ua),
-                      _lib.ptr(Ub), launched, n, _lib.ptr(hmax), _lib.ptr(dts), capacity, None, 0, None, None, 0,
-                      _lib.ptr(ws["buf"]), model._rows(U0), stream), "advance")
-        launched += n
+        need = (3 * n + 1) * state_bytes
+        if used + need <= budget:
+            tape = torch.empty((3 * n + 1, *U0.shape), dtype=U0.dtype, device=U0.device)
+            tape[0].copy_(cur)
+            _lib.check(advance_tape(ws["grid"], ws["cfg"], _lib.ptr(bed), _lib.ptr(rough), _lib.ptr(lvl), stride,
+                                    _lib.ptr(tape[0]), _lib.ptr(tape[n + 1]), _lib.ptr(tape[2 * n + 1]), launched,
+                                    launched, n, _lib.ptr(hmax), _lib.ptr(dts), capacity, _lib.ptr(ws["buf"]), rows,
+                                    stream), "advance_tape")
+            segments.append(["tape", launched, n, tape])
+            used += need
+        else:
+            if Ua is None:
+                Ua, Ub = torch.empty_like(U0), torch.empty_like(U0)
+            src = Ua if launched % 2 == 0 else Ub
+            src.copy_(cur)
+            segments.append(["ckpt", launched, n, cur.clone()])
+            _lib.check(advance(ws["grid"], ws["cfg"], _lib.ptr(bed), _lib.ptr(rough), _lib.ptr(lvl), stride,
+                               _lib.ptr(Ua), _lib.ptr(Ub), launched, n, _lib.ptr(hmax), _lib.ptr(dts), capacity, None,
+                               0, None, None, 0, _lib.ptr(ws["buf"]), rows, stream), "advance")
         ctl = _lib.read_control(ws["buf"])
         steps = ctl.steps
+        kind, first, _, data = segments[-1]
+        done_here = max(0, min(steps, launched + n) - first)
+        cur = data[done_here] if kind == "tape" else (Ua if steps % 2 == 0 else Ub)
+        launched += n
         if ctl.done:
             break
+    sync_parity(ws, ctl)
     if ctl.status != 0:
         if ctl.status == 1:
             raise FloatingPointError("The calculated timestep is non-finite.")
         raise FloatingPointError(f"Non-positive timestep: {ctl.bad_dt}.")
-    U = Ua if steps % 2 == 0 else Ub
+    U = cur.clone()
     dt_list = dts[:steps].cpu().tolist()
-    # a checkpoint taken after the last step is not needed
-    checkpoints = checkpoints[:(steps + K - 1) // K] if steps > 0 else checkpoints[:1]
-    return U, hmax, dt_list, checkpoints, bool(ctl.done), ctl.elapsed[steps & 1]
+    kept = []
+    for kind, first, n, data in segments:
+        m = min(n, steps - first)
+        if m <= 0:
+            continue
+        if kind == "tape":
+            data = (data[:m], data[n + 1:n + 1 + m], data[2 * n + 1:2 * n + 1 + m])
+        kept.append((kind, first, m, data))
+    return U, hmax, dt_list, kept, bool(ctl.done), ctl.elapsed[steps & 1]
 
 
 class _ForwardFn(torch.autograd.Function):
@@ -171,75 +224,84 @@ class _ForwardFn(torch.autograd.Function):
         U0_ = U0.detach().contiguous()
         lvl = level.detach()
         K = int(getattr(model, "adjoint_checkpoint_every", 64))
-        U, hmax, dts, checkpoints, done, elapsed = _checkpointed_run(model, U0_, lvl, duration, max_steps, cap, K)
+        if K < 1:
+            raise ValueError("adjoint_checkpoint_every must be at least 1")
+        budget = tape_budget(model, U0_)
+        U, hmax, dts, segments, done, elapsed = _checkpointed_run(model, U0_, lvl, duration, max_steps, cap, K,
+                                                                  budget)
         if not done:
             raise RuntimeError(
                 f"max_steps={max_steps} reached after {elapsed:.8f} s; requested duration={duration:.8f} s.")
-        ctx.model, ctx.dts, ctx.K = model, dts, K
-        ctx.save_for_backward(lvl, *checkpoints)
+        ctx.model, ctx.dts, ctx.segments = model, dts, segments
+        ctx.save_for_backward(lvl)
         ctx.mark_non_differentiable(hmax)
         model.last_dt_series = dts
+        model.last_adjoint_segments = [(kind, first, n) for kind, first, n, _ in segments]
         return U, hmax
 
     @staticmethod
     def backward(ctx, g_U, _g_hmax):
-        lvl, *checkpoints = ctx.saved_tensors
-        model, dts, K = ctx.model, ctx.dts, ctx.K
+        lvl, = ctx.saved_tensors
+        model, dts, segments = ctx.model, ctx.dts, ctx.segments
         lam = g_U.contiguous().clone()
         lam_level = torch.zeros(lam.shape[0], dtype=lam.dtype, device=lam.device)
-        n = len(dts)
-        if n == 0:
-            g_level = _level_grad(lvl, lam_level) if ctx.needs_input_grad[1] else None
-            return lam, g_level, None, None, None, None
-        if getattr(model, "adjoint_verify_replay", False):
-            lam = _replay_checked(model, checkpoints, lvl, dts, K, lam, lam_level)
-        else:
-            lam = _replay_segments(model, checkpoints, lvl, dts, K, lam, lam_level)
+        if segments:
+            if getattr(model, "adjoint_verify_replay", False):
+                lam = _replay_checked(model, segments, lvl, dts, lam, lam_level)
+            else:
+                lam = _reverse_segments(model, segments, lvl, dts, lam, lam_level)
+        ctx.segments = None
         g_level = _level_grad(lvl, lam_level) if ctx.needs_input_grad[1] else None
         return lam, g_level, None, None, None, None
 
 
-def _replay_segments(model, checkpoints, lvl, dts, K, lam, lam_level):
-    """Each segment: one library call replays it onto a tape and sweeps it in reverse."""
+def _reverse_segments(model, segments, lvl, dts, lam, lam_level):
+    """Last segment first: a taped segment is swept in reverse at once; a
+    checkpointed one is replayed onto a scratch tape and swept, in one call."""
     lib = _lib.load()
-    U0 = checkpoints[0]
+    U0 = segments[0][3][0][0] if segments[0][0] == "tape" else segments[0][3]  # a state of the run
     bed, rough = model._fields(U0)
     ws = model._workspace(U0, bed, rough)
     aws = _adj_ws(model, U0)
     l, stride = model._level_arg(lvl, U0)
-    n = len(dts)
-    seg_len = min(K, n)
-    tape = torch.empty(lib.swe_segment_tape_bytes(ws["grid"], seg_len, U0.element_size()) // U0.element_size(),
-                       dtype=U0.dtype, device=U0.device)
-    lam_tmp = torch.empty_like(lam)
-    host_dts = (ctypes.c_double * n)(*dts)
-    fn = _fn(lib, "swe_segment_adjoint", U0.dtype)
+    host_dts = (ctypes.c_double * len(dts))(*dts)
     stream = _lib.stream_ptr(U0.device)
     rows = int(model.rows_hint)
-    for seg in range((n + K - 1) // K - 1, -1, -1):
-        first, last = seg * K, min(n, (seg + 1) * K)
-        _lib.check(fn(ws["grid"], ws["cfg"], _lib.ptr(checkpoints[seg]), _lib.ptr(bed), _lib.ptr(rough), _lib.ptr(l),
-                      stride, ctypes.addressof(host_dts) + 8 * first, last - first, ws["parity"], _lib.ptr(lam),
-                      _lib.ptr(lam_tmp), _lib.ptr(lam_level), _lib.ptr(tape), _lib.ptr(ws["buf"]),
-                      _lib.ptr(aws["buf"]), rows, stream), "segment_adjoint", _ADJ_MSG)
-        ws["parity"] ^= (last - first) & 1
+    lam_tmp = torch.empty_like(lam)
+    scratch = None
+    for kind, first, n, data in reversed(segments):
+        dts_ptr = ctypes.addressof(host_dts) + 8 * first
+        if kind == "tape":
+            tU, tk1, tk2 = data
+            _lib.check(_fn(lib, "swe_segment_reverse", U0.dtype)(
+                ws["grid"], ws["cfg"], _lib.ptr(tU), _lib.ptr(tk1), _lib.ptr(tk2), _lib.ptr(bed), _lib.ptr(rough),
+                _lib.ptr(l), stride, dts_ptr, n, _lib.ptr(lam), _lib.ptr(lam_tmp), _lib.ptr(lam_level),
+                _lib.ptr(aws["buf"]), rows, stream), "segment_reverse", _ADJ_MSG)
+        else:
+            need = lib.swe_segment_tape_bytes(ws["grid"], n, U0.element_size()) // U0.element_size()
+            if scratch is None or scratch.numel() < need:
+                scratch = torch.empty(need, dtype=U0.dtype, device=U0.device)
+            _lib.check(_fn(lib, "swe_segment_adjoint", U0.dtype)(
+                ws["grid"], ws["cfg"], _lib.ptr(data), _lib.ptr(bed), _lib.ptr(rough), _lib.ptr(l), stride, dts_ptr,
+                n, ws["parity"], _lib.ptr(lam), _lib.ptr(lam_tmp), _lib.ptr(lam_level), _lib.ptr(scratch),
+                _lib.ptr(ws["buf"]), _lib.ptr(aws["buf"]), rows, stream), "segment_adjoint", _ADJ_MSG)
+            ws["parity"] ^= n & 1
     return lam
 
 
-def _replay_checked(model, checkpoints, lvl, dts, K, lam, lam_level):
-    """Step-by-step replay that reads every dt back and checks it against the recorded one."""
-    n = len(dts)
-    for seg in range((n + K - 1) // K - 1, -1, -1):
-        first, last = seg * K, min(n, (seg + 1) * K)
-        U = checkpoints[seg]
+def _replay_checked(model, segments, lvl, dts, lam, lam_level):
+    """Step-by-step replay from each segment's first state, reading every dt back
+    and checking it against the recorded one (taped segments are replayed too)."""
+    for kind, first, n, data in reversed(segments):
+        U = data[0][0] if kind == "tape" else data
         tape = []
-        for k in range(first, last):
+        for k in range(first, first + n):
             U_next, k1, k2, dt = record_step(model, U, lvl, dts[k], read_dt=True)
             if dt != dts[k]:
                 raise RuntimeError(f"adjoint replay diverged at step {k}: dt {dt} != {dts[k]}")
             tape.append((U, k1, k2))
             U = U_next
-        for k in range(last - 1, first - 1, -1):
+        for k in range(first + n - 1, first - 1, -1):
             Uk, k1, k2 = tape[k - first]
             lam = step_adjoint(model, Uk, k1, k2, lvl, dts[k], lam, lam_level)
         del tape

@@ -1,16 +1,17 @@
 // Hand-written adjoint of one time step on gfx950.
 //
 // The step U -> U' (solver.py:572-663) with dt held fixed (the reference
-// detaches dt, :625-626) is reversed by two L^T sweeps, each followed by the
-// ghost fold (four per-side launches and a level reduction):
-//   adj_L<2>    friction and corrector clean-up reversed on load (-> lam_k2 in
-//               registers, lam_acc written by the owning lane), then L^T at the
-//               predictor P = clean(U, U + dt/2 k1)             -> lam_P (+ ghosts)
-//   adj_ghost   ghost-cell adjoints folded back onto the edge cells; level
-//   adj_L<1>    predictor clean-up reversed on load (-> lam_k1 in registers,
-//               lam_acc updated by the owning lane), then L^T at U
-//                                                              -> lam_U = lam_acc + ...
-//   adj_ghost   as above
+// detaches dt, :625-626) is reversed by
+//   adj_corrector  friction and corrector clean-up reversed  -> lam_k2, lam_acc
+//   adj_L<2>       L^T at the predictor P = clean(U, U + dt/2 k1) -> lam_P (+ ghosts)
+//   adj_ghost      ghost-cell adjoints folded back onto the edge cells (four
+//                  per-side launches) and the boundary-level reduction
+//   adj_predictor  predictor clean-up reversed               -> lam_k1, lam_acc
+//   adj_L<1>       L^T at U                                  -> lam_U = lam_acc + ...
+//   adj_ghost      as above
+// The elementwise reverses stay separate launches: folded into the L^T sweeps
+// they would be recomputed on every halo cell at the sweeps' low occupancy,
+// which measured slower than one streaming pass each.
 // L^T mirrors the forward stage's decomposition: one wavefront streams down a
 // strip of 64 columns, now with a 3-cell halo (58 outputs): a cell's adjoint
 // depends on the four faces within two cells of it and those faces on the
@@ -47,13 +48,13 @@ template <class T> struct AdjArgs {
   T* out;                // adjoint of the stage input state [B,3,ny,nx]
   T* ghost;              // [B][4 sides][2 layers][lmax][3]
   int lmax;
-  // fused elementwise reverse on load (FUSE 1: corrector + friction, FUSE 2: predictor)
-  const T* k2;           // second-stage tendency (FUSE 1)
-  const T* lam_out;      // adjoint of the step output (FUSE 1)
-  const T* lam_P;        // adjoint of the predictor state (FUSE 2)
+  // elementwise reverses
+  const T* k2;           // second-stage tendency
+  const T* lam_out;      // adjoint of the step output
+  const T* lam_P;        // adjoint of the predictor state
   T dt;
   int friction, friction_model;
-  T* acc;                // adjoint reaching U directly: written (FUSE 1) / updated (FUSE 2) by the owning lane
+  T* acc;                // adjoint reaching U directly (written by the corrector, updated by the predictor)
   T* part;               // per-block level-gradient partials of the ghost fold
   int part_blocks;       // blocks per side in the ghost fold
 };
@@ -148,32 +149,40 @@ __device__ inline void predictor_bwd(const AdjArgs<T>& a, int64_t i, Lam3<T>& la
   lam = Lam3<T>{half * gc_h, half * gc_u, half * gc_v};
 }
 
-// adjoint of the stage tendency at padded cell (n, col); with FUSE it is built from
-// the elementwise reverse, whose direct part the owning lane stores in a.acc
-template <class T, int FUSE>
-__device__ inline Lam3<T> stage_lam(const AdjArgs<T>& a, int b, int n, int col, bool own) {
-  if (FUSE == 0) return load_lam(a, b, n, col);
-  if (n < 0 || n >= a.f.ny || col < 0 || col >= a.f.nx) return Lam3<T>{T(0), T(0), T(0)};
-  const int64_t cell = (int64_t)n * a.f.nx + col;
-  const int64_t i = (int64_t)b * 3 * a.f.plane + cell;
-  const int64_t P = a.f.plane;
+// one thread per interior cell: lam_k2 = adjoint of k2, acc = adjoint reaching U
+template <class T>
+__global__ __launch_bounds__(256) void adj_corrector_kernel(AdjArgs<T> a, T* lam_k2, int64_t n) {
+  int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  int b = (int)(t / a.f.plane);
+  int64_t cell = t - (int64_t)b * a.f.plane;
+  int64_t i = (int64_t)b * 3 * a.f.plane + cell, P = a.f.plane;
   Lam3<T> lam, acc;
-  if (FUSE == 1) {
-    corrector_bwd(a, b, cell, i, lam, acc);
-    if (own) {
-      a.acc[i] = acc.h;
-      a.acc[i + P] = acc.u;
-      a.acc[i + 2 * P] = acc.v;
-    }
-  } else {
-    predictor_bwd(a, i, lam, acc);
-    if (own) {
-      a.acc[i] += acc.h;
-      a.acc[i + P] += acc.u;
-      a.acc[i + 2 * P] += acc.v;
-    }
-  }
-  return lam;
+  corrector_bwd(a, b, cell, i, lam, acc);
+  a.acc[i] = acc.h;
+  a.acc[i + P] = acc.u;
+  a.acc[i + 2 * P] = acc.v;
+  lam_k2[i] = lam.h;
+  lam_k2[i + P] = lam.u;
+  lam_k2[i + 2 * P] = lam.v;
+}
+
+// one thread per interior cell: lam_k1 = adjoint of k1, acc += adjoint reaching U
+template <class T>
+__global__ __launch_bounds__(256) void adj_predictor_kernel(AdjArgs<T> a, T* lam_k1, int64_t n) {
+  int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  int b = (int)(t / a.f.plane);
+  int64_t cell = t - (int64_t)b * a.f.plane;
+  int64_t i = (int64_t)b * 3 * a.f.plane + cell, P = a.f.plane;
+  Lam3<T> lam, acc;
+  predictor_bwd(a, i, lam, acc);
+  a.acc[i] += acc.h;
+  a.acc[i + P] += acc.u;
+  a.acc[i + 2 * P] += acc.v;
+  lam_k1[i] = lam.h;
+  lam_k1[i + P] = lam.u;
+  lam_k1[i + 2 * P] = lam.v;
 }
 
 // x sweep of one row (all lanes take part): adjoint of the primitives of each
@@ -213,7 +222,7 @@ __device__ inline PrimGrad<T> x_sweep_bwd(const AdjArgs<T>& a, const Prim<T>& P,
 // latency the third wave hides.  FP64 keeps its natural allocation.
 template <class T> constexpr int adj_waves_per_eu() { return sizeof(T) == 4 ? 3 : 1; }
 
-template <class T, int STAGE, int FUSE>
+template <class T, int STAGE>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(adj_waves_per_eu<T>())))
 void adj_L_kernel(AdjArgs<T> a) {
   const int lane = threadIdx.x & (kWave - 1);
@@ -242,8 +251,7 @@ void adj_L_kernel(AdjArgs<T> a) {
   for (int it = 0; it < rows + 6; ++it) {
     const int n = y0 - kHaloA + it;
     Prim<T> Pn = fetch<T, STAGE, AdjArgs<T>>(a, b, n, col, half);
-    // the lane that emits row n (three iterations later) owns its elementwise reverse
-    Lam3<T> Ln = stage_lam<T, FUSE>(a, b, n, col, lane_out && n >= y0 && n < y0 + rows);
+    Lam3<T> Ln = load_lam(a, b, n, col);
     Incr<T> yi_cur{};
     bool reset_cur = false;
     PrimGrad<T> An{T(0), T(0), T(0), T(0)};
@@ -403,7 +411,7 @@ inline size_t al256(size_t v) { return (v + 255) & ~(size_t)255; }
 constexpr int kGhostThreads = 256;
 
 template <class T> struct AdjWs {
-  T *lam_acc, *lam_P, *ghost, *part;
+  T *lam_acc, *lam_P, *lam_k, *ghost, *part;  // lam_k holds lam_k2, then lam_k1
   int lmax, part_blocks;
 };
 
@@ -412,13 +420,13 @@ inline int ghost_blocks(const swe_grid* g) {
   return (lmax + kGhostThreads - 1) / kGhostThreads;
 }
 
-// [lam_acc | lam_P | ghost adjoints | level partials], each 256-byte aligned
+// [lam_acc | lam_P | lam_k | ghost adjoints | level partials], each 256-byte aligned
 template <class T> size_t adj_ws_bytes(const swe_grid* g) {
   size_t state = (size_t)g->batch * 3 * (size_t)g->ny * g->nx * sizeof(T);
   int lmax = g->ny > g->nx ? g->ny : g->nx;
   size_t ghost = (size_t)g->batch * 4 * 2 * lmax * 3 * sizeof(T);
   size_t part = (size_t)g->batch * 4 * ghost_blocks(g) * sizeof(T);
-  return 2 * al256(state) + al256(ghost) + al256(part);
+  return 3 * al256(state) + al256(ghost) + al256(part);
 }
 
 template <class T> AdjWs<T> adj_ws(void* ws, const swe_grid* g) {
@@ -429,8 +437,9 @@ template <class T> AdjWs<T> adj_ws(void* ws, const swe_grid* g) {
   AdjWs<T> w;
   w.lam_acc = (T*)p;
   w.lam_P = (T*)(p + state);
-  w.ghost = (T*)(p + 2 * state);
-  w.part = (T*)(p + 2 * state + ghost);
+  w.lam_k = (T*)(p + 2 * state);
+  w.ghost = (T*)(p + 3 * state);
+  w.part = (T*)(p + 3 * state + ghost);
   w.lmax = lmax;
   w.part_blocks = ghost_blocks(g);
   return w;
@@ -502,7 +511,9 @@ int step_adjoint_impl(const swe_grid* g, const swe_config* c, const T* U, const 
   dim3 lg((unsigned)((waves + kWavesPerBlock - 1) / kWavesPerBlock), (unsigned)g->batch);
   dim3 lb(kWave * kWavesPerBlock);
 
-  // stage 2: corrector + friction reversed on load, L^T at the predictor
+  int64_t n = (int64_t)g->batch * a.f.plane;
+  dim3 eb(256), eg((unsigned)((n + 255) / 256));
+  // stage 2: corrector + friction reversed, then L^T at the predictor
   AdjArgs<T> a2 = a;
   a2.k2 = k2;
   a2.lam_out = lam_out;
@@ -510,17 +521,21 @@ int step_adjoint_impl(const swe_grid* g, const swe_config* c, const T* U, const 
   a2.friction = c->apply_friction;
   a2.friction_model = c->friction_model;
   a2.acc = w.lam_acc;
+  hipLaunchKernelGGL((adj_corrector_kernel<T>), eg, eb, 0, s, a2, w.lam_k, n);
+  a2.lam = w.lam_k;
   a2.base = nullptr;
   a2.out = w.lam_P;
-  hipLaunchKernelGGL((adj_L_kernel<T, 2, 1>), lg, lb, 0, s, a2);
+  hipLaunchKernelGGL((adj_L_kernel<T, 2>), lg, lb, 0, s, a2);
   ghost_fold(a2, 2, g->batch, lam_level, s);
-  // stage 1: predictor clean-up reversed on load, L^T at U
+  // stage 1: predictor clean-up reversed, then L^T at U
   AdjArgs<T> a1 = a;
   a1.lam_P = w.lam_P;
   a1.acc = w.lam_acc;
+  hipLaunchKernelGGL((adj_predictor_kernel<T>), eg, eb, 0, s, a1, w.lam_k, n);
+  a1.lam = w.lam_k;
   a1.base = w.lam_acc;
   a1.out = lam_U;
-  hipLaunchKernelGGL((adj_L_kernel<T, 1, 2>), lg, lb, 0, s, a1);
+  hipLaunchKernelGGL((adj_L_kernel<T, 1>), lg, lb, 0, s, a1);
   ghost_fold(a1, 1, g->batch, lam_level, s);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return afail(SWE_ERR_HIP, hipGetErrorString(e));
@@ -564,7 +579,7 @@ int operator_adjoint_impl(const swe_grid* g, const swe_config* c, const T* U, co
   a.part_blocks = w.part_blocks;
   long long waves = (long long)a.strips * a.segs;
   dim3 lg((unsigned)((waves + kWavesPerBlock - 1) / kWavesPerBlock), (unsigned)g->batch);
-  hipLaunchKernelGGL((adj_L_kernel<T, 1, 0>), lg, dim3(kWave * kWavesPerBlock), 0, s, a);
+  hipLaunchKernelGGL((adj_L_kernel<T, 1>), lg, dim3(kWave * kWavesPerBlock), 0, s, a);
   ghost_fold(a, 1, g->batch, lam_level, s);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return afail(SWE_ERR_HIP, hipGetErrorString(e));
@@ -592,8 +607,33 @@ inline int record_step_t(const swe_grid* g, const swe_config* c, const double* U
   return swe_step_record_f64(g, c, U, bed, rough, level, ls, dt, parity, out, k1, k2, ws, rows, stream);
 }
 
-// Replay one checkpoint segment (recording U_k, k1_k, k2_k on the tape) and
-// sweep it in reverse.  tape holds 3 * n_steps + 1 states of B*3*ny*nx values.
+// Reverse sweep over a recorded segment: Ustates holds the n_steps states the
+// steps start from, K1 / K2 their stage tendencies (each B*3*ny*nx values).
+template <class T>
+int segment_reverse_impl(const swe_grid* g, const swe_config* c, const T* Ustates, const T* K1, const T* K2,
+                         const T* bed, const T* rough, const T* level, int ls, const double* dts, int n_steps, T* lam,
+                         T* lam_tmp, T* lam_level, void* adj_ws, int rows, void* stream) {
+  if (!g || !c || !Ustates || !K1 || !K2 || !dts || !lam || !lam_tmp || !adj_ws || n_steps <= 0)
+    return afail(SWE_ERR_ARG, "bad segment arguments");
+  const size_t S = (size_t)g->batch * 3 * (size_t)g->ny * g->nx;
+  T* cur = lam;
+  T* nxt = lam_tmp;
+  for (int k = n_steps - 1; k >= 0; --k) {
+    int rc = step_adjoint_impl<T>(g, c, Ustates + k * S, K1 + k * S, K2 + k * S, bed, rough, level, ls, dts[k], cur,
+                                  nxt, lam_level, adj_ws, rows, stream);
+    if (rc) return rc;
+    T* t = cur;
+    cur = nxt;
+    nxt = t;
+  }
+  if (cur != lam) {
+    hipError_t e = hipMemcpyAsync(lam, cur, S * sizeof(T), hipMemcpyDeviceToDevice, (hipStream_t)stream);
+    if (e != hipSuccess) return afail(SWE_ERR_HIP, hipGetErrorString(e));
+  }
+  return 0;
+}
+
+// Replay one checkpoint segment onto the tape (U_k, k1_k, k2_k), then sweep it in reverse.
 template <class T>
 int segment_adjoint_impl(const swe_grid* g, const swe_config* c, const T* U_ckpt, const T* bed, const T* rough,
                          const T* level, int ls, const double* dts, int n_steps, int parity, T* lam, T* lam_tmp,
@@ -601,32 +641,18 @@ int segment_adjoint_impl(const swe_grid* g, const swe_config* c, const T* U_ckpt
   if (!g || !c || !U_ckpt || !dts || !lam || !lam_tmp || !tape || !fwd_ws || !adj_ws || n_steps <= 0)
     return afail(SWE_ERR_ARG, "bad segment arguments");
   const size_t S = (size_t)g->batch * 3 * (size_t)g->ny * g->nx;
-  T* Ustates = tape;                            // n_steps + 1 states
-  T* K1 = tape + (size_t)(n_steps + 1) * S;     // n_steps
-  T* K2 = K1 + (size_t)n_steps * S;             // n_steps
+  T* Ustates = tape;
+  T* K1 = tape + (size_t)(n_steps + 1) * S;
+  T* K2 = K1 + (size_t)n_steps * S;
   hipError_t e = hipMemcpyAsync(Ustates, U_ckpt, S * sizeof(T), hipMemcpyDeviceToDevice, (hipStream_t)stream);
   if (e != hipSuccess) return afail(SWE_ERR_HIP, hipGetErrorString(e));
-  int rc;
   for (int k = 0; k < n_steps; ++k) {
-    rc = record_step_t(g, c, Ustates + k * S, bed, rough, level, ls, dts[k], (parity + k) & 1, Ustates + (k + 1) * S,
-                       K1 + k * S, K2 + k * S, fwd_ws, rows, stream);
+    int rc = record_step_t(g, c, Ustates + k * S, bed, rough, level, ls, dts[k], (parity + k) & 1,
+                           Ustates + (k + 1) * S, K1 + k * S, K2 + k * S, fwd_ws, rows, stream);
     if (rc) return afail(rc, swe_last_error_message());
   }
-  T* cur = lam;
-  T* nxt = lam_tmp;
-  for (int k = n_steps - 1; k >= 0; --k) {
-    rc = step_adjoint_impl<T>(g, c, Ustates + k * S, K1 + k * S, K2 + k * S, bed, rough, level, ls, dts[k], cur, nxt,
-                              lam_level, adj_ws, rows, stream);
-    if (rc) return rc;
-    T* t = cur;
-    cur = nxt;
-    nxt = t;
-  }
-  if (cur != lam) {
-    e = hipMemcpyAsync(lam, cur, S * sizeof(T), hipMemcpyDeviceToDevice, (hipStream_t)stream);
-    if (e != hipSuccess) return afail(SWE_ERR_HIP, hipGetErrorString(e));
-  }
-  return 0;
+  return segment_reverse_impl<T>(g, c, Ustates, K1, K2, bed, rough, level, ls, dts, n_steps, lam, lam_tmp, lam_level,
+                                 adj_ws, rows, stream);
 }
 
 }  // namespace
@@ -689,6 +715,22 @@ int swe_step_adjoint_f64(const swe_grid* g, const swe_config* c, const double* U
                          void* ws, int rows_hint, void* stream) {
   return step_adjoint_impl<double>(g, c, U, k1, k2, bed, rough, level, level_stride, dt, lam_out, lam_U, lam_level,
                                    ws, rows_hint, stream);
+}
+
+int swe_segment_reverse_f32(const swe_grid* g, const swe_config* c, const float* tape_U, const float* tape_k1,
+                            const float* tape_k2, const float* bed, const float* rough, const float* level,
+                            int level_stride, const double* dts, int n_steps, float* lam, float* lam_tmp,
+                            float* lam_level, void* adj_ws, int rows_hint, void* stream) {
+  return segment_reverse_impl<float>(g, c, tape_U, tape_k1, tape_k2, bed, rough, level, level_stride, dts, n_steps,
+                                     lam, lam_tmp, lam_level, adj_ws, rows_hint, stream);
+}
+
+int swe_segment_reverse_f64(const swe_grid* g, const swe_config* c, const double* tape_U, const double* tape_k1,
+                            const double* tape_k2, const double* bed, const double* rough, const double* level,
+                            int level_stride, const double* dts, int n_steps, double* lam, double* lam_tmp,
+                            double* lam_level, void* adj_ws, int rows_hint, void* stream) {
+  return segment_reverse_impl<double>(g, c, tape_U, tape_k1, tape_k2, bed, rough, level, level_stride, dts, n_steps,
+                                      lam, lam_tmp, lam_level, adj_ws, rows_hint, stream);
 }
 
 }  // extern "C"

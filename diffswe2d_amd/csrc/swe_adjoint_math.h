@@ -412,8 +412,8 @@ SWE_HD void friction_bwd(T h_in, T hu, T hv, T coef, T dt, const SweConsts<T>& k
     T num = k.manning_g * (n * n);
     T p = pow_third(sh, k.third);
     cf = num / p;
-    // d/dsh [num / sh^(1/3)] = -num / p^2 * (1/3) sh^(1/3 - 1)
-    T dp = k.third * pow_third(sh, k.third - T(1));
+    // d/dsh [num / sh^(1/3)] = -num / p^2 * (1/3) sh^(1/3 - 1), with sh^(1/3 - 1) = p / sh
+    T dp = gdiv(k.third * p, sh);
     if (h >= k.fric_floor) gcf_dh = -(num / (p * p)) * dp;
   } else {
     T fl = k.fric_floor;

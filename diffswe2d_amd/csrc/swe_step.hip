@@ -319,10 +319,13 @@ template <class T>
 int advance_impl(const swe_grid* g, const swe_config* c, const T* bed, const T* rough, const T* level,
                  int level_stride, T* U_a, T* U_b, int first_step, int n_steps, T* hmax, T* dt_series,
                  int dt_capacity, const int32_t* gauges, int n_gauges, T* gauge_depth, T* gauge_level,
-                 int gauge_capacity, void* ws, int rows_hint, void* stream) {
+                 int gauge_capacity, void* ws, int rows_hint, void* stream, T* tape_k1 = nullptr,
+                 T* tape_k2 = nullptr, int tape_first = 0) {
   int e;
   if ((e = check_grid(g)) || (e = check_cfg(c)) || (e = check_level(level, level_stride, g, c))) return e;
-  if (!U_a || !U_b || !bed || !ws || (c->apply_friction && !rough)) return fail(SWE_ERR_ARG, "null pointer argument");
+  if (!U_a || (!U_b && !tape_k1) || !bed || !ws || (c->apply_friction && !rough))
+    return fail(SWE_ERR_ARG, "null pointer argument");
+  if (tape_k1 && (!tape_k2 || first_step < tape_first)) return fail(SWE_ERR_ARG, "bad tape arguments");
   if (first_step < 0 || n_steps < 0) return fail(SWE_ERR_ARG, "negative step index");
   if (dt_series && first_step + n_steps > dt_capacity) return fail(SWE_ERR_CAPACITY, "dt series capacity exceeded");
   if (n_gauges > 0 && (!gauges || !gauge_depth || first_step + n_steps > gauge_capacity))
@@ -335,13 +338,23 @@ int advance_impl(const swe_grid* g, const swe_config* c, const T* bed, const T* 
   a.clock = 1;
   a.hmax = hmax;
   a.dt_series = dt_series;
+  const size_t state = (size_t)g->batch * 3 * (size_t)g->ny * g->nx;
   for (int k = first_step; k < first_step + n_steps; ++k) {
     int p = k & 1;
     T* src = p == 0 ? U_a : U_b;
     T* dst = p == 0 ? U_b : U_a;
+    T* k1 = w.k1;
+    T* k2 = nullptr;
+    if (tape_k1) {  // tape: U_a holds the states (slot j = step tape_first + j), k1/k2 per step
+      size_t j = (size_t)(k - tape_first);
+      src = U_a + j * state;
+      dst = U_a + (j + 1) * state;
+      k1 = tape_k1 + j * state;
+      k2 = tape_k2 + j * state;
+    }
     StageArgs<T> s1 = a;
     s1.U = src;
-    s1.out = w.k1;
+    s1.out = k1;
     s1.acc = w.acc + p * 2 * kSlots;
     s1.acc_next = w.acc + (1 - p) * 2 * kSlots;
     s1.parity = p;
@@ -349,7 +362,8 @@ int advance_impl(const swe_grid* g, const swe_config* c, const T* bed, const T* 
     s1.record_wave = 1;
     if ((e = launch_stage<T, 1>(s1, s))) return e;
     StageArgs<T> s2 = s1;
-    s2.k1 = w.k1;
+    s2.k1 = k1;
+    s2.k2_out = k2;
     s2.out = dst;
     s2.record_wave = 0;
     if ((e = launch_stage<T, 2>(s2, s))) return e;
@@ -480,6 +494,26 @@ int swe_advance_f64(const swe_grid* g, const swe_config* c, const double* bed, c
   return advance_impl<double>(g, c, bed, rough, level, level_stride, U_a, U_b, first_step, n_steps, hmax,
                               dt_series, dt_capacity, gauges, n_gauges, gauge_depth, gauge_level, gauge_capacity,
                               ws, rows_hint, stream);
+}
+
+int swe_advance_tape_f32(const swe_grid* g, const swe_config* c, const float* bed, const float* rough,
+                         const float* level, int level_stride, float* tape_U, float* tape_k1, float* tape_k2,
+                         int tape_first, int first_step, int n_steps, float* hmax, float* dt_series, int dt_capacity,
+                         void* ws, int rows_hint, void* stream) {
+  if (!tape_U || !tape_k1 || !tape_k2) return fail(SWE_ERR_ARG, "tape pointers are required");
+  return advance_impl<float>(g, c, bed, rough, level, level_stride, tape_U, nullptr, first_step, n_steps, hmax,
+                             dt_series, dt_capacity, nullptr, 0, nullptr, nullptr, 0, ws, rows_hint, stream, tape_k1,
+                             tape_k2, tape_first);
+}
+
+int swe_advance_tape_f64(const swe_grid* g, const swe_config* c, const double* bed, const double* rough,
+                         const double* level, int level_stride, double* tape_U, double* tape_k1, double* tape_k2,
+                         int tape_first, int first_step, int n_steps, double* hmax, double* dt_series,
+                         int dt_capacity, void* ws, int rows_hint, void* stream) {
+  if (!tape_U || !tape_k1 || !tape_k2) return fail(SWE_ERR_ARG, "tape pointers are required");
+  return advance_impl<double>(g, c, bed, rough, level, level_stride, tape_U, nullptr, first_step, n_steps, hmax,
+                              dt_series, dt_capacity, nullptr, 0, nullptr, nullptr, 0, ws, rows_hint, stream, tape_k1,
+                              tape_k2, tape_first);
 }
 
 }  // extern "C"

@@ -30,6 +30,16 @@ ScalarOrTensor = Union[float, int, torch.Tensor]
 _DTYPES = {torch.float32: "f32", torch.float64: "f64"}
 
 
+def sync_parity(ws, ctl):
+    """Point the host-side step parity at the wave-speed bank a device-driven run left cleared.
+
+    Step k accumulates its wave speeds into bank k & 1 and clears the other
+    bank for step k + 1.  After a run that ended with ``ctl.steps`` steps the
+    cleared bank is ``steps & 1``; a run stopped by a bad dt at step ``steps``
+    had already cleared the bank of step ``steps + 1``."""
+    ws["parity"] = (ctl.steps & 1) if ctl.status == 0 else 1 - (ctl.steps & 1)
+
+
 class SWE2D(nn.Module):
     """B-Flood-style 2-D shallow-water solver; state channels are [h, hu, hv].
 
@@ -446,6 +456,7 @@ class SWE2D(nn.Module):
             est = int(remaining / ctl.last_dt) + 2 if ctl.last_dt > 0 else chunk
             chunk = max(4, min(1024, est))
         steps = ctl.steps
+        sync_parity(ws, ctl)
         if ctl.status != 0:
             if ctl.status == 1:
                 raise FloatingPointError("The calculated timestep is non-finite.")

@@ -146,6 +146,21 @@ int swe_advance_f64(const swe_grid* grid, const swe_config* cfg, const double* b
                     int n_gauges, double* gauge_depth, double* gauge_level, int gauge_capacity, void* ws,
                     int rows_hint, void* stream);
 
+/* swe_advance_* that also records the adjoint tape of every step: tape_U holds
+ * the states (slot j = step tape_first + j; slot 0 is the input, step k writes
+ * slot k - tape_first + 1), tape_k1 / tape_k2 the two stage tendencies (slot
+ * k - tape_first).  The layout [tape_U | tape_k1 | tape_k2] of one segment is
+ * what swe_segment_reverse_* reads, so a forward that can afford the memory
+ * skips the backward's replay. */
+int swe_advance_tape_f32(const swe_grid* grid, const swe_config* cfg, const float* bed, const float* roughness,
+                         const float* level, int level_stride, float* tape_U, float* tape_k1, float* tape_k2,
+                         int tape_first, int first_step, int n_steps, float* hmax, float* dt_series, int dt_capacity,
+                         void* ws, int rows_hint, void* stream);
+int swe_advance_tape_f64(const swe_grid* grid, const swe_config* cfg, const double* bed, const double* roughness,
+                         const double* level, int level_stride, double* tape_U, double* tape_k1, double* tape_k2,
+                         int tape_first, int first_step, int n_steps, double* hmax, double* dt_series,
+                         int dt_capacity, void* ws, int rows_hint, void* stream);
+
 /* ---- adjoint (reverse mode) --------------------------------------------
  * The reference differentiates step() with torch autograd (dt detached,
  * solver.py:625-626; caller bc_inversion_train.py:332-452).  These entry
@@ -190,6 +205,19 @@ int swe_segment_adjoint_f64(const swe_grid* grid, const swe_config* cfg, const d
                             const double* roughness, const double* level, int level_stride, const double* dts,
                             int n_steps, int parity, double* lam, double* lam_tmp, double* lam_level, double* tape,
                             void* fwd_ws, void* adj_ws, int rows_hint, void* stream);
+
+/* Reverse sweep of a recorded segment (no replay): tape_U holds the n_steps
+ * states the steps start from, tape_k1 / tape_k2 their stage tendencies, as
+ * written by swe_advance_tape_*; dts, lam, lam_tmp, lam_level as for
+ * swe_segment_adjoint_*. */
+int swe_segment_reverse_f32(const swe_grid* grid, const swe_config* cfg, const float* tape_U, const float* tape_k1,
+                            const float* tape_k2, const float* bed, const float* roughness, const float* level,
+                            int level_stride, const double* dts, int n_steps, float* lam, float* lam_tmp,
+                            float* lam_level, void* adj_ws, int rows_hint, void* stream);
+int swe_segment_reverse_f64(const swe_grid* grid, const swe_config* cfg, const double* tape_U,
+                            const double* tape_k1, const double* tape_k2, const double* bed, const double* roughness,
+                            const double* level, int level_stride, const double* dts, int n_steps, double* lam,
+                            double* lam_tmp, double* lam_level, void* adj_ws, int rows_hint, void* stream);
 
 /* ---- ensemble of forward runs (FP32) -------------------------------------
  * Uneven grids, each with its own adaptive dt, advanced in lockstep by one

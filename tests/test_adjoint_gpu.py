@@ -89,8 +89,8 @@ def test_step_adjoint(golden, cuda_device, ieee_oracle, case, dtype):
         assert abs(float(gl) - float(gl_ref)) <= tol * max(1.0, abs(float(gl_ref))) * 10
 
 
-@pytest.mark.parametrize("verify", [False, True])
-def test_forward_adjoint_with_checkpoints_fp64(cuda_device, ieee_oracle, verify):
+@pytest.mark.parametrize("mode", ["replay", "verify", "tape", "mixed"])
+def test_forward_adjoint_with_checkpoints_fp64(cuda_device, ieee_oracle, mode):
     """A 0.3 s forward call (many steps, K = 5) against oracle autograd through the same steps.
 
     The initial surface carries a smooth ripple: a lake exactly at rest has
@@ -105,7 +105,12 @@ def test_forward_adjoint_with_checkpoints_fp64(cuda_device, ieee_oracle, verify)
     rough = torch.tensor(sc["rough"])
     m = swe.SWE2D(bed, rough, 0.136, 0.136, config=cfg, train_dem=False).to(cuda_device)
     m.adjoint_checkpoint_every = 5
-    m.adjoint_verify_replay = verify  # True: per-step replay with dt read-back; False: one call per segment
+    # replay: checkpoints only, one library call per segment; verify: per-step replay
+    # with dt read-back; tape: the forward records every segment's tape (no replay);
+    # mixed: the tape budget covers the first two segments only
+    m.adjoint_verify_replay = mode == "verify"
+    state_bytes = 3 * 26 * 40 * 8
+    m.adjoint_tape = {"replay": False, "verify": False, "tape": True, "mixed": 2 * 16 * state_bytes}[mode]
     U0 = torch.tensor(sc["U"])
     yy, xx = torch.meshgrid(torch.arange(26, dtype=torch.float64), torch.arange(40, dtype=torch.float64),
                             indexing="ij")
@@ -123,6 +128,10 @@ def test_forward_adjoint_with_checkpoints_fp64(cuda_device, ieee_oracle, verify)
     outg, _ = m(Ug, t_end=0.3, boundary_level=lg)
     assert relerr(outg.detach(), out.detach()) < 1e-12
     nsteps = len(m.last_dt_series)
+    kinds = [kind for kind, _, _ in m.last_adjoint_segments]
+    expect = {"replay": {"ckpt"}, "verify": {"ckpt"}, "tape": {"tape"}, "mixed": {"tape", "ckpt"}}[mode]
+    assert set(kinds) == expect, kinds
+    assert sum(n for _, _, n in m.last_adjoint_segments) == nsteps
     gU, gl = torch.autograd.grad(outg, (Ug, lg), lam.to(cuda_device))
     e = relerr(gU, gU_ref)
     el = abs(float(gl) - float(gl_ref)) / abs(float(gl_ref))
