@@ -143,6 +143,35 @@ def test_forward_device_loop_matches_host_steps(cuda_device):
     assert abs(m.ts_time[-1] - 1.5) < 1e-12
 
 
+@pytest.mark.parametrize("n_steps_parity", [0, 1])
+def test_step_after_device_run_sees_only_its_own_wave_speeds(cuda_device, n_steps_parity):
+    """A single step right after a device-driven run must not inherit that run's wave speeds.
+
+    The run integrates a deep, fast state (an odd or even number of steps); the
+    step then starts from a shallow calm state whose CFL dt is much larger.  Its
+    dt must equal the oracle's dt for that state alone."""
+    n = 24
+    cfg = swe.SWEConfig(boundary_left="wall", boundary_right="wall", boundary_top="wall", boundary_bottom="wall",
+                        apply_friction=False)
+    bed = torch.zeros(1, 1, n, n)
+    rough = torch.zeros(1, 1, n, n)
+    m = swe.SWE2D(bed, rough, 0.1, 0.1, config=cfg, train_dem=False).to(cuda_device)
+    fast = torch.zeros(1, 3, n, n)
+    fast[0, 0] = 5.0
+    fast[0, 0, : n // 2] = 8.0
+    Uf = fast.to(cuda_device)
+    d0 = float(O.step(fast, bed, rough, cfg, 0.1, 0.1, 1e30, 0.0)[1])  # the fast state's first dt
+    steps_wanted = 3 + n_steps_parity  # the run ends after an odd / even number of steps
+    m(Uf, t_end=(steps_wanted - 0.5) * d0)
+    assert len(m.last_dt_series) >= 3
+    calm = torch.zeros(1, 3, n, n)
+    calm[0, 0] = 0.01
+    out, dt = m.step(calm.to(cuda_device), 0.0, 1e30)
+    out_ref, dt_ref = O.step(calm, bed, rough, cfg, 0.1, 0.1, 1e30, 0.0)
+    print(f"run of {len(m.last_dt_series)} steps, then calm step dt {dt:.6e} vs oracle {float(dt_ref):.6e}")
+    assert abs(dt - float(dt_ref)) <= 1e-6 * float(dt_ref)
+
+
 def oracle_runup_ieee(d):
     """The oracle with correctly rounded FP32 sqrt on the same run-up (dt series, final state)."""
     import oracle.swe_oracle as OO
