@@ -17,12 +17,18 @@
 namespace swe {
 
 // ---------------------------------------------------------------- torch-like helpers
-template <class T> SWE_HD T tmax(T a, T b) { return (a > b || a != a) ? a : b; }
-template <class T> SWE_HD T tmin(T a, T b) { return (a < b || a != a) ? a : b; }
-template <class T> SWE_HD T clamp_lo(T x, T lo) { return x < lo ? lo : x; }
-template <class T> SWE_HD T clamp_hi(T x, T hi) { return x > hi ? hi : x; }
-template <class T> SWE_HD T clamp2(T x, T lo, T hi) { return x < lo ? lo : (x > hi ? hi : x); }
-template <class T> SWE_HD T tabs(T x) { return x < T(0) ? -x : x; }
+// torch.maximum / minimum / clamp propagate NaN.  IEEE 754-2019 maximum and
+// minimum do the same in one gfx950 instruction (v_maximum3_f32 /
+// v_minimum3_f32) where a compare-and-select would take two or three.  The
+// two forms agree on every value except the sign of an exact zero result
+// (maximum(-0, +0) is +0), which no later operation of the step can see: zeros
+// only meet comparisons, products and sums, never a division or sign test.
+template <class T> SWE_HD T tmax(T a, T b) { return __builtin_elementwise_maximum(a, b); }
+template <class T> SWE_HD T tmin(T a, T b) { return __builtin_elementwise_minimum(a, b); }
+template <class T> SWE_HD T clamp_lo(T x, T lo) { return __builtin_elementwise_maximum(x, lo); }
+template <class T> SWE_HD T clamp_hi(T x, T hi) { return __builtin_elementwise_minimum(x, hi); }
+template <class T> SWE_HD T clamp2(T x, T lo, T hi) { return clamp_hi(clamp_lo(x, lo), hi); }
+template <class T> SWE_HD T tabs(T x) { return __builtin_fabs(x); }
 
 // Constants of one run, folded on the host in double then cast to T once.
 template <class T> struct SweConsts {

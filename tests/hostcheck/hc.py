@@ -27,7 +27,9 @@ _h = None
 def lib():
     global _h
     if _h is None:
-        if not LIB.exists():
+        root = HERE.parent.parent
+        sources = [HERE / "hostcheck.cpp", *sorted((root / "diffswe2d_amd" / "csrc").glob("*.h"))]
+        if not LIB.exists() or LIB.stat().st_mtime < max(p.stat().st_mtime for p in sources):
             build()
         _h = ctypes.CDLL(str(LIB))
     return _h
