@@ -38,8 +38,8 @@ SWE_HD void limited_increment_bwd(T qm, T q, T qp, T theta, T g, T& gqm, T& gq, 
   T a = theta * (q - qm);
   T b = T(0.5) * (qp - qm);
   T c = theta * (qp - q);
-  bool pos = a > T(0) && b > T(0) && c > T(0);
-  bool neg = a < T(0) && b < T(0) && c < T(0);
+  bool pos = tmin(a, tmin(b, c)) > T(0);  // all three > 0 (the forward's branch test)
+  bool neg = tmax(a, tmax(b, c)) < T(0);  // all three < 0
   if (!(pos || neg) || g == T(0)) return;
   T aa = tabs(a), ab = tabs(b), ac = tabs(c);
   T mbc = tmin(ab, ac);

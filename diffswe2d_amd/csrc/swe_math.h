@@ -51,10 +51,12 @@ template <class T> struct SweConsts {
 // ---------------------------------------------------------------- limiter (numerics.py:4-27)
 // minmod of (theta*backward, centred, theta*forward): the smallest magnitude when
 // all three share a strict sign, else 0 -- equal to sign(a)*min(|a|,|b|,|c|).
+// All three are > 0 exactly when their minimum is (and < 0 when their maximum
+// is); a NaN makes both tests false, as it does for the three separate tests.
 template <class T> SWE_HD T minmod3(T a, T b, T c) {
-  if (a > T(0) && b > T(0) && c > T(0)) return tmin(a, tmin(b, c));
-  if (a < T(0) && b < T(0) && c < T(0)) return tmax(a, tmax(b, c));
-  return T(0);
+  T lo = tmin(a, tmin(b, c));
+  T hi = tmax(a, tmax(b, c));
+  return lo > T(0) ? lo : (hi < T(0) ? hi : T(0));
 }
 
 template <class T> SWE_HD T limited_increment(T qm, T q, T qp, T theta) {
