@@ -48,7 +48,9 @@ class SweControl(ctypes.Structure):
     _fields_ = [("elapsed", ctypes.c_double * 2), ("duration", ctypes.c_double), ("tol", ctypes.c_double),
                 ("cap", ctypes.c_double), ("bad_dt", ctypes.c_double), ("done", ctypes.c_int32),
                 ("steps", ctypes.c_int32), ("status", ctypes.c_int32), ("pad", ctypes.c_int32),
-                ("last_dt", ctypes.c_double)]
+                ("last_dt", ctypes.c_double), ("chunked", ctypes.c_int32), ("chunk", ctypes.c_int32),
+                ("n_chunks", ctypes.c_int32), ("pad2", ctypes.c_int32), ("level_f64", ctypes.c_double),
+                ("level_f32", ctypes.c_float), ("pad3", ctypes.c_float), ("levels", ctypes.c_void_p)]
 
 
 _P = ctypes.c_void_p
@@ -74,6 +76,9 @@ _SIGNATURES = {
     "swe_run_begin": ([_P, _D, _D, _I, _P], _I),
     "swe_advance_f32": ([_G, _C, _P, _P, _P, _I, _P, _P, _I, _I, _P, _P, _I, _P, _I, _P, _P, _I, _P, _I, _P], _I),
     "swe_advance_f64": ([_G, _C, _P, _P, _P, _I, _P, _P, _I, _I, _P, _P, _I, _P, _I, _P, _P, _I, _P, _I, _P], _I),
+    "swe_run_begin_chunked": ([_P, _D, _I, _P, _I, _P], _I),
+    "swe_advance_chunked_f32": ([_G, _C, _P, _P, _P, _P, _P, _P, _I, _I, _I, _P, _P, _I, _P, _I, _P, _P, _I, _P], _I),
+    "swe_advance_chunked_f64": ([_G, _C, _P, _P, _P, _P, _P, _P, _I, _I, _I, _P, _P, _I, _P, _I, _P, _P, _I, _P], _I),
     "swe_advance_tape_f32": ([_G, _C, _P, _P, _P, _I, _P, _P, _P, _I, _I, _I, _P, _P, _I, _P, _I, _P], _I),
     "swe_advance_tape_f64": ([_G, _C, _P, _P, _P, _I, _P, _P, _P, _I, _I, _I, _P, _P, _I, _P, _I, _P], _I),
     "swe_adjoint_workspace_bytes": ([_G, _I], ctypes.c_size_t),
@@ -87,6 +92,14 @@ _SIGNATURES = {
     "swe_segment_adjoint_f64": ([_G, _C, _P, _P, _P, _P, _I, _P, _I, _I, _P, _P, _P, _P, _P, _P, _I, _P], _I),
     "swe_segment_reverse_f32": ([_G, _C, _P, _P, _P, _P, _P, _P, _I, _P, _I, _P, _P, _P, _P, _I, _P], _I),
     "swe_segment_reverse_f64": ([_G, _C, _P, _P, _P, _P, _P, _P, _I, _P, _I, _P, _P, _P, _P, _I, _P], _I),
+    "swe_segment_reverse_chunked_f32": ([_G, _C, _P, _P, _P, _P, _P, _P, _P, _P, _P, _I, _P, _I, _P, _P, _P, _P, _P,
+                                         _I, _P], _I),
+    "swe_segment_reverse_chunked_f64": ([_G, _C, _P, _P, _P, _P, _P, _P, _P, _P, _P, _I, _P, _I, _P, _P, _P, _P, _P,
+                                         _I, _P], _I),
+    "swe_segment_adjoint_chunked_f32": ([_G, _C, _P, _P, _P, _P, _P, _P, _P, _I, _I, _P, _I, _P, _P, _P, _P, _P, _P,
+                                         _P, _I, _P], _I),
+    "swe_segment_adjoint_chunked_f64": ([_G, _C, _P, _P, _P, _P, _P, _P, _P, _I, _I, _P, _I, _P, _P, _P, _P, _P, _P,
+                                         _P, _I, _P], _I),
     "swe_ensemble_workspace_bytes": ([_I], ctypes.c_size_t),
     "swe_ensemble_last_error_message": ([], ctypes.c_char_p),
     "swe_ensemble_init": ([_C, _I, _P, _P, ctypes.c_size_t, ctypes.POINTER(ctypes.c_longlong), _P], _I),

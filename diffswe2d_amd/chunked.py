@@ -1,0 +1,213 @@
+"""Whole chunked record in one device-driven run, with its reverse mode.
+
+The reference's inversion driver advances the model chunk by chunk
+(Examples/inverse/.../bc_inversion_train.py:332-349: ``forward(U,
+t_end=chunk, boundary_level=series[c])`` per chunk) and observes h + z at the
+gauges after every chunk (:401-410).  Driven from the host that costs one
+launch sequence and two synchronisations per chunk; here the chunk clock lives
+on the device (csrc/swe_step.hip ``chunk_clock_kernel``): the run only returns
+to the host every few hundred steps.  The result equals the chunk-by-chunk
+loop bit for bit (same steps, same dt, same levels).
+
+``gauge_run(model, U0, series, chunk, gauges)`` returns (obs [n_chunks,
+n_gauges] of h + z, final state); with gradients enabled it records tapes (or
+checkpoints, as ``SWE2D.forward`` does) and the backward sweeps the run in
+reverse, adding the observation adjoints at chunk ends and collecting the
+level adjoint of every chunk (csrc/swe_adjoint.hip, ``*_chunked_*``).
+"""
+from __future__ import annotations
+
+import ctypes
+
+import torch
+
+from . import _lib
+from .adjoint import _ADJ_MSG, _adj_ws, _fn, tape_budget
+from .solver import sync_parity
+
+_TAPE_SEGMENT = 512     # steps per recorded segment when the tape fits
+_PLAIN_SEGMENT = 1024   # steps per launch batch without gradients
+
+
+def _run(model, U0, levels, chunk, gauges, max_steps, K, budget):
+    """Device-driven chunked run; returns (obs, U, dts, chunks, segments)."""
+    lib = _lib.load()
+    n_chunks = levels.numel()
+    bed, rough = model._fields(U0)
+    ws = model._workspace(U0, bed, rough)
+    stream = _lib.stream_ptr(U0.device)
+    rows = model._rows(U0)
+    esize = U0.element_size()
+    _lib.check(lib.swe_run_begin_chunked(_lib.ptr(ws["buf"]), float(chunk), n_chunks, _lib.ptr(levels), esize,
+                                         stream), "run_begin_chunked")
+    advance = _fn(lib, "swe_advance_chunked", U0.dtype)
+    n_g = gauges.shape[0]
+    obs = torch.full((n_chunks, max(n_g, 1)), float("nan"), dtype=U0.dtype, device=U0.device)
+    capacity = 1024
+    dts = torch.empty(capacity, dtype=U0.dtype, device=U0.device)
+    chunks = torch.empty(capacity, dtype=torch.int32, device=U0.device)
+    state_bytes = U0.numel() * esize
+    segments, used = [], 0
+    cur = U0.clone()
+    Ua = Ub = None
+    launched, ctl = 0, None
+    while launched < max_steps:
+        kind, n = "ckpt", min(K, max_steps - launched)
+        if budget is None:
+            kind, n = "plain", min(_PLAIN_SEGMENT, max_steps - launched)
+        else:
+            for m in (min(_TAPE_SEGMENT, max_steps - launched), n):
+                if used + (3 * m + 1) * state_bytes <= budget:
+                    kind, n = "tape", m
+                    break
+        if launched + n > capacity:
+            capacity = max(2 * capacity, launched + n)
+            dts = torch.cat((dts, torch.empty(capacity - dts.numel(), dtype=dts.dtype, device=dts.device)))
+            chunks = torch.cat((chunks, torch.empty(capacity - chunks.numel(), dtype=chunks.dtype,
+                                                    device=chunks.device)))
+        if kind == "tape":
+            tape = torch.empty((3 * n + 1, *U0.shape), dtype=U0.dtype, device=U0.device)
+            tape[0].copy_(cur)
+            _lib.check(advance(ws["grid"], ws["cfg"], _lib.ptr(bed), _lib.ptr(rough), _lib.ptr(tape[0]), None,
+                               _lib.ptr(tape[n + 1]), _lib.ptr(tape[2 * n + 1]), launched, launched, n,
+                               _lib.ptr(dts), _lib.ptr(chunks), capacity, _lib.ptr(gauges), n_g, _lib.ptr(obs),
+                               _lib.ptr(ws["buf"]), rows, stream), "advance_chunked")
+            segments.append(["tape", launched, n, tape])
+            used += (3 * n + 1) * state_bytes
+        else:
+            if Ua is None:
+                Ua, Ub = torch.empty_like(U0), torch.empty_like(U0)
+            src = Ua if launched % 2 == 0 else Ub
+            src.copy_(cur)
+            if kind == "ckpt":
+                segments.append(["ckpt", launched, n, cur.clone()])
+            _lib.check(advance(ws["grid"], ws["cfg"], _lib.ptr(bed), _lib.ptr(rough), _lib.ptr(Ua), _lib.ptr(Ub),
+                               None, None, 0, launched, n, _lib.ptr(dts), _lib.ptr(chunks), capacity,
+                               _lib.ptr(gauges), n_g, _lib.ptr(obs), _lib.ptr(ws["buf"]), rows, stream),
+                       "advance_chunked")
+        ctl = _lib.read_control(ws["buf"])
+        steps = ctl.steps
+        if kind == "tape":
+            cur = segments[-1][3][max(0, min(steps, launched + n) - launched)]
+        else:
+            cur = Ua if steps % 2 == 0 else Ub
+        launched += n
+        if ctl.done:
+            break
+    sync_parity(ws, ctl)
+    if ctl.status != 0:
+        if ctl.status == 1:
+            raise FloatingPointError("The calculated timestep is non-finite.")
+        raise FloatingPointError(f"Non-positive timestep: {ctl.bad_dt}.")
+    if not ctl.done:
+        raise RuntimeError(f"max_steps={max_steps} reached in chunk {ctl.chunk} of {n_chunks}.")
+    steps = ctl.steps
+    kept = []
+    for kind, first, n, data in segments:
+        m = min(n, steps - first)
+        if m <= 0:
+            continue
+        if kind == "tape":
+            data = (data[:m], data[n + 1:n + 1 + m], data[2 * n + 1:2 * n + 1 + m])
+        kept.append((kind, first, m, data))
+    return obs[:, :n_g], cur.clone(), dts[:steps], chunks[:steps], kept
+
+
+class _ChunkedFn(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, U0, levels, model, chunk, gauges, max_steps):
+        U0_ = U0.detach().contiguous()
+        lv = levels.detach().contiguous()
+        K = int(getattr(model, "adjoint_checkpoint_every", 64))
+        if K < 1:
+            raise ValueError("adjoint_checkpoint_every must be at least 1")
+        obs, U, dts, chunks, segments = _run(model, U0_, lv, chunk, gauges, max_steps, K, tape_budget(model, U0_))
+        ctx.model, ctx.segments, ctx.gauges = model, segments, gauges
+        ctx.dts = dts.cpu().tolist()
+        ctx.chunks = chunks.cpu().tolist()
+        ctx.save_for_backward(lv)
+        model.last_dt_series = ctx.dts
+        model.last_adjoint_segments = [(kind, first, n) for kind, first, n, _ in segments]
+        return obs, U
+
+    @staticmethod
+    def backward(ctx, g_obs, g_U):
+        lv, = ctx.saved_tensors
+        model, segments, gauges = ctx.model, ctx.segments, ctx.gauges
+        lib = _lib.load()
+        some = segments[0][3]
+        U0 = some[0][0] if segments[0][0] == "tape" else some
+        dtype, dev = U0.dtype, U0.device
+        bed, rough = model._fields(U0)
+        ws = model._workspace(U0, bed, rough)
+        aws = _adj_ws(model, U0)
+        stream = _lib.stream_ptr(dev)
+        rows = int(model.rows_hint)
+        lam = g_U.contiguous().clone() if g_U is not None else torch.zeros_like(U0)
+        lam_tmp = torch.empty_like(lam)
+        level_grad = torch.zeros(lv.numel(), dtype=dtype, device=dev)
+        n_g = gauges.shape[0]
+        g_obs = (g_obs.contiguous() if g_obs is not None
+                 else torch.zeros(lv.numel(), max(n_g, 1), dtype=dtype, device=dev))
+        n = len(ctx.dts)
+        chunk = ctx.chunks
+        ends = [1 if k == n - 1 or chunk[k + 1] != chunk[k] else 0 for k in range(n)]
+        host_dts = (ctypes.c_double * n)(*ctx.dts)
+        host_chunk = (ctypes.c_int32 * n)(*chunk)
+        host_ends = (ctypes.c_int32 * n)(*ends)
+        scratch = None
+        for kind, first, m, data in reversed(segments):
+            dptr = ctypes.addressof(host_dts) + 8 * first
+            cptr = ctypes.addressof(host_chunk) + 4 * first
+            eptr = ctypes.addressof(host_ends) + 4 * first
+            if kind == "tape":
+                tU, tk1, tk2 = data
+                _lib.check(_fn(lib, "swe_segment_reverse_chunked", dtype)(
+                    ws["grid"], ws["cfg"], _lib.ptr(tU), _lib.ptr(tk1), _lib.ptr(tk2), _lib.ptr(bed),
+                    _lib.ptr(rough), _lib.ptr(lv), dptr, cptr, eptr, m, _lib.ptr(gauges), n_g, _lib.ptr(g_obs),
+                    _lib.ptr(lam), _lib.ptr(lam_tmp), _lib.ptr(level_grad), _lib.ptr(aws["buf"]), rows, stream),
+                    "segment_reverse_chunked", _ADJ_MSG)
+            else:
+                need = lib.swe_segment_tape_bytes(ws["grid"], m, U0.element_size()) // U0.element_size()
+                if scratch is None or scratch.numel() < need:
+                    scratch = torch.empty(need, dtype=dtype, device=dev)
+                _lib.check(_fn(lib, "swe_segment_adjoint_chunked", dtype)(
+                    ws["grid"], ws["cfg"], _lib.ptr(data), _lib.ptr(bed), _lib.ptr(rough), _lib.ptr(lv), dptr,
+                    cptr, eptr, m, ws["parity"], _lib.ptr(gauges), n_g, _lib.ptr(g_obs), _lib.ptr(lam),
+                    _lib.ptr(lam_tmp), _lib.ptr(level_grad), _lib.ptr(scratch), _lib.ptr(ws["buf"]),
+                    _lib.ptr(aws["buf"]), rows, stream), "segment_adjoint_chunked", _ADJ_MSG)
+                ws["parity"] ^= m & 1
+        ctx.segments = None
+        return lam, level_grad, None, None, None, None
+
+
+def gauge_run(model, U0, series, chunk, gauges, max_steps=10_000_000):
+    """Run len(series) chunks of `chunk` seconds with level series[c] in chunk c.
+
+    Returns (obs [n_chunks, n_gauges] = h + z at the gauge cells of member 0 at
+    each chunk end, final state).  U0 must be [1, 3, ny, nx]; gradients flow to
+    U0 and series when either requires them."""
+    if U0.dim() != 4 or U0.shape[0] != 1:
+        raise ValueError("gauge_run takes a single member U0 of shape [1, 3, ny, nx]")
+    if model.use_dynamic_bc:
+        raise NotImplementedError("gauge_run uses its explicit per-chunk levels, not a dynamic boundary series")
+    if model.rainfall is not None:
+        raise NotImplementedError("rainfall forcing is not built into the HIP step")
+    model._check_state(U0, "U0")
+    ny, nx = U0.shape[-2:]
+    g = torch.as_tensor(gauges, dtype=torch.int32).reshape(-1, 2)
+    if g.numel() and not (((g[:, 0] >= 0) & (g[:, 0] < ny) & (g[:, 1] >= 0) & (g[:, 1] < nx)).all()):
+        raise IndexError(f"a gauge lies outside the {ny}x{nx} grid")
+    if g.shape[0] > 1024:
+        raise ValueError("at most 1024 gauges")
+    g = g.to(U0.device).contiguous()
+    levels = series.reshape(-1).to(dtype=U0.dtype)
+    if not (chunk > 0):
+        raise ValueError("chunk duration must be positive")
+    if torch.is_grad_enabled() and (U0.requires_grad or levels.requires_grad):
+        return _ChunkedFn.apply(U0, levels, model, float(chunk), g, int(max_steps))
+    with torch.no_grad():
+        obs, U, dts, _, _ = _run(model, U0.detach().contiguous(), levels.detach().contiguous(), float(chunk), g,
+                                 int(max_steps), 0, None)
+    model.last_dt_series = dts.cpu().tolist()
+    return obs, U

@@ -392,6 +392,14 @@ __global__ __launch_bounds__(256) void adj_level_reduce_kernel(const T* part, in
   if (threadIdx.x == 0) lam_level[b] += acc[0];
 }
 
+// adjoint of a gauge observation h + z (member 0): lam_h += g at each gauge cell,
+// gauges in order (two gauges may share a cell)
+template <class T>
+__global__ void inject_gauges_kernel(T* lam, int nx, const int32_t* gauges, int n_gauges, const T* g) {
+  if (threadIdx.x != 0) return;
+  for (int i = 0; i < n_gauges; ++i) lam[(int64_t)gauges[2 * i] * nx + gauges[2 * i + 1]] += g[i];
+}
+
 }  // namespace swe
 
 // ===================================================================== host side
@@ -633,6 +641,68 @@ int segment_reverse_impl(const swe_grid* g, const swe_config* c, const T* Ustate
   return 0;
 }
 
+// Chunked runs (per-chunk boundary level, gauges observed at chunk ends): step k
+// of the segment used levels[chunk[k]]; where ends_chunk[k] is set, the step's
+// output was observed, so the observation adjoint g_obs[chunk[k]] enters lam
+// before the step is reversed.  The level adjoint of step k accumulates into
+// level_grad[chunk[k]] (member 0; batch 1).
+template <class T>
+int segment_reverse_chunked_impl(const swe_grid* g, const swe_config* c, const T* Ustates, const T* K1, const T* K2,
+                                 const T* bed, const T* rough, const T* levels, const double* dts,
+                                 const int32_t* chunk, const int32_t* ends_chunk, int n_steps, const int32_t* gauges,
+                                 int n_gauges, const T* g_obs, T* lam, T* lam_tmp, T* level_grad, void* adj_ws,
+                                 int rows, void* stream) {
+  if (!g || !c || !Ustates || !K1 || !K2 || !levels || !dts || !chunk || !ends_chunk || !lam || !lam_tmp ||
+      !level_grad || !adj_ws || n_steps <= 0 || (n_gauges > 0 && (!gauges || !g_obs)))
+    return afail(SWE_ERR_ARG, "bad chunked segment arguments");
+  if (g->batch != 1) return afail(SWE_ERR_SHAPE, "chunked runs have one member");
+  const size_t S = (size_t)3 * (size_t)g->ny * g->nx;
+  hipStream_t s = (hipStream_t)stream;
+  T* cur = lam;
+  T* nxt = lam_tmp;
+  for (int k = n_steps - 1; k >= 0; --k) {
+    if (ends_chunk[k] && n_gauges > 0) {
+      hipLaunchKernelGGL((inject_gauges_kernel<T>), dim3(1), dim3(64), 0, s, cur, g->nx, gauges, n_gauges,
+                         g_obs + (size_t)chunk[k] * n_gauges);
+    }
+    int rc = step_adjoint_impl<T>(g, c, Ustates + k * S, K1 + k * S, K2 + k * S, bed, rough, levels + chunk[k], 0,
+                                  dts[k], cur, nxt, level_grad + chunk[k], adj_ws, rows, stream);
+    if (rc) return rc;
+    T* t = cur;
+    cur = nxt;
+    nxt = t;
+  }
+  if (cur != lam) {
+    hipError_t e = hipMemcpyAsync(lam, cur, S * sizeof(T), hipMemcpyDeviceToDevice, s);
+    if (e != hipSuccess) return afail(SWE_ERR_HIP, hipGetErrorString(e));
+  }
+  hipError_t e = hipGetLastError();
+  return e == hipSuccess ? 0 : afail(SWE_ERR_HIP, hipGetErrorString(e));
+}
+
+template <class T>
+int segment_adjoint_chunked_impl(const swe_grid* g, const swe_config* c, const T* U_ckpt, const T* bed,
+                                 const T* rough, const T* levels, const double* dts, const int32_t* chunk,
+                                 const int32_t* ends_chunk, int n_steps, int parity, const int32_t* gauges,
+                                 int n_gauges, const T* g_obs, T* lam, T* lam_tmp, T* level_grad, T* tape,
+                                 void* fwd_ws, void* adj_ws, int rows, void* stream) {
+  if (!g || !c || !U_ckpt || !levels || !dts || !chunk || !tape || !fwd_ws || n_steps <= 0)
+    return afail(SWE_ERR_ARG, "bad chunked segment arguments");
+  const size_t S = (size_t)g->batch * 3 * (size_t)g->ny * g->nx;
+  T* Ustates = tape;
+  T* K1 = tape + (size_t)(n_steps + 1) * S;
+  T* K2 = K1 + (size_t)n_steps * S;
+  hipError_t e = hipMemcpyAsync(Ustates, U_ckpt, S * sizeof(T), hipMemcpyDeviceToDevice, (hipStream_t)stream);
+  if (e != hipSuccess) return afail(SWE_ERR_HIP, hipGetErrorString(e));
+  for (int k = 0; k < n_steps; ++k) {
+    int rc = record_step_t(g, c, Ustates + k * S, bed, rough, levels + chunk[k], 0, dts[k], (parity + k) & 1,
+                           Ustates + (k + 1) * S, K1 + k * S, K2 + k * S, fwd_ws, rows, stream);
+    if (rc) return afail(rc, swe_last_error_message());
+  }
+  return segment_reverse_chunked_impl<T>(g, c, Ustates, K1, K2, bed, rough, levels, dts, chunk, ends_chunk, n_steps,
+                                         gauges, n_gauges, g_obs, lam, lam_tmp, level_grad, adj_ws, rows, stream);
+}
+
 // Replay one checkpoint segment onto the tape (U_k, k1_k, k2_k), then sweep it in reverse.
 template <class T>
 int segment_adjoint_impl(const swe_grid* g, const swe_config* c, const T* U_ckpt, const T* bed, const T* rough,
@@ -731,6 +801,50 @@ int swe_segment_reverse_f64(const swe_grid* g, const swe_config* c, const double
                             double* lam_level, void* adj_ws, int rows_hint, void* stream) {
   return segment_reverse_impl<double>(g, c, tape_U, tape_k1, tape_k2, bed, rough, level, level_stride, dts, n_steps,
                                       lam, lam_tmp, lam_level, adj_ws, rows_hint, stream);
+}
+
+int swe_segment_reverse_chunked_f32(const swe_grid* g, const swe_config* c, const float* tape_U,
+                                    const float* tape_k1, const float* tape_k2, const float* bed, const float* rough,
+                                    const float* levels, const double* dts, const int32_t* chunk_of_step,
+                                    const int32_t* ends_chunk, int n_steps, const int32_t* gauges, int n_gauges,
+                                    const float* g_obs, float* lam, float* lam_tmp, float* level_grad, void* adj_ws,
+                                    int rows_hint, void* stream) {
+  return segment_reverse_chunked_impl<float>(g, c, tape_U, tape_k1, tape_k2, bed, rough, levels, dts, chunk_of_step,
+                                             ends_chunk, n_steps, gauges, n_gauges, g_obs, lam, lam_tmp, level_grad,
+                                             adj_ws, rows_hint, stream);
+}
+
+int swe_segment_reverse_chunked_f64(const swe_grid* g, const swe_config* c, const double* tape_U,
+                                    const double* tape_k1, const double* tape_k2, const double* bed,
+                                    const double* rough, const double* levels, const double* dts,
+                                    const int32_t* chunk_of_step, const int32_t* ends_chunk, int n_steps,
+                                    const int32_t* gauges, int n_gauges, const double* g_obs, double* lam,
+                                    double* lam_tmp, double* level_grad, void* adj_ws, int rows_hint, void* stream) {
+  return segment_reverse_chunked_impl<double>(g, c, tape_U, tape_k1, tape_k2, bed, rough, levels, dts,
+                                              chunk_of_step, ends_chunk, n_steps, gauges, n_gauges, g_obs, lam,
+                                              lam_tmp, level_grad, adj_ws, rows_hint, stream);
+}
+
+int swe_segment_adjoint_chunked_f32(const swe_grid* g, const swe_config* c, const float* U_ckpt, const float* bed,
+                                    const float* rough, const float* levels, const double* dts,
+                                    const int32_t* chunk_of_step, const int32_t* ends_chunk, int n_steps, int parity,
+                                    const int32_t* gauges, int n_gauges, const float* g_obs, float* lam,
+                                    float* lam_tmp, float* level_grad, float* tape, void* fwd_ws, void* adj_ws,
+                                    int rows_hint, void* stream) {
+  return segment_adjoint_chunked_impl<float>(g, c, U_ckpt, bed, rough, levels, dts, chunk_of_step, ends_chunk,
+                                             n_steps, parity, gauges, n_gauges, g_obs, lam, lam_tmp, level_grad, tape,
+                                             fwd_ws, adj_ws, rows_hint, stream);
+}
+
+int swe_segment_adjoint_chunked_f64(const swe_grid* g, const swe_config* c, const double* U_ckpt,
+                                    const double* bed, const double* rough, const double* levels, const double* dts,
+                                    const int32_t* chunk_of_step, const int32_t* ends_chunk, int n_steps, int parity,
+                                    const int32_t* gauges, int n_gauges, const double* g_obs, double* lam,
+                                    double* lam_tmp, double* level_grad, double* tape, void* fwd_ws, void* adj_ws,
+                                    int rows_hint, void* stream) {
+  return segment_adjoint_chunked_impl<double>(g, c, U_ckpt, bed, rough, levels, dts, chunk_of_step, ends_chunk,
+                                              n_steps, parity, gauges, n_gauges, g_obs, lam, lam_tmp, level_grad,
+                                              tape, fwd_ws, adj_ws, rows_hint, stream);
 }
 
 }  // extern "C"

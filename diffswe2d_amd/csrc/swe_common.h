@@ -30,6 +30,17 @@ struct Control {
   int32_t status;      // 0 ok, 1 non-finite dt, 2 non-positive dt
   int32_t pad;
   double last_dt;      // dt of the most recent step (host-driven steps)
+  // chunked runs: the duration is one chunk; at each chunk end the clock kernel
+  // samples the gauges, moves to the next chunk's boundary level and restarts
+  // the chunk's elapsed time, until the last chunk ends
+  int32_t chunked;
+  int32_t chunk;       // current chunk
+  int32_t n_chunks;
+  int32_t pad2;
+  double level_f64;    // the current chunk's boundary level, read by the stage kernels
+  float level_f32;
+  float pad3;
+  const void* levels;  // per-chunk levels (device, the run's dtype)
 };
 
 // Grid geometry and field pointers of one model (one batch of members).

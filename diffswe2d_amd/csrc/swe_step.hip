@@ -116,6 +116,58 @@ __global__ void run_begin_kernel(Control* c, uint32_t* acc_words, int n_acc_word
     c->steps = 0;
     c->status = 0;
     c->last_dt = 0.0;
+    c->chunked = 0;
+    c->chunk = 0;
+    c->n_chunks = 0;
+    c->levels = nullptr;
+  }
+}
+
+// Arms a chunked run: every chunk lasts `duration`, chunk c uses levels[c].
+template <class T>
+__global__ void run_begin_chunked_kernel(Control* c, int n_chunks, const T* levels) {
+  if (threadIdx.x != 0) return;
+  c->chunked = 1;
+  c->chunk = 0;
+  c->n_chunks = n_chunks;
+  c->levels = levels;
+  c->level_f64 = (double)levels[0];
+  c->level_f32 = (float)levels[0];
+}
+
+// After stage 2 of a chunked step: record the step's chunk; at a chunk end sample
+// h + z at the gauges (member 0) from the step's output, then move to the next
+// chunk (its level, elapsed time 0) or finish the run after the last one.
+template <class T>
+__global__ void chunk_clock_kernel(Control* c, const T* U_out, const T* bed, int nx, const int32_t* gauges,
+                                   int n_gauges, T* obs, int32_t* chunk_series, int step_index, int parity) {
+  __shared__ int complete;
+  if (threadIdx.x == 0) {
+    complete = 0;
+    if (!c->done) {
+      if (chunk_series != nullptr) chunk_series[step_index] = c->chunk;
+      complete = (c->duration - c->elapsed[1 - parity]) <= c->tol;
+    }
+  }
+  __syncthreads();
+  if (!complete) return;
+  const int chunk = c->chunk;
+  for (int g = threadIdx.x; g < n_gauges; g += blockDim.x) {
+    int64_t cell = (int64_t)gauges[2 * g] * nx + gauges[2 * g + 1];
+    obs[(int64_t)chunk * n_gauges + g] = U_out[cell] + bed[cell];
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int next = chunk + 1;
+    c->chunk = next;
+    if (next >= c->n_chunks) {
+      c->done = 1;
+    } else {
+      c->elapsed[1 - parity] = 0.0;
+      T lv = ((const T*)c->levels)[next];
+      c->level_f64 = (double)lv;
+      c->level_f32 = (float)lv;
+    }
   }
 }
 
@@ -320,7 +372,9 @@ int advance_impl(const swe_grid* g, const swe_config* c, const T* bed, const T* 
                  int level_stride, T* U_a, T* U_b, int first_step, int n_steps, T* hmax, T* dt_series,
                  int dt_capacity, const int32_t* gauges, int n_gauges, T* gauge_depth, T* gauge_level,
                  int gauge_capacity, void* ws, int rows_hint, void* stream, T* tape_k1 = nullptr,
-                 T* tape_k2 = nullptr, int tape_first = 0) {
+                 T* tape_k2 = nullptr, int tape_first = 0, const int32_t* chunk_gauges = nullptr,
+                 int chunk_n_gauges = 0, T* chunk_obs = nullptr, int32_t* chunk_series = nullptr,
+                 bool chunked = false) {
   int e;
   if ((e = check_grid(g)) || (e = check_cfg(c)) || (e = check_level(level, level_stride, g, c))) return e;
   if (!U_a || (!U_b && !tape_k1) || !bed || !ws || (c->apply_friction && !rough))
@@ -367,6 +421,12 @@ int advance_impl(const swe_grid* g, const swe_config* c, const T* bed, const T* 
     s2.out = dst;
     s2.record_wave = 0;
     if ((e = launch_stage<T, 2>(s2, s))) return e;
+    if (chunked) {
+      hipLaunchKernelGGL((chunk_clock_kernel<T>), dim3(1), dim3(256), 0, s, w.ctl, (const T*)dst, bed, g->nx,
+                         chunk_gauges, chunk_n_gauges, chunk_obs, chunk_series, k, p);
+      hipError_t he = hipGetLastError();
+      if (he != hipSuccess) return fail(SWE_ERR_HIP, hipGetErrorString(he));
+    }
     if (n_gauges > 0) {
       hipLaunchKernelGGL((gauge_kernel<T>), dim3(1), dim3(1024), 0, s, (const T*)dst, bed, a.f.plane, g->nx,
                          a.f.bed_bstride, gauges, n_gauges, gauge_depth, gauge_level, k, (const Control*)w.ctl);
@@ -375,6 +435,23 @@ int advance_impl(const swe_grid* g, const swe_config* c, const T* bed, const T* 
     }
   }
   return 0;
+}
+
+template <class T>
+int advance_chunked(const swe_grid* g, const swe_config* c, const T* bed, const T* rough, T* U_a, T* U_b, T* tape_k1,
+                    T* tape_k2, int tape_first, int first_step, int n_steps, T* dt_series, int32_t* chunk_series,
+                    int capacity, const int32_t* gauges, int n_gauges, T* obs, void* ws, int rows_hint,
+                    void* stream) {
+  if (!ws || (n_gauges > 0 && (!gauges || !obs))) return fail(SWE_ERR_ARG, "null pointer argument");
+  if (n_gauges < 0 || n_gauges > 1024) return fail(SWE_ERR_ARG, "0 to 1024 gauges");
+  if ((dt_series || chunk_series) && first_step + n_steps > capacity)
+    return fail(SWE_ERR_CAPACITY, "step series capacity exceeded");
+  // the stage kernels read the chunk's level from the control block
+  Control* ctl = (Control*)ws;
+  const T* level = sizeof(T) == 8 ? (const T*)&ctl->level_f64 : (const T*)&ctl->level_f32;
+  return advance_impl<T>(g, c, bed, rough, level, 0, U_a, U_b, first_step, n_steps, nullptr, dt_series, capacity,
+                         nullptr, 0, nullptr, nullptr, 0, ws, rows_hint, stream, tape_k1, tape_k2, tape_first, gauges,
+                         n_gauges, obs, chunk_series, true);
 }
 
 }  // namespace
@@ -494,6 +571,39 @@ int swe_advance_f64(const swe_grid* g, const swe_config* c, const double* bed, c
   return advance_impl<double>(g, c, bed, rough, level, level_stride, U_a, U_b, first_step, n_steps, hmax,
                               dt_series, dt_capacity, gauges, n_gauges, gauge_depth, gauge_level, gauge_capacity,
                               ws, rows_hint, stream);
+}
+
+int swe_run_begin_chunked(void* ws, double chunk_seconds, int n_chunks, const void* levels, int dtype_bytes,
+                          void* stream) {
+  if (!levels || n_chunks <= 0) return fail(SWE_ERR_ARG, "chunked runs need levels and n_chunks > 0");
+  if (!(chunk_seconds > 0.0)) return fail(SWE_ERR_ARG, "chunk duration must be positive");
+  int e = swe_run_begin(ws, chunk_seconds, -1.0, dtype_bytes, stream);
+  if (e) return e;
+  if (dtype_bytes == 8)
+    hipLaunchKernelGGL((run_begin_chunked_kernel<double>), dim3(1), dim3(64), 0, (hipStream_t)stream, (Control*)ws,
+                       n_chunks, (const double*)levels);
+  else
+    hipLaunchKernelGGL((run_begin_chunked_kernel<float>), dim3(1), dim3(64), 0, (hipStream_t)stream, (Control*)ws,
+                       n_chunks, (const float*)levels);
+  hipError_t he = hipGetLastError();
+  return he == hipSuccess ? 0 : fail(SWE_ERR_HIP, hipGetErrorString(he));
+}
+
+int swe_advance_chunked_f32(const swe_grid* g, const swe_config* c, const float* bed, const float* rough,
+                            float* U_a, float* U_b, float* tape_k1, float* tape_k2, int tape_first, int first_step,
+                            int n_steps, float* dt_series, int32_t* chunk_series, int capacity,
+                            const int32_t* gauges, int n_gauges, float* obs, void* ws, int rows_hint, void* stream) {
+  return advance_chunked<float>(g, c, bed, rough, U_a, U_b, tape_k1, tape_k2, tape_first, first_step, n_steps,
+                                dt_series, chunk_series, capacity, gauges, n_gauges, obs, ws, rows_hint, stream);
+}
+
+int swe_advance_chunked_f64(const swe_grid* g, const swe_config* c, const double* bed, const double* rough,
+                            double* U_a, double* U_b, double* tape_k1, double* tape_k2, int tape_first,
+                            int first_step, int n_steps, double* dt_series, int32_t* chunk_series, int capacity,
+                            const int32_t* gauges, int n_gauges, double* obs, void* ws, int rows_hint,
+                            void* stream) {
+  return advance_chunked<double>(g, c, bed, rough, U_a, U_b, tape_k1, tape_k2, tape_first, first_step, n_steps,
+                                 dt_series, chunk_series, capacity, gauges, n_gauges, obs, ws, rows_hint, stream);
 }
 
 int swe_advance_tape_f32(const swe_grid* g, const swe_config* c, const float* bed, const float* rough,

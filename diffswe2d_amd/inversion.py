@@ -12,7 +12,10 @@ path:
                                chunk (:332-349), observation h + z at the gauge
                                cells after each chunk (:401-410), loss = mean
                                over gauges of each gauge's MSE plus the dense-
-                               series smoothness terms (:421-452)
+                               series smoothness terms (:421-452); the chunk
+                               loop runs on the device (``chunked.gauge_run``)
+                               or, with ``fused=False``, chunk by chunk as the
+                               reference does
 * ``adam_invert``             Adam with gradient-norm clipping (:632-720)
 
 Gradients come from the hand-written adjoint (``diffswe2d_amd.adjoint``).
@@ -98,6 +101,7 @@ class GaugeProblem:
     observe: str = "water_level"
     weights: tuple = (1.0e-3, 1.0e-2, 1.0e-6)  # first, second derivative, amplitude
     stats: dict = field(default_factory=dict)
+    fused: bool = True  # the whole record as one device-driven run (diffswe2d_amd.chunked)
 
     def __post_init__(self):
         bed = self.model.bed.detach().to(device=self.U0.device, dtype=self.U0.dtype)
@@ -109,6 +113,12 @@ class GaugeProblem:
         """Observations [n_chunks, n_gauges] for the dense boundary series."""
         if series.numel() < self.n_chunks:
             raise ValueError("series shorter than the number of chunks")
+        if self.fused and self.U0.shape[0] == 1 and not self.model.use_dynamic_bc:
+            from .chunked import gauge_run
+
+            obs, _ = gauge_run(self.model, self.U0, series[:self.n_chunks], self.chunk, self.gauges)
+            self.stats["steps"] = len(self.model.last_dt_series)
+            return obs if self.observe == "water_level" else obs - self._bed_g
         s = self.U0
         obs = []
         steps = 0

@@ -79,6 +79,10 @@ typedef struct swe_control {
   double duration, tol, cap, bad_dt;
   int32_t done, steps, status, pad;
   double last_dt;
+  int32_t chunked, chunk, n_chunks, pad2;
+  double level_f64;
+  float level_f32, pad3;
+  const void* levels;
 } swe_control;
 
 int swe_abi_version(void);
@@ -145,6 +149,29 @@ int swe_advance_f64(const swe_grid* grid, const swe_config* cfg, const double* b
                     int n_steps, double* hmax, double* dt_series, int dt_capacity, const int32_t* gauges,
                     int n_gauges, double* gauge_depth, double* gauge_level, int gauge_capacity, void* ws,
                     int rows_hint, void* stream);
+
+/* Chunked run -- replaces the inversion driver's chunk loop
+ * (bc_inversion_train.py:332-349, 401-410: forward(U, t_end=chunk,
+ * boundary_level=series[c]) per chunk, then h + z at the gauges) with one
+ * device-driven run.  swe_run_begin_chunked arms n_chunks chunks of
+ * chunk_seconds each with per-chunk levels (device array of the run's dtype);
+ * each swe_advance_chunked_* step is stage 1, stage 2 and a one-block clock
+ * launch that, at a chunk end, writes obs[chunk][gauge] = h + z (member 0;
+ * batch 1) and moves to the next chunk's level.  chunk_series[k] receives the
+ * chunk of step k.  With tape_k1/tape_k2 the states and stage tendencies are
+ * recorded as in swe_advance_tape_* (U_a is then the state tape, U_b unused);
+ * without them U_a/U_b alternate as in swe_advance_*. */
+int swe_run_begin_chunked(void* ws, double chunk_seconds, int n_chunks, const void* levels, int dtype_bytes,
+                          void* stream);
+int swe_advance_chunked_f32(const swe_grid* grid, const swe_config* cfg, const float* bed, const float* roughness,
+                            float* U_a, float* U_b, float* tape_k1, float* tape_k2, int tape_first, int first_step,
+                            int n_steps, float* dt_series, int32_t* chunk_series, int capacity,
+                            const int32_t* gauges, int n_gauges, float* obs, void* ws, int rows_hint, void* stream);
+int swe_advance_chunked_f64(const swe_grid* grid, const swe_config* cfg, const double* bed, const double* roughness,
+                            double* U_a, double* U_b, double* tape_k1, double* tape_k2, int tape_first,
+                            int first_step, int n_steps, double* dt_series, int32_t* chunk_series, int capacity,
+                            const int32_t* gauges, int n_gauges, double* obs, void* ws, int rows_hint,
+                            void* stream);
 
 /* swe_advance_* that also records the adjoint tape of every step: tape_U holds
  * the states (slot j = step tape_first + j; slot 0 is the input, step k writes
@@ -218,6 +245,39 @@ int swe_segment_reverse_f64(const swe_grid* grid, const swe_config* cfg, const d
                             const double* tape_k1, const double* tape_k2, const double* bed, const double* roughness,
                             const double* level, int level_stride, const double* dts, int n_steps, double* lam,
                             double* lam_tmp, double* lam_level, void* adj_ws, int rows_hint, void* stream);
+
+/* Reverse of a chunked run's segment (autograd through the inversion driver's
+ * chunk loop and gauge misfit, bc_inversion_train.py:332-452): chunk_of_step
+ * and ends_chunk are host arrays for the segment's steps (ends_chunk[k] = 1 when
+ * step k ended its chunk, whose observation adjoint g_obs[chunk] [n_gauges] is
+ * then added to lam's h plane at the gauges before step k is reversed); the
+ * level adjoint of step k accumulates into level_grad[chunk_of_step[k]].
+ * The _adjoint_ variant replays the segment from U_ckpt onto tape first (as
+ * swe_segment_adjoint_*). */
+int swe_segment_reverse_chunked_f32(const swe_grid* grid, const swe_config* cfg, const float* tape_U,
+                                    const float* tape_k1, const float* tape_k2, const float* bed,
+                                    const float* roughness, const float* levels, const double* dts,
+                                    const int32_t* chunk_of_step, const int32_t* ends_chunk, int n_steps,
+                                    const int32_t* gauges, int n_gauges, const float* g_obs, float* lam,
+                                    float* lam_tmp, float* level_grad, void* adj_ws, int rows_hint, void* stream);
+int swe_segment_reverse_chunked_f64(const swe_grid* grid, const swe_config* cfg, const double* tape_U,
+                                    const double* tape_k1, const double* tape_k2, const double* bed,
+                                    const double* roughness, const double* levels, const double* dts,
+                                    const int32_t* chunk_of_step, const int32_t* ends_chunk, int n_steps,
+                                    const int32_t* gauges, int n_gauges, const double* g_obs, double* lam,
+                                    double* lam_tmp, double* level_grad, void* adj_ws, int rows_hint, void* stream);
+int swe_segment_adjoint_chunked_f32(const swe_grid* grid, const swe_config* cfg, const float* U_ckpt,
+                                    const float* bed, const float* roughness, const float* levels, const double* dts,
+                                    const int32_t* chunk_of_step, const int32_t* ends_chunk, int n_steps, int parity,
+                                    const int32_t* gauges, int n_gauges, const float* g_obs, float* lam,
+                                    float* lam_tmp, float* level_grad, float* tape, void* fwd_ws, void* adj_ws,
+                                    int rows_hint, void* stream);
+int swe_segment_adjoint_chunked_f64(const swe_grid* grid, const swe_config* cfg, const double* U_ckpt,
+                                    const double* bed, const double* roughness, const double* levels,
+                                    const double* dts, const int32_t* chunk_of_step, const int32_t* ends_chunk,
+                                    int n_steps, int parity, const int32_t* gauges, int n_gauges,
+                                    const double* g_obs, double* lam, double* lam_tmp, double* level_grad,
+                                    double* tape, void* fwd_ws, void* adj_ws, int rows_hint, void* stream);
 
 /* ---- ensemble of forward runs (FP32) -------------------------------------
  * Uneven grids, each with its own adaptive dt, advanced in lockstep by one
