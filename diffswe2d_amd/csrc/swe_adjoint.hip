@@ -470,7 +470,7 @@ int adj_rows(const swe_grid* g, int hint) {
   int strips = (g->nx + 4 + kOutA - 1) / kOutA;
   long long per = (long long)strips * g->batch;
   int S = (int)((long long)(g->ny + 4) * per / 4096);
-  if (S < 8) S = 8;
+  if (S < 2) S = 2;  // small grids: more, shorter strips (latency-bound)
   if (S > 64) S = 64;
   return S;
 }

@@ -255,7 +255,9 @@ int pick_rows(const swe_grid* g, int rows_hint) {
   long long target = 4096;  // wavefronts wanted in flight
   long long per_col = (long long)strips * g->batch;
   int S = (int)((long long)g->ny * per_col / target);
-  if (S < 8) S = 8;
+  // small grids: short strips, so that enough wavefronts run at once; each
+  // wavefront's rows form one dependent chain, whose latency sets the step time
+  if (S < 2) S = 2;
   if (S > 64) S = 64;
   if (S > g->ny) S = g->ny;
   return S;

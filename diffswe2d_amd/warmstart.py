@@ -21,7 +21,7 @@ import time
 import numpy as np
 import torch
 
-from . import inversion, scenarios
+from . import autotune, inversion, scenarios
 from .config import SWEConfig
 from .ensemble import Ensemble
 from .solver import SWE2D
@@ -126,6 +126,8 @@ class InversionSetup:
         self.model = SWE2D(torch.tensor(base["bed"], dtype=dtype), torch.tensor(base["rough"], dtype=dtype), dx, dx,
                            config=runup_config(), train_dem=False).to(device)
         U0 = torch.tensor(base["U"], dtype=dtype, device=device)
+        if dtype == torch.float32:  # strip height for this grid (the step is latency-bound at this size)
+            autotune.tune_rows(self.model, U0, candidates=(1, 2, 4, 8, 16, 32), save=False)
         self.problem = inversion.GaugeProblem(self.model, U0, base["gauges"], chunk=sample_dt, n_chunks=n_chunks)
         times = torch.tensor(p["times"], dtype=torch.float64)
         self.basis = inversion.clamped_bspline_basis(times, n_controls).to(device=device, dtype=dtype)
