@@ -329,51 +329,78 @@ void adj_L_kernel(AdjArgs<T> a) {
 // four sides run as four launches in turn, so a cell reached from two sides
 // (near a corner) is updated in a fixed order and no two threads of a launch
 // touch the same cell.
+// Folds the two ghost layers at position i along `side` of member b; returns
+// their boundary-level adjoint.
 template <class T>
-__global__ __launch_bounds__(256) void adj_ghost_side_kernel(AdjArgs<T> a, int stage, int side) {
-  __shared__ T part[256];
+__device__ inline T ghost_fold_at(const AdjArgs<T>& a, int stage, int side, int b, int i) {
   const Fields<T>& f = a.f;
-  const int b = blockIdx.y;
   const int64_t base = (int64_t)b * 3 * f.plane;
-  const int len = (side == LEFT || side == RIGHT) ? f.ny : f.nx;
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
   T glev_sum = T(0);
-  if (i < len) {
-    for (int layer = 0; layer < 2; ++layer) {
-      int r, c;
-      if (side == LEFT) { r = i; c = -1 - layer; }
-      else if (side == RIGHT) { r = i; c = f.nx + layer; }
-      else if (side == TOP) { r = -1 - layer; c = i; }
-      else { r = f.ny + layer; c = i; }
-      GhostMap g = ghost_map(r, c, f.ny, f.nx, f.bc);
-      int64_t gs = ((((int64_t)b * 4 + side) * 2 + layer) * a.lmax + i) * 3;
-      T gh = a.ghost[gs], gu = a.ghost[gs + 1], gv = a.ghost[gs + 2];
-      T eh, eu, ev;
-      if (stage == 2) load_state<T, 2, AdjArgs<T>>(a, base, g.r, g.c, a.half_dt, eh, eu, ev);
-      else load_state<T, 1, AdjArgs<T>>(a, base, g.r, g.c, a.half_dt, eh, eu, ev);
-      T z = f.bed[(int64_t)b * f.bed_bstride + (int64_t)g.r * f.nx + g.c];
-      T level = (g.kind == BC_WATER_LEVEL || g.kind == BC_ABSORBING) ? f.level[b * f.level_stride] : T(0);
-      T ge_h, ge_u, ge_v, gmir, glev;
-      ghost_bwd(g, eh, z, level, gh, gu, gv, a.k, ge_h, ge_u, ge_v, gmir, glev);
-      int64_t e = base + (int64_t)g.r * f.nx + g.c;
-      a.out[e] += ge_h;
-      a.out[e + f.plane] += ge_u;
-      a.out[e + 2 * f.plane] += ge_v;
-      if (gmir != T(0)) {
-        int64_t m = base + (int64_t)g.mr * f.nx + g.mc;
-        if (side == LEFT || side == RIGHT) a.out[m + f.plane] += gmir;
-        else a.out[m + 2 * f.plane] += gmir;
-      }
-      glev_sum += glev;
+  for (int layer = 0; layer < 2; ++layer) {
+    int r, c;
+    if (side == LEFT) { r = i; c = -1 - layer; }
+    else if (side == RIGHT) { r = i; c = f.nx + layer; }
+    else if (side == TOP) { r = -1 - layer; c = i; }
+    else { r = f.ny + layer; c = i; }
+    GhostMap g = ghost_map(r, c, f.ny, f.nx, f.bc);
+    int64_t gs = ((((int64_t)b * 4 + side) * 2 + layer) * a.lmax + i) * 3;
+    T gh = a.ghost[gs], gu = a.ghost[gs + 1], gv = a.ghost[gs + 2];
+    T eh, eu, ev;
+    if (stage == 2) load_state<T, 2, AdjArgs<T>>(a, base, g.r, g.c, a.half_dt, eh, eu, ev);
+    else load_state<T, 1, AdjArgs<T>>(a, base, g.r, g.c, a.half_dt, eh, eu, ev);
+    T z = f.bed[(int64_t)b * f.bed_bstride + (int64_t)g.r * f.nx + g.c];
+    T level = (g.kind == BC_WATER_LEVEL || g.kind == BC_ABSORBING) ? f.level[b * f.level_stride] : T(0);
+    T ge_h, ge_u, ge_v, gmir, glev;
+    ghost_bwd(g, eh, z, level, gh, gu, gv, a.k, ge_h, ge_u, ge_v, gmir, glev);
+    int64_t e = base + (int64_t)g.r * f.nx + g.c;
+    a.out[e] += ge_h;
+    a.out[e + f.plane] += ge_u;
+    a.out[e + 2 * f.plane] += ge_v;
+    if (gmir != T(0)) {
+      int64_t m = base + (int64_t)g.mr * f.nx + g.mc;
+      if (side == LEFT || side == RIGHT) a.out[m + f.plane] += gmir;
+      else a.out[m + 2 * f.plane] += gmir;
     }
+    glev_sum += glev;
   }
-  part[threadIdx.x] = glev_sum;
+  return glev_sum;
+}
+
+template <class T> __device__ inline T block_sum(T v, T* part) {
+  part[threadIdx.x] = v;
   __syncthreads();
   for (int w = blockDim.x / 2; w > 0; w >>= 1) {
     if (threadIdx.x < w) part[threadIdx.x] += part[threadIdx.x + w];
     __syncthreads();
   }
-  if (threadIdx.x == 0) a.part[((int64_t)b * 4 + side) * a.part_blocks + blockIdx.x] = part[0];
+  return part[0];
+}
+
+template <class T>
+__global__ __launch_bounds__(256) void adj_ghost_side_kernel(AdjArgs<T> a, int stage, int side) {
+  __shared__ T part[256];
+  const int b = blockIdx.y;
+  const int len = (side == LEFT || side == RIGHT) ? a.f.ny : a.f.nx;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  T glev = i < len ? ghost_fold_at(a, stage, side, b, i) : T(0);
+  T sum = block_sum(glev, part);
+  if (threadIdx.x == 0) a.part[((int64_t)b * 4 + side) * a.part_blocks + blockIdx.x] = sum;
+}
+
+// Small grids: all four sides in one block per member (sides in turn, separated
+// by barriers), level adjoint summed in the block -- one launch instead of five.
+template <class T>
+__global__ __launch_bounds__(1024) void adj_ghost_all_kernel(AdjArgs<T> a, int stage, T* lam_level) {
+  __shared__ T part[1024];
+  const int b = blockIdx.x;
+  T glev = T(0);
+  for (int side = 0; side < 4; ++side) {
+    const int len = (side == LEFT || side == RIGHT) ? a.f.ny : a.f.nx;
+    for (int i = threadIdx.x; i < len; i += blockDim.x) glev += ghost_fold_at(a, stage, side, b, i);
+    __syncthreads();
+  }
+  T sum = block_sum(glev, part);
+  if (threadIdx.x == 0 && lam_level != nullptr) lam_level[b] += sum;
 }
 
 // lam_level[b] += sum of the member's ghost-fold partials, in a fixed order
@@ -417,6 +444,7 @@ int afail(int code, const char* msg) {
 inline size_t al256(size_t v) { return (v + 255) & ~(size_t)255; }
 
 constexpr int kGhostThreads = 256;
+constexpr int kGhostOneBlock = 2048;  // longest side folded by one block per member
 
 template <class T> struct AdjWs {
   T *lam_acc, *lam_P, *lam_k, *ghost, *part;  // lam_k holds lam_k2, then lam_k1
@@ -456,6 +484,10 @@ template <class T> AdjWs<T> adj_ws(void* ws, const swe_grid* g) {
 // the ghost fold of one L^T sweep: four side launches, then the level reduction
 template <class T>
 void ghost_fold(const AdjArgs<T>& a, int stage, int batch, T* lam_level, hipStream_t s) {
+  if (a.lmax <= kGhostOneBlock) {
+    hipLaunchKernelGGL((adj_ghost_all_kernel<T>), dim3(batch), dim3(1024), 0, s, a, stage, lam_level);
+    return;
+  }
   dim3 grid((unsigned)a.part_blocks, (unsigned)batch);
   for (int side = 0; side < 4; ++side)
     hipLaunchKernelGGL((adj_ghost_side_kernel<T>), grid, dim3(kGhostThreads), 0, s, a, stage, side);

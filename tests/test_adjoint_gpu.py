@@ -89,6 +89,36 @@ def test_step_adjoint(golden, cuda_device, ieee_oracle, case, dtype):
         assert abs(float(gl) - float(gl_ref)) <= tol * max(1.0, abs(float(gl_ref))) * 10
 
 
+@pytest.mark.parametrize("bc", [("water_level", "wall", "periodic", "periodic"),
+                                ("transmissive", "absorbing", "wall", "water_level")])
+def test_step_adjoint_long_sides_fp64(cuda_device, ieee_oracle, bc):
+    """A grid whose long side exceeds one block's ghost fold (2100 > 2048): the four
+    per-side launches and the partial-sum level reduction against oracle autograd."""
+    ny, nx = 2100, 12
+    g = torch.Generator().manual_seed(5)
+    yy = torch.arange(ny, dtype=torch.float64)[:, None]
+    xx = torch.arange(nx, dtype=torch.float64)[None, :]
+    bed = (-0.5 + 0.01 * xx + 0.05 * torch.sin(0.01 * yy)).expand(ny, nx).contiguous()[None, None]
+    rough = torch.full((1, 1, ny, nx), 0.02, dtype=torch.float64)
+    h = (0.3 - bed[0, 0]).clamp_min(0) + 0.01 * torch.rand(ny, nx, generator=g, dtype=torch.float64)
+    U = torch.stack([h, 0.02 * torch.rand(ny, nx, generator=g, dtype=torch.float64) * h,
+                     0.02 * torch.rand(ny, nx, generator=g, dtype=torch.float64) * h])[None]
+    cfg = swe.SWEConfig(boundary_left=bc[0], boundary_right=bc[1], boundary_top=bc[2], boundary_bottom=bc[3])
+    m = swe.SWE2D(bed, rough, 0.1, 0.1, config=cfg, train_dem=False).to(cuda_device)
+    lvl = torch.tensor(0.35, dtype=torch.float64)
+    lam = torch.randn(U.shape, dtype=torch.float64, generator=g)
+    Ur, lr = U.clone().requires_grad_(True), lvl.clone().requires_grad_(True)
+    out, _ = O.step(Ur, bed, rough, cfg, 0.1, 0.1, 1e30, lr)
+    gU_ref, gl_ref = torch.autograd.grad(out, (Ur, lr), lam)
+    Ug, lg = U.to(cuda_device).requires_grad_(True), lvl.to(cuda_device).requires_grad_(True)
+    outg, _ = m.step(Ug, 0.0, 1e30, boundary_level=lg)
+    gU, gl = torch.autograd.grad(outg, (Ug, lg), lam.to(cuda_device))
+    e = relerr(gU, gU_ref)
+    el = abs(float(gl) - float(gl_ref)) / max(abs(float(gl_ref)), 1e-300)
+    print(f"{bc}: step adjoint rel err {e:.2e}, level grad {float(gl):.6e} vs {float(gl_ref):.6e}")
+    assert e <= 1e-10 and el <= 1e-9
+
+
 @pytest.mark.parametrize("mode", ["replay", "verify", "tape", "mixed"])
 def test_forward_adjoint_with_checkpoints_fp64(cuda_device, ieee_oracle, mode):
     """A 0.3 s forward call (many steps, K = 5) against oracle autograd through the same steps.
