@@ -68,6 +68,29 @@ def basin_model(n, device):
     return model, U, sc, cfg
 
 
+def valu_peak(dev, blocks=8192, iters=4096, reps=3):
+    """Measured FP32 VALU issue ceiling (lane-instructions / s) from the library's probe kernel."""
+    import torch
+
+    from diffswe2d_amd import _lib
+
+    lib = _lib.load()
+    out = torch.empty(blocks * 256, dtype=torch.float32, device=dev)
+    stream = _lib.stream_ptr(dev)
+    _lib.check(lib.swe_valu_probe(_lib.ptr(out), blocks, 16, stream), "valu_probe")
+    best = None
+    for _ in range(reps):
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        _lib.check(lib.swe_valu_probe(_lib.ptr(out), blocks, iters, stream), "valu_probe")
+        e1.record()
+        torch.cuda.synchronize(dev)
+        ms = e0.elapsed_time(e1)
+        best = ms if best is None else min(best, ms)
+    wave_instr = blocks * 4 * iters * 16  # 4 wavefronts per block, 16 VALU instructions per iteration
+    return wave_instr * 64 / (best * 1e-3)
+
+
 def time_kernels(model, U, steps, rows):
     """Average duration of each launch, bracketed by events on torch's current stream."""
     import torch
@@ -302,8 +325,23 @@ def main():
     # HBM bytes per step from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE runs of this workload
     traffic, traffic_src = None, None
     tfile = ROOT / "profiles" / "basin8192_fp32_traffic.json"
+    valu = None
     if n == 8192 and tfile.exists():
-        traffic = round(json.loads(tfile.read_text())["step_hbm_bytes"])
+        prof = json.loads(tfile.read_text())
+        traffic = round(prof["step_hbm_bytes"])
+        # second roof: VALU issue.  Instructions per cell from the committed SQ_INSTS_VALU run of
+        # this workload, the ceiling from the probe kernel measured here.
+        per_cell = prof["stage1"]["valu_lane_instr_per_cell"] + prof["stage2"]["valu_lane_instr_per_cell"]
+        peak = valu_peak(dev)
+        got = per_cell * cells / (step_kernel_ms * 1e-3)
+        valu = {"bound": "valu", "achieved": round(got / 1e12, 2), "peak": round(peak / 1e12, 2),
+                "unit": "T lane-instr/s", "frac": round(got / peak, 4),
+                "lane_instr_per_cell_update": round(per_cell, 1),
+                "source": "profiles/basin8192_fp32_traffic.json (SQ_INSTS_VALU); peak from swe_valu_probe",
+                "note": "peak is the measured issue rate of plain FP32 mul/add; a frac near or above 1 means "
+                        "the stage kernels issue VALU at that ceiling"}
+        print(f"# VALU: {got / 1e12:.2f} of {peak / 1e12:.2f} T lane-instr/s measured by the probe "
+              f"({got / peak:.2f})", file=sys.stderr, flush=True)
         traffic_src = "profiles/basin8192_fp32_traffic.json (FETCH_SIZE x2.0 + WRITE_SIZE x1.0, copy-calibrated)"
 
     cpu = None
@@ -325,6 +363,7 @@ def main():
                      "traffic_source": traffic_src,
                      "kernel": "step (stage1 + stage2)", "algorithmic_bytes_per_cell_update": 32,
                      "kernel_ms": round(step_kernel_ms, 4)},
+        "roofline_valu": valu,
         "kernel_ms": {"stage1": round(s1, 4), "stage2": round(s2, 4)},
         "adjoint": adj,
         "cpu_baseline": cpu,

@@ -19,7 +19,8 @@ ROOT = HERE.parent
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("SWE_OFFLOAD_ARCH", "gfx950")
 
-SOURCES = ["csrc/swe_step.hip", "csrc/swe_adjoint.hip", "csrc/swe_ensemble.hip", "csrc/surrogate.hip"]
+SOURCES = ["csrc/swe_step.hip", "csrc/swe_adjoint.hip", "csrc/swe_ensemble.hip", "csrc/surrogate.hip",
+           "csrc/probe.hip"]
 FLAGS = ["-O3", "-std=c++17", "-fPIC", "-shared", "-ffp-contract=off",
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
          f"--offload-arch={ARCH}", f"-I{ROOT / 'include'}", f"-I{HERE / 'csrc'}"]

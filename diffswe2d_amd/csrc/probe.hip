@@ -1,0 +1,35 @@
+// VALU issue-rate probe: the ceiling the stage kernels are measured against.
+//
+// Each lane runs 8 independent FP32 multiply/add chains (no memory traffic in
+// the loop; 16 non-packed VALU instructions per iteration in the gfx950 ISA),
+// so with every SIMD holding several wavefronts the launch time is set by how
+// fast the CUs issue plain FP32 VALU instructions.
+#include <hip/hip_runtime.h>
+
+#include "../../include/swe2d_hip.h"
+
+namespace {
+
+__global__ __launch_bounds__(256) void valu_probe_kernel(float* out, int iters, float x, float y) {
+  float a0 = threadIdx.x, a1 = a0 + 1, a2 = a0 + 2, a3 = a0 + 3, a4 = a0 + 4, a5 = a0 + 5, a6 = a0 + 6, a7 = a0 + 7;
+  for (int i = 0; i < iters; ++i) {
+    a0 = a0 * x + y;
+    a1 = a1 * x + y;
+    a2 = a2 * x + y;
+    a3 = a3 * x + y;
+    a4 = a4 * x + y;
+    a5 = a5 * x + y;
+    a6 = a6 * x + y;
+    a7 = a7 * x + y;
+  }
+  out[(int64_t)blockIdx.x * blockDim.x + threadIdx.x] = a0 + a1 + a2 + a3 + a4 + a5 + a6 + a7;
+}
+
+}  // namespace
+
+extern "C" int swe_valu_probe(float* out, int blocks, int iters, void* stream) {
+  if (!out || blocks <= 0 || iters <= 0) return SWE_ERR_ARG;
+  hipLaunchKernelGGL(valu_probe_kernel, dim3(blocks), dim3(256), 0, (hipStream_t)stream, out, iters, 0.999f,
+                     0.001f);
+  return hipGetLastError() == hipSuccess ? 0 : SWE_ERR_HIP;
+}

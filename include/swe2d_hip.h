@@ -279,6 +279,14 @@ int swe_segment_adjoint_chunked_f64(const swe_grid* grid, const swe_config* cfg,
                                     const double* g_obs, double* lam, double* lam_tmp, double* level_grad,
                                     double* tape, void* fwd_ws, void* adj_ws, int rows_hint, void* stream);
 
+/* ---- measurement ---------------------------------------------------------
+ * VALU issue-rate probe (no counterpart in the reference): `blocks` blocks of
+ * 256 lanes each run `iters` iterations of 16 independent non-packed FP32
+ * multiply / add instructions, then write one float per lane to out
+ * (blocks * 256 floats).  Timing it gives the FP32 VALU ceiling against which
+ * the stage kernels' instruction counts are priced. */
+int swe_valu_probe(float* out, int blocks, int iters, void* stream);
+
 /* ---- ensemble of forward runs (FP32) -------------------------------------
  * Uneven grids, each with its own adaptive dt, advanced in lockstep by one
  * launch sequence (the reference batches members on one grid with one shared
