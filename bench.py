@@ -39,7 +39,7 @@ def parse():
     ap.add_argument("--rows", type=int, default=0, help="rows per wavefront strip (0 = autotuned / heuristic)")
     ap.add_argument("--kernel-steps", type=int, default=50, help="steps timed launch by launch with events")
     ap.add_argument("--cpu-side", type=int, default=2048, help="crop side of the CPU-baseline sample")
-    ap.add_argument("--cpu-steps", type=int, default=3)
+    ap.add_argument("--cpu-steps", type=int, default=50)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--adjoint-steps", type=int, default=10,
                     help="steps of the adjoint leg (taped forward, reverse sweep, checkpointed backward); 0 to skip")
