@@ -44,6 +44,12 @@ class SweScenario(ctypes.Structure):
                 ("n_gauges", ctypes.c_int32), ("gauge_out", ctypes.c_void_p), ("rows_hint", ctypes.c_int32)]
 
 
+class SweForcing(ctypes.Structure):
+    _fields_ = [("rain_times", ctypes.c_void_p), ("rain_rates", ctypes.c_void_p), ("rain_nt", ctypes.c_int32),
+                ("pad", ctypes.c_int32), ("infiltration", ctypes.c_void_p), ("infil_bstride", ctypes.c_int64),
+                ("time", ctypes.c_double)]
+
+
 class SweControl(ctypes.Structure):
     _fields_ = [("elapsed", ctypes.c_double * 2), ("duration", ctypes.c_double), ("tol", ctypes.c_double),
                 ("cap", ctypes.c_double), ("bad_dt", ctypes.c_double), ("done", ctypes.c_int32),
@@ -58,6 +64,7 @@ _I = ctypes.c_int
 _D = ctypes.c_double
 _G = ctypes.POINTER(SweGrid)
 _C = ctypes.POINTER(SweConfigC)
+_F = ctypes.POINTER(SweForcing)
 
 _SIGNATURES = {
     "swe_abi_version": ([], _I),
@@ -76,6 +83,12 @@ _SIGNATURES = {
     "swe_run_begin": ([_P, _D, _D, _I, _P], _I),
     "swe_advance_f32": ([_G, _C, _P, _P, _P, _I, _P, _P, _I, _I, _P, _P, _I, _P, _I, _P, _P, _I, _P, _I, _P], _I),
     "swe_advance_f64": ([_G, _C, _P, _P, _P, _I, _P, _P, _I, _I, _P, _P, _I, _P, _I, _P, _P, _I, _P, _I, _P], _I),
+    "swe_step_forced_f32": ([_G, _C, _F, _P, _P, _P, _P, _I, _D, _I, _P, _P, _P, _P, _I, _P], _I),
+    "swe_step_forced_f64": ([_G, _C, _F, _P, _P, _P, _P, _I, _D, _I, _P, _P, _P, _P, _I, _P], _I),
+    "swe_advance_forced_f32": ([_G, _C, _F, _P, _P, _P, _I, _P, _P, _I, _I, _P, _P, _I, _P, _I, _P, _P, _I, _P, _I,
+                                _P], _I),
+    "swe_advance_forced_f64": ([_G, _C, _F, _P, _P, _P, _I, _P, _P, _I, _I, _P, _P, _I, _P, _I, _P, _P, _I, _P, _I,
+                                _P], _I),
     "swe_run_begin_chunked": ([_P, _D, _I, _P, _I, _P], _I),
     "swe_advance_chunked_f32": ([_G, _C, _P, _P, _P, _P, _P, _P, _I, _I, _I, _P, _P, _I, _P, _I, _P, _P, _I, _P], _I),
     "swe_advance_chunked_f64": ([_G, _C, _P, _P, _P, _P, _P, _P, _I, _I, _I, _P, _P, _I, _P, _I, _P, _P, _I, _P], _I),

@@ -47,21 +47,29 @@ def _fn(lib, name, dtype):
     return getattr(lib, f"{name}_{'f32' if dtype == torch.float32 else 'f64'}")
 
 
-def record_step(model, U, level, max_dt, read_dt=True):
+def record_step(model, U, level, max_dt, read_dt=True, forcing=None):
     """One forward step that also returns k1, k2 and dt (for replay).
 
     With read_dt False the dt is not read back (no host synchronisation) and
     max_dt is returned: a replay with the recorded dt reproduces it exactly,
-    because the recorded dt never exceeds the CFL value it is the minimum of."""
+    because the recorded dt never exceeds the CFL value it is the minimum of.
+    forcing: None or (infiltration, model time) of a step with source terms."""
     lib = _lib.load()
     bed, rough = model._fields(U)
     ws = model._workspace(U, bed, rough)
     lvl, stride = model._level_arg(level, U)
     out, k1, k2 = torch.empty_like(U), torch.empty_like(U), torch.empty_like(U)
-    _lib.check(_fn(lib, "swe_step_record", U.dtype)(
-        ws["grid"], ws["cfg"], _lib.ptr(U), _lib.ptr(bed), _lib.ptr(rough), _lib.ptr(lvl), stride, float(max_dt),
-        ws["parity"], _lib.ptr(out), _lib.ptr(k1), _lib.ptr(k2), _lib.ptr(ws["buf"]), int(model.rows_hint),
-        _lib.stream_ptr(U.device)), "step_record")
+    if forcing is not None:
+        fz, keep = model._forcing(U, *forcing)
+        _lib.check(_fn(lib, "swe_step_forced", U.dtype)(
+            ws["grid"], ws["cfg"], ctypes.byref(fz), _lib.ptr(U), _lib.ptr(bed), _lib.ptr(rough), _lib.ptr(lvl),
+            stride, float(max_dt), ws["parity"], _lib.ptr(out), _lib.ptr(k1), _lib.ptr(k2), _lib.ptr(ws["buf"]),
+            int(model.rows_hint), _lib.stream_ptr(U.device)), "step_forced")
+    else:
+        _lib.check(_fn(lib, "swe_step_record", U.dtype)(
+            ws["grid"], ws["cfg"], _lib.ptr(U), _lib.ptr(bed), _lib.ptr(rough), _lib.ptr(lvl), stride,
+            float(max_dt), ws["parity"], _lib.ptr(out), _lib.ptr(k1), _lib.ptr(k2), _lib.ptr(ws["buf"]),
+            int(model.rows_hint), _lib.stream_ptr(U.device)), "step_record")
     ws["parity"] ^= 1
     dt = _lib.read_control(ws["buf"]).last_dt if read_dt else float(max_dt)
     return out, k1, k2, dt
@@ -91,10 +99,11 @@ def _level_grad(level, lam_level):
 
 class _StepFn(torch.autograd.Function):
     @staticmethod
-    def forward(ctx, U, level, model, max_dt):
+    def forward(ctx, U, level, model, max_dt, forcing):
         U_ = U.detach().contiguous()
         lvl = level.detach()
-        out, k1, k2, dt = record_step(model, U_, lvl, max_dt)
+        # source terms enter k1 and k2 only; their adjoint with respect to U is zero
+        out, k1, k2, dt = record_step(model, U_, lvl, max_dt, forcing=forcing)
         if not (dt == dt and abs(dt) != float("inf")):
             raise FloatingPointError("The calculated timestep is non-finite.")
         if dt <= 0.0:
@@ -111,12 +120,12 @@ class _StepFn(torch.autograd.Function):
         lam_level = torch.zeros(U.shape[0], dtype=U.dtype, device=U.device)
         lam_U = step_adjoint(ctx.model, U, k1, k2, lvl, ctx.dt, g_out.contiguous(), lam_level)
         g_level = _level_grad(lvl, lam_level) if ctx.needs_input_grad[1] else None
-        return lam_U, g_level, None, None
+        return lam_U, g_level, None, None, None
 
 
-def differentiable_step(model, U, level, max_dt):
+def differentiable_step(model, U, level, max_dt, forcing=None):
     """SWE2D.step with gradients to U and the level; returns (U_new, float dt)."""
-    out, dt = _StepFn.apply(U, level, model, max_dt)
+    out, dt = _StepFn.apply(U, level, model, max_dt, forcing)
     return out, float(dt)
 
 

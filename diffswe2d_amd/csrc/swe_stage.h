@@ -40,6 +40,12 @@ template <class T> struct StageArgs {
   const T* rain;         // optional rain rate field [B or 1, ny, nx]
   const T* infil;        // optional infiltration field
   int64_t rain_bstride, infil_bstride;
+  // optional rainfall record (RainfallForcing, forcing.py:5-38): rates [nt, ny, nx]
+  // at times [nt], interpolated in time by each stage at its own model time
+  const T* rain_times;
+  const T* rain_rates;
+  int rain_nt;
+  double time0;          // model time of the step (host-driven) or of step 0 (device clock)
   typename Bits<T>::U* acc;       // stage 1 writes, stage 2 reads: [2][kSlots]
   typename Bits<T>::U* acc_next;  // stage 2 zeroes for the following step
   Control* ctl;          // control block at the head of the workspace
@@ -68,6 +74,41 @@ template <class T> __device__ inline T step_dt(const typename Bits<T>::U* acc, c
 
 template <class T> __device__ inline bool dt_ok(T dt) { return isfinite(dt) && dt > T(0); }
 
+// Where the rainfall record is sampled at one model time (forcing.py:24-38):
+// right = searchsorted(times, t) clamped to [1, nt-1]; weight from the bracketing
+// pair; times before the first / after the last record hold the end value.
+template <class T> struct RainAt {
+  int left, right, hold;  // hold: -1 interpolate, else the record index to copy
+  T w;
+};
+
+template <class T> __device__ inline RainAt<T> rain_at(const T* times, int nt, double time) {
+  RainAt<T> r;
+  T t = (T)time;  // torch.as_tensor(time, dtype=times.dtype)
+  if (nt == 1) {
+    r.left = r.right = r.hold = 0;
+    r.w = T(0);
+    return r;
+  }
+  int lo = 0, hi = nt;
+  while (lo < hi) {  // first index with times[i] >= t
+    int mid = (lo + hi) >> 1;
+    if (times[mid] < t) lo = mid + 1;
+    else hi = mid;
+  }
+  r.right = lo < 1 ? 1 : (lo > nt - 1 ? nt - 1 : lo);
+  r.left = r.right - 1;
+  T width = clamp_lo(times[r.right] - times[r.left], T(1.0e-12));
+  r.w = (t - times[r.left]) / width;
+  r.hold = t >= times[nt - 1] ? nt - 1 : (t <= times[0] ? 0 : -1);
+  return r;
+}
+
+template <class T> __device__ inline T rain_value(const T* rates, int64_t plane, const RainAt<T>& r, int64_t cell) {
+  if (r.hold >= 0) return rates[(int64_t)r.hold * plane + cell];
+  return (T(1) - r.w) * rates[(int64_t)r.left * plane + cell] + r.w * rates[(int64_t)r.right * plane + cell];
+}
+
 template <class T> __device__ inline T wave_max(T v) {
   for (int off = 32; off > 0; off >>= 1) {
     T o = __shfl_xor(v, off, kWave);
@@ -81,7 +122,7 @@ template <class T> __device__ inline T wave_max(T v) {
 // wave-local maxima of the x- and y-face speeds it evaluated.
 template <class T, int STAGE>
 __device__ inline void stage_strip(const StageArgs<T>& a, int strip, int seg, int b, T dt, T half_dt, T& ax_out,
-                                   T& ay_out) {
+                                   T& ay_out, bool rain_rec = false, RainAt<T> rec = RainAt<T>{0, 0, 0, T(0)}) {
   const int lane = threadIdx.x & (kWave - 1);
   const Fields<T>& f = a.f;
   const SweConsts<T>& k = a.k;
@@ -135,6 +176,7 @@ __device__ inline void stage_strip(const StageArgs<T>& a, int strip, int seg, in
             int64_t i = mbase + (int64_t)o * f.nx + col;
             int64_t cell = (int64_t)o * f.nx + col;
             if (a.rain != nullptr) th = th + a.rain[(int64_t)b * a.rain_bstride + cell];
+            else if (rain_rec) th = th + rain_value(a.rain_rates, f.plane, rec, cell);
             if (a.infil != nullptr) th = th - clamp_lo(a.infil[(int64_t)b * a.infil_bstride + cell], T(0));
             if (STAGE == 1) {
               a.out[i] = th;

@@ -82,7 +82,15 @@ __global__ __launch_bounds__(256) void stage_kernel(StageArgs<T> a) {
   }
 
   T ax, ay;
-  stage_strip<T, STAGE>(a, strip, seg, b, dt, half_dt, ax, ay);
+  RainAt<T> rec{0, 0, 0, T(0)};
+  if (a.rain_rates != nullptr) {
+    // model_time = U.new_tensor(start + elapsed) (solver.py:751-757); stage 2 samples
+    // the record at time + 0.5 dt in double (:634-637)
+    double t = a.clock ? a.time0 + a.ctl->elapsed[a.parity] : a.time0;
+    double ts = (double)(T)t;
+    rec = rain_at<T>(a.rain_times, a.rain_nt, STAGE == 1 ? ts : ts + 0.5 * (double)dt);
+  }
+  stage_strip<T, STAGE>(a, strip, seg, b, dt, half_dt, ax, ay, a.rain_rates != nullptr, rec);
   if (STAGE == 1 && a.record_wave) {
     ax = wave_max(ax);
     ay = wave_max(ay);
@@ -334,10 +342,25 @@ int spatial_operator_impl(const swe_grid* g, const swe_config* c, const T* U, co
   return launch_stage<T, 1>(a, s);
 }
 
+template <class T> int apply_forcing(StageArgs<T>& a, const swe_grid* g, const swe_forcing* fz) {
+  if (fz == nullptr) return 0;
+  if ((fz->rain_rates != nullptr) != (fz->rain_times != nullptr) || (fz->rain_rates && fz->rain_nt < 1))
+    return fail(SWE_ERR_ARG, "rainfall record needs times, rates and rain_nt >= 1");
+  if (fz->infiltration && fz->infil_bstride != 0 && fz->infil_bstride != (int64_t)g->ny * g->nx)
+    return fail(SWE_ERR_ARG, "infiltration member stride must be 0 or ny * nx");
+  a.rain_times = (const T*)fz->rain_times;
+  a.rain_rates = (const T*)fz->rain_rates;
+  a.rain_nt = fz->rain_nt;
+  a.time0 = fz->time;
+  a.infil = (const T*)fz->infiltration;
+  a.infil_bstride = fz->infil_bstride;
+  return 0;
+}
+
 template <class T>
 int step_impl(const swe_grid* g, const swe_config* c, int stages, const T* U, const T* bed, const T* rough,
               const T* level, int level_stride, double max_dt, int parity, T* U_out, void* ws, int rows_hint,
-              void* stream, T* k1_rec = nullptr, T* k2_rec = nullptr) {
+              void* stream, T* k1_rec = nullptr, T* k2_rec = nullptr, const swe_forcing* fz = nullptr) {
   int e;
   if ((e = check_grid(g)) || (e = check_cfg(c)) || (e = check_level(level, level_stride, g, c))) return e;
   if (stages < 1 || stages > 3) return fail(SWE_ERR_ARG, "stages must be 1, 2 or 3");
@@ -348,6 +371,7 @@ int step_impl(const swe_grid* g, const swe_config* c, int stages, const T* U, co
   WsView<T> w = ws_view<T>(ws);
   int p = parity & 1;
   StageArgs<T> a = make_args<T>(g, c, bed, rough, level, level_stride, rows_hint);
+  if ((e = apply_forcing<T>(a, g, fz))) return e;
   T* k1buf = k1_rec != nullptr ? k1_rec : w.k1;
   a.U = U;
   a.out = k1buf;
@@ -376,7 +400,7 @@ int advance_impl(const swe_grid* g, const swe_config* c, const T* bed, const T* 
                  int gauge_capacity, void* ws, int rows_hint, void* stream, T* tape_k1 = nullptr,
                  T* tape_k2 = nullptr, int tape_first = 0, const int32_t* chunk_gauges = nullptr,
                  int chunk_n_gauges = 0, T* chunk_obs = nullptr, int32_t* chunk_series = nullptr,
-                 bool chunked = false) {
+                 bool chunked = false, const swe_forcing* fz = nullptr) {
   int e;
   if ((e = check_grid(g)) || (e = check_cfg(c)) || (e = check_level(level, level_stride, g, c))) return e;
   if (!U_a || (!U_b && !tape_k1) || !bed || !ws || (c->apply_friction && !rough))
@@ -390,6 +414,7 @@ int advance_impl(const swe_grid* g, const swe_config* c, const T* bed, const T* 
   hipStream_t s = (hipStream_t)stream;
   WsView<T> w = ws_view<T>(ws);
   StageArgs<T> a = make_args<T>(g, c, bed, rough, level, level_stride, rows_hint);
+  if ((e = apply_forcing<T>(a, g, fz))) return e;
   a.ctl = w.ctl;
   a.clock = 1;
   a.hmax = hmax;
@@ -573,6 +598,44 @@ int swe_advance_f64(const swe_grid* g, const swe_config* c, const double* bed, c
   return advance_impl<double>(g, c, bed, rough, level, level_stride, U_a, U_b, first_step, n_steps, hmax,
                               dt_series, dt_capacity, gauges, n_gauges, gauge_depth, gauge_level, gauge_capacity,
                               ws, rows_hint, stream);
+}
+
+int swe_step_forced_f32(const swe_grid* g, const swe_config* c, const swe_forcing* fz, const float* U,
+                        const float* bed, const float* rough, const float* level, int level_stride, double max_dt,
+                        int parity, float* U_out, float* k1_out, float* k2_out, void* ws, int rows_hint,
+                        void* stream) {
+  if ((k1_out == nullptr) != (k2_out == nullptr)) return fail(SWE_ERR_ARG, "record both k1 and k2 or neither");
+  return step_impl<float>(g, c, 3, U, bed, rough, level, level_stride, max_dt, parity, U_out, ws, rows_hint, stream,
+                          k1_out, k2_out, fz);
+}
+
+int swe_step_forced_f64(const swe_grid* g, const swe_config* c, const swe_forcing* fz, const double* U,
+                        const double* bed, const double* rough, const double* level, int level_stride, double max_dt,
+                        int parity, double* U_out, double* k1_out, double* k2_out, void* ws, int rows_hint,
+                        void* stream) {
+  if ((k1_out == nullptr) != (k2_out == nullptr)) return fail(SWE_ERR_ARG, "record both k1 and k2 or neither");
+  return step_impl<double>(g, c, 3, U, bed, rough, level, level_stride, max_dt, parity, U_out, ws, rows_hint, stream,
+                           k1_out, k2_out, fz);
+}
+
+int swe_advance_forced_f32(const swe_grid* g, const swe_config* c, const swe_forcing* fz, const float* bed,
+                           const float* rough, const float* level, int level_stride, float* U_a, float* U_b,
+                           int first_step, int n_steps, float* hmax, float* dt_series, int dt_capacity,
+                           const int32_t* gauges, int n_gauges, float* gauge_depth, float* gauge_level,
+                           int gauge_capacity, void* ws, int rows_hint, void* stream) {
+  return advance_impl<float>(g, c, bed, rough, level, level_stride, U_a, U_b, first_step, n_steps, hmax, dt_series,
+                             dt_capacity, gauges, n_gauges, gauge_depth, gauge_level, gauge_capacity, ws, rows_hint,
+                             stream, nullptr, nullptr, 0, nullptr, 0, nullptr, nullptr, false, fz);
+}
+
+int swe_advance_forced_f64(const swe_grid* g, const swe_config* c, const swe_forcing* fz, const double* bed,
+                           const double* rough, const double* level, int level_stride, double* U_a, double* U_b,
+                           int first_step, int n_steps, double* hmax, double* dt_series, int dt_capacity,
+                           const int32_t* gauges, int n_gauges, double* gauge_depth, double* gauge_level,
+                           int gauge_capacity, void* ws, int rows_hint, void* stream) {
+  return advance_impl<double>(g, c, bed, rough, level, level_stride, U_a, U_b, first_step, n_steps, hmax, dt_series,
+                              dt_capacity, gauges, n_gauges, gauge_depth, gauge_level, gauge_capacity, ws, rows_hint,
+                              stream, nullptr, nullptr, 0, nullptr, 0, nullptr, nullptr, false, fz);
 }
 
 int swe_run_begin_chunked(void* ws, double chunk_seconds, int n_chunks, const void* levels, int dtype_bytes,

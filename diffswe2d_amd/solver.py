@@ -15,6 +15,7 @@ Differences a caller can see:
 """
 from __future__ import annotations
 
+import ctypes
 import math
 from numbers import Integral, Real
 from typing import Optional, Union
@@ -242,9 +243,39 @@ class SWE2D(nn.Module):
             "Boundary values must be a scalar, a one-element tensor, or a tensor with one value per batch "
             f"member. Received shape {tuple(lvl.shape)}.")
 
-    def _unsupported_forcing(self, infiltration):
-        if self.rainfall is not None or infiltration is not None:
-            raise NotImplementedError("rainfall / infiltration forcing is not built into the HIP step yet.")
+    def _forcing(self, U, infiltration, time):
+        """Source terms of a step for the library (solver.py:520-525, 609-641), or None.
+
+        Returns (swe_forcing struct, tensors it points at): the rainfall record
+        (``self.rainfall.times`` [nt], ``.rates`` [nt,1,ny,nx]) is sampled on the
+        device at each stage's model time; infiltration is broadcast to the
+        depth shape (one plane when every member shares it)."""
+        if self.rainfall is None and infiltration is None:
+            return None, ()
+        f = _lib.SweForcing()
+        keep = []
+        ny, nx = U.shape[-2:]
+        if self.rainfall is not None:
+            times = self.rainfall.times.to(device=U.device, dtype=U.dtype).contiguous()
+            rates = self.rainfall.rates.to(device=U.device, dtype=U.dtype)
+            nt = times.numel()
+            if rates.shape != (nt, 1, ny, nx):
+                raise ValueError(f"rainfall rates must have shape [{nt},1,{ny},{nx}], got {tuple(rates.shape)}")
+            rates = rates.reshape(nt, ny, nx).contiguous()
+            f.rain_times, f.rain_rates, f.rain_nt = times.data_ptr(), rates.data_ptr(), nt
+            keep += [times, rates]
+        if infiltration is not None:
+            v = (infiltration.to(dtype=U.dtype, device=U.device) if torch.is_tensor(infiltration)
+                 else U.new_tensor(infiltration))
+            v = torch.broadcast_to(v, U[:, 0:1].shape)
+            if U.shape[0] == 1 or bool((v == v[:1]).all()):
+                plane, stride = v[:1].contiguous(), 0
+            else:
+                plane, stride = v.contiguous(), ny * nx
+            f.infiltration, f.infil_bstride = plane.data_ptr(), stride
+            keep.append(plane)
+        f.time = float(time)
+        return f, keep
 
     # ------------------------------------------------------------------ spatial operator
     def spatial_operator(self, U, rainfall_rate=None, infiltration=None, return_wave_speed=False,
@@ -316,27 +347,38 @@ class SWE2D(nn.Module):
         or the level requiring grad, the hand-written adjoint carries gradients.
         """
         self._check_state(U)
-        self._unsupported_forcing(infiltration)
         level = self._prepare_boundary_level(
             self.cfg.constant_level if boundary_level is None else boundary_level, U)
         max_dt_value = self._scalar_float(max_dt, "max_dt") if torch.is_tensor(max_dt) else float(max_dt)
+        forcing = None
+        if self.rainfall is not None or infiltration is not None:
+            # time_float = float(model_time) of a tensor in U's dtype (:608-611)
+            t = model_time.detach() if torch.is_tensor(model_time) else U.new_tensor(model_time)
+            forcing = (infiltration, float(t.cpu()))
         needs_grad = torch.is_grad_enabled() and (U.requires_grad or level.requires_grad)
         if needs_grad:
             from .adjoint import differentiable_step
-            return differentiable_step(self, U, level, max_dt_value)
-        out, dt = self._step_raw(U.detach().contiguous(), level.detach(), max_dt_value)
+            return differentiable_step(self, U, level, max_dt_value, forcing)
+        out, dt = self._step_raw(U.detach().contiguous(), level.detach(), max_dt_value, forcing)
         return out, dt
 
-    def _step_raw(self, U, level, max_dt_value):
+    def _step_raw(self, U, level, max_dt_value, forcing=None):
         bed, rough = self._fields(U)
         ws = self._workspace(U, bed, rough)
         lvl, stride = self._level_arg(level, U)
         lib = _lib.load()
         out = torch.empty_like(U)
-        fn = lib.swe_step_f32 if U.dtype == torch.float32 else lib.swe_step_f64
-        _lib.check(fn(ws["grid"], ws["cfg"], _lib.ptr(U), _lib.ptr(bed), _lib.ptr(rough), _lib.ptr(lvl), stride,
-                      float(max_dt_value), ws["parity"], _lib.ptr(out), _lib.ptr(ws["buf"]), self._rows(U),
-                      _lib.stream_ptr(U.device)), "step")
+        if forcing is not None:
+            fz, keep = self._forcing(U, *forcing)
+            fn = lib.swe_step_forced_f32 if U.dtype == torch.float32 else lib.swe_step_forced_f64
+            _lib.check(fn(ws["grid"], ws["cfg"], ctypes.byref(fz), _lib.ptr(U), _lib.ptr(bed), _lib.ptr(rough),
+                          _lib.ptr(lvl), stride, float(max_dt_value), ws["parity"], _lib.ptr(out), None, None,
+                          _lib.ptr(ws["buf"]), self._rows(U), _lib.stream_ptr(U.device)), "step")
+        else:
+            fn = lib.swe_step_f32 if U.dtype == torch.float32 else lib.swe_step_f64
+            _lib.check(fn(ws["grid"], ws["cfg"], _lib.ptr(U), _lib.ptr(bed), _lib.ptr(rough), _lib.ptr(lvl),
+                          stride, float(max_dt_value), ws["parity"], _lib.ptr(out), _lib.ptr(ws["buf"]),
+                          self._rows(U), _lib.stream_ptr(U.device)), "step")
         ws["parity"] ^= 1
         dt = _lib.read_control(ws["buf"]).last_dt
         if not math.isfinite(dt):
@@ -370,19 +412,22 @@ class SWE2D(nn.Module):
         if duration == 0.0:
             return U0, U0[:, 0].clone()
         self._check_state(U0, "U0")
-        self._unsupported_forcing(infiltration)
         cap = None if max_dt_cap is None else self._scalar_float(max_dt_cap, "max_dt_cap")
         level_req = boundary_level is not None and torch.is_tensor(boundary_level) and boundary_level.requires_grad
         needs_grad = torch.is_grad_enabled() and (U0.requires_grad or level_req)
-        if needs_grad:
+        forced = self.rainfall is not None or infiltration is not None
+        if needs_grad and not forced:
             from .adjoint import differentiable_forward
             return differentiable_forward(self, U0, duration, absolute_start, max_steps, boundary_level, cap)
-        if callback is not None or (boundary_level is None and self.use_dynamic_bc):
-            return self._forward_host(U0, duration, absolute_start, max_steps, callback, boundary_level, cap)
+        if needs_grad or callback is not None or (boundary_level is None and self.use_dynamic_bc):
+            # host loop of (differentiable) steps, like the reference's loop
+            return self._forward_host(U0, duration, absolute_start, max_steps, callback, boundary_level, cap,
+                                      infiltration)
         return self._forward_device(U0.detach().contiguous(), duration, absolute_start, max_steps,
-                                    boundary_level, cap)
+                                    boundary_level, cap, infiltration)
 
-    def _forward_host(self, U0, duration, absolute_start, max_steps, callback, boundary_level, cap):
+    def _forward_host(self, U0, duration, absolute_start, max_steps, callback, boundary_level, cap,
+                      infiltration=None):
         U, hmax, elapsed = U0, U0[:, 0].clone(), 0.0
         tolerance = max(1e-12, 1e-12 * abs(duration))
         for iteration in range(max_steps):
@@ -392,7 +437,8 @@ class SWE2D(nn.Module):
             step_max_dt = remaining if cap is None else min(remaining, cap)
             current_time = absolute_start + elapsed
             level = self._resolve_boundary_level(current_time, U, boundary_level)
-            U, dt_value = self.step(U, model_time=current_time, max_dt=float(U.new_tensor(step_max_dt)),
+            U, dt_value = self.step(U, model_time=U.new_tensor(current_time),
+                                    max_dt=float(U.new_tensor(step_max_dt)), infiltration=infiltration,
                                     boundary_level=level)
             hmax = torch.maximum(hmax, U[:, 0])
             elapsed += dt_value
@@ -405,7 +451,7 @@ class SWE2D(nn.Module):
         raise RuntimeError(
             f"max_steps={max_steps} reached after {elapsed:.8f} s; requested duration={duration:.8f} s.")
 
-    def _forward_device(self, U0, duration, absolute_start, max_steps, boundary_level, cap):
+    def _forward_device(self, U0, duration, absolute_start, max_steps, boundary_level, cap, infiltration=None):
         """Device-driven loop: dt, elapsed time and stopping live on the GPU."""
         lib = _lib.load()
         bed, rough = self._fields(U0)
@@ -432,7 +478,14 @@ class SWE2D(nn.Module):
         gl = torch.empty_like(gd)
         _lib.check(lib.swe_run_begin(_lib.ptr(ws["buf"]), float(duration), -1.0 if cap is None else float(cap),
                                      U0.element_size(), stream), "run_begin")
-        fn = lib.swe_advance_f32 if U0.dtype == torch.float32 else lib.swe_advance_f64
+        fz, keep = self._forcing(U0, infiltration, absolute_start)
+        if fz is None:
+            fn = lib.swe_advance_f32 if U0.dtype == torch.float32 else lib.swe_advance_f64
+        else:
+            forced = lib.swe_advance_forced_f32 if U0.dtype == torch.float32 else lib.swe_advance_forced_f64
+
+            def fn(*args):
+                return forced(args[0], args[1], ctypes.byref(fz), *args[2:])
         launched, chunk = 0, 8
         ctl = None
         while launched < max_steps:

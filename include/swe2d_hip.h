@@ -74,6 +74,24 @@ typedef struct swe_config {
 } swe_config;
 
 /* Device control block at the head of the workspace (read back by the host). */
+/* Source terms of a step (solver.py:520-525, 609-641): a rainfall record
+ * (RainfallForcing, forcing.py:5-38: rates [nt, ny, nx] at increasing times
+ * [nt], both in the run's dtype, shared by all members), sampled by stage 1 at
+ * the step's model time and by stage 2 at time + dt/2; and an infiltration
+ * field ([ny, nx] with infil_bstride 0, or [B, ny, nx] with ny*nx), clamped at 0
+ * and subtracted from the depth tendency.  `time` is the model time of a
+ * host-driven step, or of step 0 of a device-driven run (step k then uses time
+ * + the run's elapsed time, rounded to the run's dtype as the reference's
+ * U.new_tensor(current_time)). */
+typedef struct swe_forcing {
+  const void* rain_times;
+  const void* rain_rates;
+  int32_t rain_nt, pad;
+  const void* infiltration;
+  int64_t infil_bstride;
+  double time;
+} swe_forcing;
+
 typedef struct swe_control {
   double elapsed[2];
   double duration, tol, cap, bad_dt;
@@ -149,6 +167,27 @@ int swe_advance_f64(const swe_grid* grid, const swe_config* cfg, const double* b
                     int n_steps, double* hmax, double* dt_series, int dt_capacity, const int32_t* gauges,
                     int n_gauges, double* gauge_depth, double* gauge_level, int gauge_capacity, void* ws,
                     int rows_hint, void* stream);
+
+/* swe_step_* / swe_advance_* with source terms (swe_forcing); k1_out and k2_out
+ * (both or neither) record the stage tendencies like swe_step_record_*. */
+int swe_step_forced_f32(const swe_grid* grid, const swe_config* cfg, const swe_forcing* forcing, const float* U,
+                        const float* bed, const float* roughness, const float* level, int level_stride,
+                        double max_dt, int parity, float* U_out, float* k1_out, float* k2_out, void* ws,
+                        int rows_hint, void* stream);
+int swe_step_forced_f64(const swe_grid* grid, const swe_config* cfg, const swe_forcing* forcing, const double* U,
+                        const double* bed, const double* roughness, const double* level, int level_stride,
+                        double max_dt, int parity, double* U_out, double* k1_out, double* k2_out, void* ws,
+                        int rows_hint, void* stream);
+int swe_advance_forced_f32(const swe_grid* grid, const swe_config* cfg, const swe_forcing* forcing,
+                           const float* bed, const float* roughness, const float* level, int level_stride,
+                           float* U_a, float* U_b, int first_step, int n_steps, float* hmax, float* dt_series,
+                           int dt_capacity, const int32_t* gauges, int n_gauges, float* gauge_depth,
+                           float* gauge_level, int gauge_capacity, void* ws, int rows_hint, void* stream);
+int swe_advance_forced_f64(const swe_grid* grid, const swe_config* cfg, const swe_forcing* forcing,
+                           const double* bed, const double* roughness, const double* level, int level_stride,
+                           double* U_a, double* U_b, int first_step, int n_steps, double* hmax, double* dt_series,
+                           int dt_capacity, const int32_t* gauges, int n_gauges, double* gauge_depth,
+                           double* gauge_level, int gauge_capacity, void* ws, int rows_hint, void* stream);
 
 /* Chunked run -- replaces the inversion driver's chunk loop
  * (bc_inversion_train.py:332-349, 401-410: forward(U, t_end=chunk,
