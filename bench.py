@@ -237,7 +237,8 @@ def cpu_baseline(sc, cfg, side, steps):
 def main():
     a = parse()
     if a.gpus != 1:
-        print(json.dumps({"error": "this build measures one GPU only; --gpus must be 1"}))
+        print(json.dumps({"error": "the benchmark runs on one GPU; --gpus must be 1 (multi-GPU sharding is "
+                                   "not implemented)"}))
         sys.exit(2)
     import torch
 

@@ -148,7 +148,7 @@ def load(path: Path | None = None):
     p = Path(path) if path is not None else LIB_PATH
     if not p.exists():
         raise RuntimeError(
-            f"HIP library {p} is missing; build it with `python -c 'import __graft_entry__ as g; g.build()'`"
+            f"HIP library {p} is missing; build it with `python -m diffswe2d_amd.build`"
         )
     lib = ctypes.CDLL(str(p))
     for name, (args, res) in _SIGNATURES.items():

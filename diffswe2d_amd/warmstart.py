@@ -208,7 +208,7 @@ if __name__ == "__main__":
     ap.add_argument("--scenarios", type=int, default=256)
     ap.add_argument("--epochs", type=int, default=60)
     ap.add_argument("--surrogate-epochs", type=int, default=40)
-    ap.add_argument("--out", default="gpurun_out/warmstart.json")
+    ap.add_argument("--out", default="warmstart.json")
     a = ap.parse_args()
     rep = run(torch.device("cuda:0"), n_scenarios=a.scenarios, epochs_plain=a.epochs,
               surrogate_epochs=a.surrogate_epochs)

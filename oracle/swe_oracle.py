@@ -1,8 +1,8 @@
 """CPU oracle for the DiffSWE2d per-time-step finite-volume update.
 
-TEST INFRASTRUCTURE ONLY.  This module is the checker: only ``tests/``,
-``__graft_entry__.smoke()`` and the ``cpu_baseline`` leg of ``bench.py`` may
-import it.  The product path (``diffswe2d_amd``) never calls it and fails
+TEST INFRASTRUCTURE ONLY.  This module is the checker: only ``tests/``, the
+package's smoke check and the ``cpu_baseline`` leg of ``bench.py`` may import
+it.  The product path (``diffswe2d_amd``) never calls it and fails
 loudly when its HIP library is missing.
 
 It is an eager PyTorch (CPU) restatement of the reference algorithm, written so
@@ -44,7 +44,7 @@ import torch.nn.functional as F
 NG = 2
 BC_TYPES = ("transmissive", "periodic", "constant", "water_level", "wall", "absorbing")
 
-# torch's CPU sqrt is not correctly rounded on this build (about 0.7 % of
+# torch 2.10's CPU sqrt is not correctly rounded (about 0.7 % of
 # float32 and 0.9 % of float64 inputs land one ulp away from the IEEE result).
 # SQRT_MODE "torch" keeps the reference's own rounding; "ieee" gives the
 # correctly rounded square root that IEEE hardware (and the GPU) computes:
