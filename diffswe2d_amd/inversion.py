@@ -133,8 +133,9 @@ class GaugeProblem:
 
     @staticmethod
     def misfit(obs: torch.Tensor, target: torch.Tensor) -> torch.Tensor:
-        """Mean over gauges of each gauge's mean squared error."""
-        return ((obs - target) ** 2).mean(dim=0).mean()
+        """Mean over gauges of each gauge's mean squared error, accumulated in FP64 on the device."""
+        d = obs.double() - target.double()
+        return (d * d).mean(dim=0).mean()
 
     def objective(self, obs, target, series):
         first, second, amp = smoothness(series)

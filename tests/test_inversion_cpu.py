@@ -30,4 +30,6 @@ def test_misfit_is_mean_of_per_gauge_mse():
     obs = torch.tensor([[1.0, 2.0], [3.0, 5.0]])
     tgt = torch.zeros_like(obs)
     per = ((obs - tgt) ** 2).mean(0)
-    assert torch.isclose(inversion.GaugeProblem.misfit(obs, tgt), per.mean())
+    m = inversion.GaugeProblem.misfit(obs, tgt)
+    assert m.dtype == torch.float64  # accumulated in FP64
+    assert float(m) == float(per.mean()) == (1.0 + 9.0) / 4 + (4.0 + 25.0) / 4
