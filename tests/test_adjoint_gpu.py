@@ -193,6 +193,7 @@ def test_spline_gradient_matches_reference_autograd(golden, cuda_device, dtype):
     obs = prob.simulate(basis @ ctrl)
     loss = prob.misfit(obs, target)
     loss.backward()
+    loss = loss.detach()
     g = ctrl.grad.double().cpu()
     ref = torch.tensor(d[f"grad_{name}"])
     ref64 = torch.tensor(d["grad_f64"])
