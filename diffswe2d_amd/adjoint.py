@@ -19,6 +19,7 @@ correction) and roughness are treated as constants.
 from __future__ import annotations
 
 import ctypes
+import warnings
 
 import torch
 
@@ -90,6 +91,16 @@ def step_adjoint(model, U, k1, k2, level, dt, lam_out, lam_level):
     return lam_U
 
 
+def _warn_constant_bed(model):
+    """The adjoint treats the bed as a constant: say so once when the DEM correction is trainable."""
+    p = getattr(model, "dem_correction_parameter", None)
+    if p is not None and p.requires_grad and not getattr(model, "_warned_constant_bed", False):
+        warnings.warn("the HIP adjoint differentiates with respect to the state and the boundary level only; "
+                      "dem_correction_parameter receives no gradient (construct the model with train_dem=False "
+                      "to silence this)", stacklevel=3)
+        model._warned_constant_bed = True
+
+
 def _level_grad(level, lam_level):
     """Reduce per-member level gradients to the level tensor's shape."""
     if level.numel() == 1:
@@ -125,6 +136,7 @@ class _StepFn(torch.autograd.Function):
 
 def differentiable_step(model, U, level, max_dt, forcing=None):
     """SWE2D.step with gradients to U and the level; returns (U_new, float dt)."""
+    _warn_constant_bed(model)
     out, dt = _StepFn.apply(U, level, model, max_dt, forcing)
     return out, float(dt)
 
@@ -319,6 +331,7 @@ def _replay_checked(model, segments, lvl, dts, lam, lam_level):
 
 def differentiable_forward(model, U0, duration, absolute_start, max_steps, boundary_level, cap):
     """SWE2D.forward with gradients to U0 and an explicit boundary level."""
+    _warn_constant_bed(model)
     if boundary_level is None and model.use_dynamic_bc:
         raise NotImplementedError("gradients through a dynamic boundary series are not supported by the adjoint")
     level = model._prepare_boundary_level(
@@ -358,4 +371,5 @@ class _OperatorFn(torch.autograd.Function):
 
 def operator_adjoint(model, U, level):
     """spatial_operator(U, boundary_level=level) with gradients to U and level."""
+    _warn_constant_bed(model)
     return _OperatorFn.apply(U, level, model)

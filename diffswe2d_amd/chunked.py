@@ -22,7 +22,7 @@ import ctypes
 import torch
 
 from . import _lib
-from .adjoint import _ADJ_MSG, _adj_ws, _fn, tape_budget
+from .adjoint import _ADJ_MSG, _adj_ws, _fn, _warn_constant_bed, tape_budget
 from .solver import sync_parity
 
 _TAPE_SEGMENT = 512     # steps per recorded segment when the tape fits
@@ -205,6 +205,7 @@ def gauge_run(model, U0, series, chunk, gauges, max_steps=10_000_000):
     if not (chunk > 0):
         raise ValueError("chunk duration must be positive")
     if torch.is_grad_enabled() and (U0.requires_grad or levels.requires_grad):
+        _warn_constant_bed(model)
         return _ChunkedFn.apply(U0, levels, model, float(chunk), g, int(max_steps))
     with torch.no_grad():
         obs, U, dts, _, _ = _run(model, U0.detach().contiguous(), levels.detach().contiguous(), float(chunk), g,

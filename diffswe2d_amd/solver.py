@@ -278,6 +278,7 @@ class SWE2D(nn.Module):
         return f, keep
 
     # ------------------------------------------------------------------ spatial operator
+    @torch._dynamo.disable
     def spatial_operator(self, U, rainfall_rate=None, infiltration=None, return_wave_speed=False,
                          boundary_level=None):
         """Tendency dU/dt (src/solver.py:448-531); wave speeds are maxima over the batch."""
@@ -340,6 +341,7 @@ class SWE2D(nn.Module):
         return self.cfg.cfl / denominator
 
     # ------------------------------------------------------------------ time stepping
+    @torch._dynamo.disable
     def step(self, U, model_time, max_dt, infiltration=None, boundary_level=None):
         """One adaptive midpoint step (src/solver.py:572-663); returns (U_new, dt as float).
 
@@ -398,6 +400,7 @@ class SWE2D(nn.Module):
             return float(value)
         raise TypeError(f"{name} must be a real scalar or scalar tensor.")
 
+    @torch._dynamo.disable
     def forward(self, U0, t_end, start_time=0.0, max_steps=1_000_000, infiltration=None, callback=None,
                 boundary_level=None, max_dt_cap=None):
         """Advance for ``t_end`` seconds (src/solver.py:680-779); returns (U, hmax)."""
