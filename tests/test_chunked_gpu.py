@@ -72,7 +72,7 @@ def test_fused_gradient_equals_chunk_loop(cuda_device, mode):
         prob.U0 = U0
         loss = prob.misfit(prob.simulate(s), target)
         gs, gu = torch.autograd.grad(loss, (s, U0))
-        return float(loss), gs, gu
+        return float(loss.detach()), gs, gu
 
     lf, gsf, guf = grads(pf, m)
     kinds = {k for k, _, _ in m.last_adjoint_segments}
