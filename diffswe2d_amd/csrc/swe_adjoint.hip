@@ -330,36 +330,50 @@ void adj_L_kernel(AdjArgs<T> a) {
 // (near a corner) is updated in a fixed order and no two threads of a launch
 // touch the same cell.
 // Folds the two ghost layers at position i along `side` of member b; returns
-// their boundary-level adjoint.
+// their boundary-level adjoint.  The edge state is read only where the ghost
+// rule depends on it (absorbing), the bed only where the level enters (water
+// level, absorbing); both layers' inputs are read before the edge cells are
+// updated, layer 0 first.
 template <class T>
 __device__ inline T ghost_fold_at(const AdjArgs<T>& a, int stage, int side, int b, int i) {
   const Fields<T>& f = a.f;
   const int64_t base = (int64_t)b * 3 * f.plane;
-  T glev_sum = T(0);
+  GhostMap g[2];
+  T gh[2], gu[2], gv[2];
   for (int layer = 0; layer < 2; ++layer) {
     int r, c;
     if (side == LEFT) { r = i; c = -1 - layer; }
     else if (side == RIGHT) { r = i; c = f.nx + layer; }
     else if (side == TOP) { r = -1 - layer; c = i; }
     else { r = f.ny + layer; c = i; }
-    GhostMap g = ghost_map(r, c, f.ny, f.nx, f.bc);
+    g[layer] = ghost_map(r, c, f.ny, f.nx, f.bc);
     int64_t gs = ((((int64_t)b * 4 + side) * 2 + layer) * a.lmax + i) * 3;
-    T gh = a.ghost[gs], gu = a.ghost[gs + 1], gv = a.ghost[gs + 2];
-    T eh, eu, ev;
-    if (stage == 2) load_state<T, 2, AdjArgs<T>>(a, base, g.r, g.c, a.half_dt, eh, eu, ev);
-    else load_state<T, 1, AdjArgs<T>>(a, base, g.r, g.c, a.half_dt, eh, eu, ev);
-    T z = f.bed[(int64_t)b * f.bed_bstride + (int64_t)g.r * f.nx + g.c];
-    T level = (g.kind == BC_WATER_LEVEL || g.kind == BC_ABSORBING) ? f.level[b * f.level_stride] : T(0);
+    gh[layer] = a.ghost[gs];
+    gu[layer] = a.ghost[gs + 1];
+    gv[layer] = a.ghost[gs + 2];
+  }
+  const int kind = g[0].kind;  // both layers belong to the same side
+  const bool uses_level = kind == BC_WATER_LEVEL || kind == BC_ABSORBING;
+  const T level = uses_level ? f.level[b * f.level_stride] : T(0);
+  T glev_sum = T(0);
+  for (int layer = 0; layer < 2; ++layer) {
+    const GhostMap& m = g[layer];
+    T eh = T(0), eu, ev;
+    if (kind == BC_ABSORBING) {
+      if (stage == 2) load_state<T, 2, AdjArgs<T>>(a, base, m.r, m.c, a.half_dt, eh, eu, ev);
+      else load_state<T, 1, AdjArgs<T>>(a, base, m.r, m.c, a.half_dt, eh, eu, ev);
+    }
+    T z = uses_level ? f.bed[(int64_t)b * f.bed_bstride + (int64_t)m.r * f.nx + m.c] : T(0);
     T ge_h, ge_u, ge_v, gmir, glev;
-    ghost_bwd(g, eh, z, level, gh, gu, gv, a.k, ge_h, ge_u, ge_v, gmir, glev);
-    int64_t e = base + (int64_t)g.r * f.nx + g.c;
+    ghost_bwd(m, eh, z, level, gh[layer], gu[layer], gv[layer], a.k, ge_h, ge_u, ge_v, gmir, glev);
+    int64_t e = base + (int64_t)m.r * f.nx + m.c;
     a.out[e] += ge_h;
     a.out[e + f.plane] += ge_u;
     a.out[e + 2 * f.plane] += ge_v;
     if (gmir != T(0)) {
-      int64_t m = base + (int64_t)g.mr * f.nx + g.mc;
-      if (side == LEFT || side == RIGHT) a.out[m + f.plane] += gmir;
-      else a.out[m + 2 * f.plane] += gmir;
+      int64_t mm = base + (int64_t)m.mr * f.nx + m.mc;
+      if (side == LEFT || side == RIGHT) a.out[mm + f.plane] += gmir;
+      else a.out[mm + 2 * f.plane] += gmir;
     }
     glev_sum += glev;
   }
