@@ -259,3 +259,16 @@ def test_error_paths(cuda_device):
         m.forward(U, t_end=-1.0)
     with pytest.raises(RuntimeError, match="ROCm device only"):
         m.step(U.cpu(), 0.0, 1.0)
+
+
+def test_torch_compile_wrapper_runs_the_library_path(cuda_device):
+    """torch.compile(model) must not break the drop-in (the library calls run eagerly)."""
+    n = 24
+    m = swe.SWE2D(torch.zeros(1, 1, n, n), torch.zeros(1, 1, n, n), 0.1, 0.1, train_dem=False).to(cuda_device)
+    U = torch.zeros(1, 3, n, n, device=cuda_device)
+    U[:, 0] = 1.0
+    U[:, 0, : n // 2] = 1.5
+    compiled = torch.compile(m)
+    a, ha = compiled(U, t_end=0.05)
+    b, hb = m(U, t_end=0.05)
+    assert torch.equal(a, b) and torch.equal(ha, hb)
