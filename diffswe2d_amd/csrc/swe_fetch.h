@@ -18,11 +18,13 @@ template <class T> __device__ inline Side<T> shdn(const Side<T>& s) {
   return Side<T>{shdn(s.hlin), shdn(s.eta), shdn(s.z), shdn(s.un), shdn(s.ut)};
 }
 
+// eta = h + z is formed where a Cell is built (the value primitive_cell gives),
+// so the row pipelines carry one register less per row.
 template <class T> struct Prim {
-  T h, u, v, eta, z;
+  T h, u, v, z;
   T hr, qu, qv;  // raw (ghost-applied) state the primitives came from
-  __device__ Cell<T> x() const { return Cell<T>{h, u, v, eta, z}; }
-  __device__ Cell<T> y() const { return Cell<T>{h, v, u, eta, z}; }
+  __device__ Cell<T> x() const { return Cell<T>{h, u, v, h + z, z}; }
+  __device__ Cell<T> y() const { return Cell<T>{h, v, u, h + z, z}; }
 };
 
 // state of interior cell (r, c) as stage STAGE sees it
@@ -64,7 +66,8 @@ __device__ inline Prim<T> fetch(const Args& a, int b, int r, int c, T half_dt) {
     apply_ghost(g, h, hu, hv, mirror, z, level, a.k);
   }
   Prim<T> p;
-  primitive_cell(h, hu, hv, z, a.k, p.h, p.u, p.v, p.eta);
+  T eta;
+  primitive_cell(h, hu, hv, z, a.k, p.h, p.u, p.v, eta);
   p.z = z;
   p.hr = h;
   p.qu = hu;
