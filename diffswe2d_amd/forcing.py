@@ -6,6 +6,7 @@ step kernels, which sample it on the device at each stage's model time (the
 same searchsorted / linear interpolation / end-hold rule as ``at``)."""
 from __future__ import annotations
 
+import numpy as np
 import torch
 import torch.nn as nn
 
@@ -37,3 +38,10 @@ class RainfallForcing(nn.Module):
         value = torch.where(t <= self.times[0], self.rates[0], value)
         value = torch.where(t >= self.times[-1], self.rates[-1], value)
         return value.unsqueeze(0)
+
+
+def load_rainfall_txt(filepath):
+    """Text rainfall record (column 1: time in s, column 2: rain in mm/h) -> (times, rates in m/s),
+    as the reference's loader (src/io_ascii.py:1543-1550)."""
+    data = np.loadtxt(filepath)
+    return data[:, 0], data[:, 1] / (1000.0 * 3600.0)

@@ -4,7 +4,8 @@
 DEM file loading (NetCDF / ESRI-ASCII, model_loader.py:29-121) is outside the
 HIP time-step build: pass the bed and roughness as tensors through
 ``dem_filepath`` / ``model_grid`` as ``(bed, roughness, dx, dy)``, or build
-``SWE2D`` directly.  The tide series branch (:124-130) is kept.
+``SWE2D`` directly.  The text rainfall (:116-118) and tide series (:124-130)
+branches are kept.
 """
 from __future__ import annotations
 
@@ -12,6 +13,7 @@ import numpy as np
 import torch
 
 from .config import SWEConfig
+from .forcing import load_rainfall_txt
 from .solver import SWE2D
 
 
@@ -27,8 +29,10 @@ def load_dynamic_model(dem_filepath, model_grid, device, rain_filepath=None, tid
                 "DEM file loading is not part of the HIP time-step build; pass (bed, roughness, dx, dy) "
                 "tensors instead of a file path.")
         raise ValueError("Unsupported DEM format! Please provide a .nc or .asc file.")
+    rain_times, rain_rate_ms, use_txt_rainfall = None, None, False
     if rain_filepath:
-        raise NotImplementedError("text rainfall forcing is not part of the HIP time-step build")
+        rain_times, rain_rate_ms = load_rainfall_txt(rain_filepath)
+        use_txt_rainfall = True
     bed, rough, dx, dy = dem_filepath
     bed = torch.as_tensor(bed, dtype=torch.float64, device=device)
     rough = (torch.full_like(bed, float(default_roughness_value)) if rough is None
@@ -41,4 +45,4 @@ def load_dynamic_model(dem_filepath, model_grid, device, rain_filepath=None, tid
         model.bc_times = torch.tensor(tide[:, 0], dtype=torch.float64, device=device)
         model.bc_wls = torch.tensor(tide[:, 1], dtype=torch.float64, device=device)
         model.use_dynamic_bc = True
-    return model, False, None, None
+    return model, use_txt_rainfall, rain_times, rain_rate_ms

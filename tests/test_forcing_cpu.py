@@ -13,3 +13,23 @@ def test_rainfall_forcing_checks():
     rf = swe.RainfallForcing(torch.tensor([0.0, 2.0]), torch.stack([torch.zeros(1, 2, 2), torch.ones(1, 2, 2)]))
     assert float(rf.at(1.0)[0, 0, 0, 0]) == 0.5
     assert float(rf.at(-1.0)[0, 0, 0, 0]) == 0.0 and float(rf.at(5.0)[0, 0, 0, 0]) == 1.0
+
+
+def test_load_dynamic_model_branches(tmp_path):
+    n = 8
+    bed = torch.zeros(1, 1, n, n)
+    tide = tmp_path / "tide.txt"
+    tide.write_text("0 0.1\n10 0.3\n")
+    rain = tmp_path / "rain.txt"
+    rain.write_text("0 3.6\n60 36\n")
+    model, use_rain, times, rates = swe.load_dynamic_model((bed, None, 1.0, 1.0), None, "cpu",
+                                                           rain_filepath=str(rain), tide_filepath=str(tide))
+    assert model.cfg.boundary_left == "transmissive" and model.cfg.boundary_bottom == "water_level"
+    assert model.use_dynamic_bc and model.bc_wls.tolist() == [0.1, 0.3] and model.bc_times.tolist() == [0.0, 10.0]
+    assert use_rain and list(times) == [0.0, 60.0]
+    assert abs(rates[0] - 1e-6) < 1e-18 and abs(rates[1] - 1e-5) < 1e-17  # mm/h -> m/s
+    assert float(model.roughness_length.flatten()[0]) == 0.025
+    with pytest.raises(NotImplementedError):
+        swe.load_dynamic_model("dem.nc", None, "cpu")
+    with pytest.raises(ValueError):
+        swe.load_dynamic_model("dem.tif", None, "cpu")

@@ -272,3 +272,33 @@ def test_torch_compile_wrapper_runs_the_library_path(cuda_device):
     a, ha = compiled(U, t_end=0.05)
     b, hb = m(U, t_end=0.05)
     assert torch.equal(a, b) and torch.equal(ha, hb)
+
+
+def test_results_do_not_depend_on_strip_height(cuda_device):
+    """The strip height S only tiles the work: every S gives the same bits."""
+    sc = scenarios.coastal_runup(nx=68, ny=43, dx=0.08)
+    cfg = swe.SWEConfig(boundary_left="water_level", boundary_right="wall", boundary_top="wall",
+                        boundary_bottom="wall")
+    m = swe.SWE2D(torch.tensor(sc["bed"], dtype=torch.float32), torch.tensor(sc["rough"], dtype=torch.float32),
+                  0.08, 0.08, config=cfg, train_dem=False).to(cuda_device)
+    U0 = torch.tensor(sc["U"], dtype=torch.float32, device=cuda_device)
+    outs = []
+    for rows in (1, 2, 5, 16, 43):
+        m.rows_hint = rows
+        with torch.no_grad():
+            outs.append(m(U0, t_end=0.3, boundary_level=0.01))
+    for U, h in outs[1:]:
+        assert torch.equal(U, outs[0][0]) and torch.equal(h, outs[0][1])
+
+
+def test_autotune_picks_a_candidate(cuda_device):
+    from diffswe2d_amd import autotune
+
+    sc = scenarios.coastal_runup(nx=68, ny=43, dx=0.08)
+    m = swe.SWE2D(torch.tensor(sc["bed"], dtype=torch.float32), torch.tensor(sc["rough"], dtype=torch.float32),
+                  0.08, 0.08, train_dem=False).to(cuda_device)
+    U0 = torch.tensor(sc["U"], dtype=torch.float32, device=cuda_device)
+    best, rep = autotune.tune_rows(m, U0, candidates=(2, 4, 8), reps=2, save=False)
+    print(f"tuned rows {best}: {rep}")
+    assert best in (2, 4, 8) and m.rows_hint == best
+    assert set(rep) == {2, 4, 8} and all(v > 0 for v in rep.values())
