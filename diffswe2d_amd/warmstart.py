@@ -145,7 +145,7 @@ class InversionSetup:
 
 
 def run(device, n_scenarios=256, n_controls=64, n_chunks=450, grids=scenarios.RUNUP_GRIDS, epochs_plain=60,
-        surrogate_epochs=40, channels=160, accept=0.5, log=print, seed=5, n_starts=8):
+        surrogate_epochs=40, channels=160, accept=0.5, log=print, seed=5, n_starts=8, truth_seed=4):
     """Full pipeline; returns a report with the time-to-misfit of both inversions.
 
     The plain inversion starts from the prior (level 0) and runs ``epochs_plain``
@@ -155,7 +155,7 @@ def run(device, n_scenarios=256, n_controls=64, n_chunks=450, grids=scenarios.RU
     reported separately (a one-off cost shared by every later inversion).
     """
     report = {}
-    setup = InversionSetup(device, n_controls=n_controls, n_chunks=n_chunks)
+    setup = InversionSetup(device, n_controls=n_controls, n_chunks=n_chunks, seed=truth_seed)
     t0 = time.perf_counter()
     raw_p, hist_p = inversion.adam_invert(setup.problem, setup.basis, setup.target, setup.prior_raw,
                                           epochs=epochs_plain, lr=0.02,
@@ -208,10 +208,11 @@ if __name__ == "__main__":
     ap.add_argument("--scenarios", type=int, default=256)
     ap.add_argument("--epochs", type=int, default=60)
     ap.add_argument("--surrogate-epochs", type=int, default=40)
+    ap.add_argument("--truth-seed", type=int, default=4, help="seed of the truth waveform being inverted")
     ap.add_argument("--out", default="warmstart.json")
     a = ap.parse_args()
     rep = run(torch.device("cuda:0"), n_scenarios=a.scenarios, epochs_plain=a.epochs,
-              surrogate_epochs=a.surrogate_epochs)
+              surrogate_epochs=a.surrogate_epochs, truth_seed=a.truth_seed)
     with open(a.out, "w") as f:
         json.dump(rep, f, indent=1)
     print(json.dumps({k: v for k, v in rep.items() if k not in ("plain", "warm")}))
