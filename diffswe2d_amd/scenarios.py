@@ -18,7 +18,6 @@ Workloads
 """
 from __future__ import annotations
 
-import math
 
 import numpy as np
 
@@ -129,11 +128,6 @@ def tsunami_basin(seed: int = 3, n: int = 8192, dx: float = 100.0, land_fraction
     return dict(U=U, bed=z.astype(np.float32)[None, None],
                 rough=np.full((1, 1, n, n), 0.025, dtype=np.float32), dx=dx, dy=dx,
                 boundaries=("transmissive",) * 4, steps=2000, land=float(land.mean()))
-
-
-def cfl_estimate(h_max: float, dx: float, cfl: float = 0.5, g: float = 9.81) -> float:
-    """dt of a fluid at rest with depth h_max on a square grid."""
-    return cfl * dx / (2.0 * math.sqrt(g * h_max))
 
 
 # ---------------------------------------------------------------------------

@@ -33,3 +33,15 @@ def test_load_dynamic_model_branches(tmp_path):
         swe.load_dynamic_model("dem.nc", None, "cpu")
     with pytest.raises(ValueError):
         swe.load_dynamic_model("dem.tif", None, "cpu")
+
+
+def test_dem_regularization_closed_form():
+    bed = torch.zeros(1, 1, 3, 4)
+    m = swe.SWE2D(bed, torch.zeros_like(bed), 1.0, 1.0)  # train_dem: correction = parameter
+    with torch.no_grad():
+        m.dem_correction_parameter.copy_(torch.arange(12, dtype=torch.float32).reshape(1, 1, 3, 4))
+    # bed = 0..11 row-major: x differences 1, y differences 4, no curvature
+    prior = float(torch.arange(12.0).square().mean())
+    assert abs(float(m.dem_regularization().detach()) - prior) < 1e-4
+    assert abs(float(m.dem_regularization(0.0, 1.0, 0.0).detach()) - (1.0 + 16.0)) < 1e-5
+    assert float(m.dem_regularization(0.0, 0.0, 1.0).detach()) == 0.0

@@ -302,3 +302,46 @@ def test_autotune_picks_a_candidate(cuda_device):
     print(f"tuned rows {best}: {rep}")
     assert best in (2, 4, 8) and m.rows_hint == best
     assert set(rep) == {2, 4, 8} and all(v > 0 for v in rep.values())
+
+
+def test_dynamic_boundary_series_matches_oracle_fp64(cuda_device):
+    """set_dynamic_boundary: the level is interpolated at every step's start time (solver.py:391-415)."""
+    import oracle.swe_oracle as OO
+
+    sc = scenarios.coastal_runup(nx=68, ny=43, dx=0.08)
+    cfg = swe.SWEConfig(boundary_left="water_level", boundary_right="wall", boundary_top="wall",
+                        boundary_bottom="wall")
+    bed = torch.tensor(sc["bed"])
+    rough = torch.tensor(sc["rough"])
+    m = swe.SWE2D(bed, rough, 0.08, 0.08, config=cfg, train_dem=False).to(cuda_device)
+    times = torch.tensor([0.0, 0.2, 0.5, 1.0], dtype=torch.float64)
+    levels = torch.tensor([0.0, 0.02, -0.01, 0.03], dtype=torch.float64)
+    m.set_dynamic_boundary(times, levels)
+    U0 = torch.tensor(sc["U"])
+    old = OO.SQRT_MODE
+    OO.SQRT_MODE = "ieee"
+    try:
+        ref, href = OO.forward(U0, bed, rough, cfg, 0.08, 0.08, 0.6, start_time=0.1, dynamic=(times, levels))
+    finally:
+        OO.SQRT_MODE = old
+    with torch.no_grad():
+        U, h = m(U0.to(cuda_device), t_end=0.6, start_time=0.1)
+    e = relerr(U, ref)
+    eh = relerr(h, href)
+    print(f"dynamic boundary: rel-L2 U {e:.2e}, hmax {eh:.2e}")
+    assert e <= 1e-10 and eh <= 1e-10
+    m.clear_dynamic_boundary()
+    assert not m.use_dynamic_bc
+
+
+@pytest.mark.parametrize("name", ["f32", "f64"])
+def test_stable_dt_matches_golden(golden, cuda_device, name):
+    """SWE2D.stable_dt against the reference's wave speeds (solver.py:537-545) on the golden step case."""
+    d = golden("steps")
+    m, cfg = make_model(d, 1, DT[name], cuda_device)
+    ax, ay = (float(v) for v in d[f"s1_{name}_axay"])
+    ref = cfg.cfl / max(ax / 0.1 + ay / 0.1, cfg.epsilon)
+    U = torch.tensor(d["U0"], dtype=DT[name], device=cuda_device)
+    got = float(m.stable_dt(U, boundary_level=torch.tensor(STEP_CASES[1]["level"], dtype=DT[name])))
+    print(f"{name}: stable dt {got:.9e} vs {ref:.9e}")
+    assert abs(got - ref) <= (1e-6 if name == "f32" else 1e-13) * ref
