@@ -1,12 +1,16 @@
 """MI355X-native (gfx950 HIP) drop-in for DiffSWE2d's per-time-step update.
 
-Public names mirror the reference package (src/__init__.py:9-20) for the
-time-step path: ``SWEConfig``, ``SWE2D``, ``RainfallForcing`` and
-``load_dynamic_model``.
+Public names mirror the reference package (src/__init__.py:1-20) for the
+time-step path and the ESRI-ASCII inputs around it: ``SWEConfig``, ``SWE2D``,
+``RainfallForcing``, ``ModelGrid``, ``ascii_to_tensor``, ``parse_model_grid``
+and ``load_dynamic_model``.
 """
 from .config import SWEConfig
 from .forcing import RainfallForcing
+from .grid import ModelGrid
+from .io_ascii import ascii_to_tensor, parse_model_grid
 from .solver import SWE2D
 from .model_loader import load_dynamic_model
 
-__all__ = ["SWEConfig", "SWE2D", "RainfallForcing", "load_dynamic_model"]
+__all__ = ["SWEConfig", "SWE2D", "RainfallForcing", "ModelGrid", "ascii_to_tensor", "parse_model_grid",
+           "load_dynamic_model"]

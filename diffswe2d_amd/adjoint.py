@@ -38,7 +38,8 @@ def _adj_ws(model, U):
         lib = _lib.load()
         grid = _lib.make_grid(B, ny, nx, bed.shape[0], rough.shape[0], model.dx, model.dy)
         nbytes = lib.swe_adjoint_workspace_bytes(grid, U.element_size())
-        buf = torch.zeros(nbytes, dtype=torch.uint8, device=U.device)
+        with torch.inference_mode(False):
+            buf = torch.zeros(nbytes, dtype=torch.uint8, device=U.device)
         ws = dict(buf=buf, grid=grid)
         model._ws[key] = ws
     return ws

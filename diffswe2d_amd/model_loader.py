@@ -1,13 +1,20 @@
 """``load_dynamic_model`` with the reference signature and defaults
-(src/model_loader.py:6-20).
+(src/model_loader.py:6-133).
 
-DEM file loading (NetCDF / ESRI-ASCII, model_loader.py:29-121) is outside the
-HIP time-step build: pass the bed and roughness as tensors through
-``dem_filepath`` / ``model_grid`` as ``(bed, roughness, dx, dy)``, or build
-``SWE2D`` directly.  The text rainfall (:116-118) and tide series (:124-130)
-branches are kept.
+* ``.asc`` DEMs: ESRI-ASCII through ``SWE2D.from_ascii`` with the reference's
+  choices (linear DEM / nearest roughness regridding, nearest extrapolation,
+  NODATA filled, FP64, ``maximum_dem_correction=1.0``; model_loader.py:68-111);
+  the text rainfall series (:116-118) is read and returned for the caller to
+  apply.
+* ``.nc`` DEMs: NetCDF-4 (HDF5) needs a reader that is not installed, so this
+  branch raises ``NotImplementedError``.
+* A tuple ``(bed, roughness, dx, dy)`` of arrays in place of a path builds the
+  same model from tensors (no file).
+* The optional tide series (:124-130) sets the dynamic boundary level.
 """
 from __future__ import annotations
+
+from pathlib import Path
 
 import numpy as np
 import torch
@@ -23,23 +30,33 @@ def load_dynamic_model(dem_filepath, model_grid, device, rain_filepath=None, tid
                        roughness_var_name="z0", default_roughness_value=0.025):
     """Build an SWE2D with the reference's boundary defaults; returns
     (model, use_txt_rainfall, rain_times, rain_rate_ms)."""
-    if isinstance(dem_filepath, str):
-        if dem_filepath.endswith((".nc", ".asc")):
-            raise NotImplementedError(
-                "DEM file loading is not part of the HIP time-step build; pass (bed, roughness, dx, dy) "
-                "tensors instead of a file path.")
-        raise ValueError("Unsupported DEM format! Please provide a .nc or .asc file.")
+    cfg = SWEConfig(boundary_left=boundary_left, boundary_right=boundary_right, boundary_top=boundary_top,
+                    boundary_bottom=boundary_bottom, frictionmodel=frictionmodel)
     rain_times, rain_rate_ms, use_txt_rainfall = None, None, False
+    if isinstance(dem_filepath, (str, Path)):
+        path = str(dem_filepath)
+        if path.endswith(".nc"):
+            raise NotImplementedError(
+                "NetCDF DEM loading needs an HDF5/NetCDF reader (xarray + netCDF4), which is not available; "
+                "use an ESRI-ASCII (.asc) DEM or pass (bed, roughness, dx, dy) tensors.")
+        if not path.endswith(".asc"):
+            raise ValueError("Unsupported DEM format! Please provide a .nc or .asc file.")
+        rough_kw = ({"roughness_path": roughness_filepath} if roughness_filepath
+                    else {"default_roughness": default_roughness_value})
+        model = SWE2D.from_ascii(path, model_grid=model_grid, config=cfg, dem_method="linear",
+                                 roughness_method="nearest", dem_outside_domain="nearest",
+                                 roughness_outside_domain="nearest", fill_dem_nodata=True,
+                                 fill_roughness_nodata=True, train_dem=False, maximum_dem_correction=1.0,
+                                 device=device, dtype=torch.float64, **rough_kw).to(device)
+    else:
+        bed, rough, dx, dy = dem_filepath
+        bed = torch.as_tensor(bed, dtype=torch.float64, device=device)
+        rough = (torch.full_like(bed, float(default_roughness_value)) if rough is None
+                 else torch.as_tensor(rough, dtype=torch.float64, device=device))
+        model = SWE2D(bed, rough, dx, dy, config=cfg, train_dem=False, maximum_dem_correction=1.0).to(device)
     if rain_filepath:
         rain_times, rain_rate_ms = load_rainfall_txt(rain_filepath)
         use_txt_rainfall = True
-    bed, rough, dx, dy = dem_filepath
-    bed = torch.as_tensor(bed, dtype=torch.float64, device=device)
-    rough = (torch.full_like(bed, float(default_roughness_value)) if rough is None
-             else torch.as_tensor(rough, dtype=torch.float64, device=device))
-    cfg = SWEConfig(boundary_left=boundary_left, boundary_right=boundary_right, boundary_top=boundary_top,
-                    boundary_bottom=boundary_bottom, frictionmodel=frictionmodel)
-    model = SWE2D(bed, rough, dx, dy, config=cfg, train_dem=False, maximum_dem_correction=1.0).to(device)
     if tide_filepath:
         tide = np.loadtxt(tide_filepath)
         model.bc_times = torch.tensor(tide[:, 0], dtype=torch.float64, device=device)

@@ -109,13 +109,47 @@ class SWE2D(nn.Module):
 
     @classmethod
     def from_netcdf(cls, *args, **kwargs):
+        """NetCDF DEM construction (src/solver.py:149-238).
+
+        The reference reads NetCDF-4 (HDF5) files through xarray; no NetCDF /
+        HDF5 reader is installed with this package, so this constructor
+        raises.  Regrid the DEM to tensors elsewhere and call ``SWE2D(...)``,
+        or use ``from_ascii`` for ESRI-ASCII rasters."""
         raise NotImplementedError(
-            "NetCDF DEM loading is not part of the HIP time-step build; construct SWE2D from tensors.")
+            "NetCDF DEM loading needs an HDF5/NetCDF reader (xarray + netCDF4), which is not available; "
+            "construct SWE2D from tensors or use SWE2D.from_ascii.")
 
     @classmethod
-    def from_ascii(cls, *args, **kwargs):
-        raise NotImplementedError(
-            "ESRI-ASCII DEM loading is not part of the HIP time-step build; construct SWE2D from tensors.")
+    def from_ascii(cls, dem_path, *, model_grid, roughness_path=None, config=None, dem_method="linear",
+                   roughness_method="nearest", dem_outside_domain="error", roughness_outside_domain="error",
+                   fill_dem_nodata=True, fill_roughness_nodata=True, default_roughness=None, train_dem=False,
+                   maximum_dem_correction=None, device=None, dtype=torch.float32, **model_kwargs):
+        """Model on ``model_grid`` from ESRI-ASCII DEM / roughness rasters (src/solver.py:240-323)."""
+        from .io_ascii import ascii_to_tensor, parse_model_grid
+
+        spec = parse_model_grid(model_grid)
+        bed, grid = ascii_to_tensor(dem_path, spec, method=dem_method, outside_domain=dem_outside_domain,
+                                    fill_internal_nodata=fill_dem_nodata, device=device, dtype=dtype)
+        if not torch.isfinite(bed).all():
+            raise ValueError("The regridded bed contains non-finite values.")
+        if roughness_path is not None:
+            rough, _ = ascii_to_tensor(roughness_path, spec, method=roughness_method,
+                                       outside_domain=roughness_outside_domain,
+                                       fill_internal_nodata=fill_roughness_nodata, device=device, dtype=dtype)
+        else:
+            rough = torch.full_like(bed, float(default_roughness) if default_roughness is not None else 0.03)
+        if not torch.isfinite(rough).all():
+            raise ValueError("The regridded roughness contains non-finite values.")
+        model = cls(bed, rough, grid.resolution, grid.resolution, config=config, train_dem=train_dem,
+                    maximum_dem_correction=maximum_dem_correction, **model_kwargs)
+        model.model_grid, model.grid_spec = model_grid, grid
+        model.dem_path = str(dem_path)
+        model.roughness_path = None if roughness_path is None else str(roughness_path)
+        model.dem_interpolation_method = dem_method
+        model.roughness_interpolation_method = roughness_method
+        model.dem_outside_domain = dem_outside_domain
+        model.roughness_outside_domain = roughness_outside_domain
+        return model
 
     # ------------------------------------------------------------------ configuration helpers
     def set_dynamic_boundary(self, times: torch.Tensor, water_levels: torch.Tensor, enabled: bool = True) -> None:
@@ -199,7 +233,8 @@ class SWE2D(nn.Module):
         hit = self._field_cache.get("fields")
         if hit is not None and hit[0] == key:
             return hit[1], hit[2]
-        with torch.no_grad():
+        # cached tensors outlive the call: build them as normal tensors even under inference_mode
+        with torch.inference_mode(False), torch.no_grad():
             bed = self.bed.detach().to(device=U.device, dtype=U.dtype).contiguous()
             rough = self.roughness_length.detach().to(device=U.device, dtype=U.dtype).contiguous()
         B = U.shape[0]
@@ -225,7 +260,8 @@ class SWE2D(nn.Module):
             nbytes = lib.swe_workspace_bytes(grid, U.element_size())
             if nbytes == 0:
                 raise ValueError(f"invalid grid {B}x{ny}x{nx}: {lib.swe_last_error_message().decode()}")
-            buf = torch.empty(nbytes, dtype=torch.uint8, device=U.device)
+            with torch.inference_mode(False):
+                buf = torch.empty(nbytes, dtype=torch.uint8, device=U.device)
             _lib.check(lib.swe_workspace_init(_lib.ptr(buf), nbytes, _lib.stream_ptr(U.device)), "workspace init")
             ws = dict(buf=buf, grid=grid, parity=0)
             self._ws[key] = ws

@@ -59,7 +59,7 @@ def test_network_forward_and_gradients(cuda_device, n_cond):
     lr = ((yr - tgt) / 0.02).square().mean()
     lr.backward()
     ef = float((y.detach().cpu() - yr.detach()).abs().max() / yr.detach().abs().max())
-    print(f"forward max rel err {ef:.2e}; loss {float(loss):.6f} vs {float(lr):.6f}")
+    print(f"forward max rel err {ef:.2e}; loss {float(loss.detach()):.6f} vs {float(lr.detach()):.6f}")
     # bf16 activations: a last-bit FP32 difference before a bf16 rounding flips that
     # activation by one bf16 ulp (0.4 %), so outputs agree to ~1e-3, not FP32 precision
     assert ef < 5e-3
