@@ -108,16 +108,43 @@ class SWE2D(nn.Module):
         return self.bed_reference + correction * self.valid_mask.to(self.bed_reference.dtype)
 
     @classmethod
-    def from_netcdf(cls, *args, **kwargs):
-        """NetCDF DEM construction (src/solver.py:149-238).
+    def from_netcdf(cls, dem_path, model_grid, rainfall_path=None, dem_variable=None, roughness_path=None,
+                    roughness_variable=None, rainfall_variable=None, dem_x="x", dem_y="y", rain_x="x", rain_y="y",
+                    rain_time="time", rainfall_units=None, dem_method="linear", default_roughness=None,
+                    roughness_method="linear", dem_outside_domain="error", config=None, dtype=torch.float64,
+                    device=None, train_dem=True, maximum_dem_correction=None):
+        """Model on ``model_grid`` from a NetCDF-4 DEM, optional roughness and gridded rainfall
+        (src/solver.py:149-238)."""
+        from .grid import ModelGrid
+        from .io_netcdf import load_dem_and_roughness_to_grid, load_rainfall_to_grid
 
-        The reference reads NetCDF-4 (HDF5) files through xarray; no NetCDF /
-        HDF5 reader is installed with this package, so this constructor
-        raises.  Regrid the DEM to tensors elsewhere and call ``SWE2D(...)``,
-        or use ``from_ascii`` for ESRI-ASCII rasters."""
-        raise NotImplementedError(
-            "NetCDF DEM loading needs an HDF5/NetCDF reader (xarray + netCDF4), which is not available; "
-            "construct SWE2D from tensors or use SWE2D.from_ascii.")
+        cfg = config or SWEConfig()
+        if isinstance(model_grid, dict):
+            model_grid = ModelGrid.from_bounds(dtype=dtype, device=device, **model_grid)
+        if not isinstance(model_grid, ModelGrid):
+            raise TypeError("model_grid must be a ModelGrid or bounds dictionary.")
+        grid = load_dem_and_roughness_to_grid(dem_path, model_grid, dem_variable, roughness_path=roughness_path,
+                                              roughness_variable=roughness_variable, x_name=dem_x, y_name=dem_y,
+                                              dem_method=dem_method, roughness_method=roughness_method,
+                                              outside_domain=dem_outside_domain, dtype=dtype, device=device)
+        rainfall = None
+        if rainfall_path is not None:
+            rainfall = load_rainfall_to_grid(rainfall_path, model_grid, variable=rainfall_variable,
+                                             time_name=rain_time, x_name=rain_x, y_name=rain_y,
+                                             units=rainfall_units, outside_domain=cfg.rainfall_outside_domain,
+                                             expected_crs=grid.crs, dtype=dtype, device=device)
+        if roughness_variable is not None:
+            roughness = grid.roughness_length
+        else:
+            roughness = torch.full_like(grid.bed, float(default_roughness) if default_roughness is not None
+                                        else 0.025)
+        model = cls(grid.bed, roughness, grid.dx, grid.dy, config=cfg, train_dem=train_dem,
+                    maximum_dem_correction=maximum_dem_correction, valid_mask=grid.valid_mask, x=grid.x,
+                    y=grid.y, rainfall=rainfall)
+        model.model_grid = model_grid
+        model.dem_path = str(dem_path)
+        model.roughness_path = None if roughness_path is None else str(roughness_path)
+        return model
 
     @classmethod
     def from_ascii(cls, dem_path, *, model_grid, roughness_path=None, config=None, dem_method="linear",

@@ -114,3 +114,11 @@ def test_cea_c1_matches_reference_records(tmp_path, cuda_device):
     assert err_log.max() <= 1.5e-6 and err_min.max() <= 5.0e-7 + 1e-8
     assert err_q.max() <= 5.0e-9 + 1e-10
     assert torch.isfinite(hmax).all() and float(hmax.min()) >= 0.0
+    # the run's maximum-depth field (hmax_n_0012.nc): every second row and column, plus whole-field sums
+    m = np.load(GOLDEN / "monai.npz")
+    h = hmax[0].cpu().numpy()
+    e = np.abs(h[::2, ::2] - m["cea_hmax_sub_n"] * 1e-7)
+    s_ref, m_ref, wet_ref, _ = m["cea_hmax_stats"]
+    print(f"  hmax field: max |diff| {e.max():.3e}; sum rel {abs(h.sum() - s_ref) / s_ref:.2e}; max "
+          f"{h.max():.7f} vs {m_ref:.7f}; cells above 1e-6 m {int((h > 1e-6).sum())} vs {int(wet_ref)}")
+    assert e.max() <= 2e-6 + 5e-8 and abs(h.sum() - s_ref) <= 1e-6 * s_ref

@@ -29,7 +29,7 @@ def test_load_dynamic_model_branches(tmp_path):
     assert use_rain and list(times) == [0.0, 60.0]
     assert abs(rates[0] - 1e-6) < 1e-18 and abs(rates[1] - 1e-5) < 1e-17  # mm/h -> m/s
     assert float(model.roughness_length.flatten()[0]) == 0.025
-    with pytest.raises(NotImplementedError):
+    with pytest.raises(TypeError, match="model_grid"):
         swe.load_dynamic_model("dem.nc", None, "cpu")
     with pytest.raises(ValueError):
         swe.load_dynamic_model("dem.tif", None, "cpu")

@@ -163,12 +163,12 @@ def test_cea_dem_reproduces_the_reference_gauge_beds(tmp_path):
 
 def test_unsupported_dem_formats(tmp_path):
     grid = ModelGrid.from_bounds(0.0, 1.0, 0.0, 1.0, 0.5)
-    with pytest.raises(NotImplementedError, match="NetCDF"):
-        swe.load_dynamic_model(dem_filepath="dem.nc", model_grid=grid, device="cpu")
+    with pytest.raises(FileNotFoundError):
+        swe.load_dynamic_model(dem_filepath=str(tmp_path / "dem.nc"), model_grid=grid, device="cpu")
     with pytest.raises(ValueError, match="Unsupported DEM format"):
         swe.load_dynamic_model(dem_filepath="dem.tif", model_grid=grid, device="cpu")
-    with pytest.raises(NotImplementedError, match="NetCDF"):
-        swe.SWE2D.from_netcdf("dem.nc", model_grid=grid)
+    with pytest.raises(TypeError, match="model_grid"):
+        swe.SWE2D.from_netcdf("dem.nc", model_grid=object())
 
 
 def test_from_ascii_roughness_raster(tmp_path):
