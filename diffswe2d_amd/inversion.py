@@ -172,3 +172,73 @@ def adam_invert(problem: GaugeProblem, basis: torch.Tensor, target: torch.Tensor
         if time_budget is not None and el > time_budget:
             break
     return raw.detach(), hist
+
+
+HISTORY_COLUMNS = ("epoch", "total_loss", "train_loss", "test_loss", "first_derivative", "second_derivative",
+                   "amplitude", "gradient_norm", "learning_rate", "elapsed_seconds")
+
+
+def spline_inversion(problem: GaugeProblem, basis: torch.Tensor, target: torch.Tensor, raw0: torch.Tensor,
+                     train, test=(), bounds=(-0.02, 0.05), lr=0.02, max_epochs=500, clip=1.0, min_lr=1.0e-5,
+                     lr_factor=0.5, lr_patience=3, min_delta=1.0e-8, stop_patience=10, log=None):
+    """The spline inversion's training loop (bc_inversion_train.py:625-766).
+
+    Each epoch: dense series = basis @ bounded controls; one simulation of all
+    gauges; objective = mean over the ``train`` gauge columns of each gauge's
+    MSE plus the weighted smoothness terms; backward; the best state is kept
+    by training loss (improvement beyond ``min_delta``); gradient-norm
+    clipping; Adam step; ReduceLROnPlateau on the training loss; early stop
+    after ``stop_patience`` epochs without improvement.  ``test`` columns are
+    reported, never differentiated.
+
+    Returns (best raw controls, best epoch, history); each history row is a
+    dict with ``HISTORY_COLUMNS``, the columns of the reference's
+    loss_history.txt.
+    """
+    train, test = list(train), list(test)
+    raw = torch.nn.Parameter(raw0.detach().clone())
+    opt = torch.optim.Adam([raw], lr=lr, weight_decay=0.0)
+    sched = torch.optim.lr_scheduler.ReduceLROnPlateau(opt, mode="min", factor=lr_factor, patience=lr_patience,
+                                                       threshold=min_delta, threshold_mode="abs", min_lr=min_lr)
+    w1, w2, w3 = problem.weights
+    best, best_epoch, best_raw, waited, history = float("inf"), 0, None, 0, []
+    for index in range(max_epochs):
+        epoch, t0 = index + 1, time.perf_counter()
+        opt.zero_grad(set_to_none=True)
+        series = basis @ bounded_controls(raw, *bounds)
+        obs = problem.simulate(series)
+        train_loss = problem.misfit(obs[:, train], target[:, train])
+        first, second, amp = smoothness(series)
+        total = train_loss + w1 * first + w2 * second + w3 * amp
+        test_loss = problem.misfit(obs[:, test], target[:, test]) if test else total.new_zeros(())
+        total.backward()
+        if not torch.isfinite(total) or raw.grad is None or not torch.isfinite(raw.grad).all():
+            raise FloatingPointError("Non-finite loss or spline-control gradient.")
+        train_value = float(train_loss.detach())
+        grad_norm = float(torch.linalg.vector_norm(raw.grad))
+        if grad_norm == 0.0:
+            raise RuntimeError("All spline-control gradients are zero.")
+        if train_value < best - min_delta:
+            best, best_epoch, best_raw, waited = train_value, epoch, raw.detach().clone(), 0
+        else:
+            waited += 1
+        torch.nn.utils.clip_grad_norm_([raw], clip, error_if_nonfinite=True)
+        opt.step()
+        sched.step(train_value)
+        if raw.is_cuda:
+            torch.cuda.synchronize(raw.device)
+        row = dict(epoch=epoch, total_loss=float(total.detach()), train_loss=train_value,
+                   test_loss=float(test_loss.detach()), first_derivative=float(first.detach()),
+                   second_derivative=float(second.detach()), amplitude=float(amp.detach()),
+                   gradient_norm=grad_norm, learning_rate=opt.param_groups[0]["lr"],
+                   elapsed_seconds=time.perf_counter() - t0)
+        history.append(row)
+        if log:
+            log(f"epoch {epoch:03d}: train {train_value:.6e} test {row['test_loss']:.6e} total "
+                f"{row['total_loss']:.6e} grad {grad_norm:.3e} lr {row['learning_rate']:.3e} "
+                f"({row['elapsed_seconds']:.3f} s)")
+        if waited >= stop_patience:
+            break
+    if best_raw is None:
+        raise RuntimeError("Training did not produce a valid best state.")
+    return best_raw, best_epoch, history

@@ -14,8 +14,17 @@ FP64 runs of 30 s on an A100 wrote: per-step gauge records (gauge_*.txt,
 (Cea2008/Cea2008_C1_new/hmax_n_0012.nc).  Only data files are read, through
 this package's own NetCDF reader; nothing is run.
 
+The inversion example (Examples/inverse/Monai_Valley/
+Monai_Velley_BC_inversion_spline_lr002, bc_inversion_train.py) contributes
+its laboratory gauge record (MonaiValley_WaveGages_30s_02s.txt, metres every
+0.2 s) and the per-epoch history its A100 run wrote
+(inverse_results_spline/loss_history.txt).
+
 Written next to this script:
   monai_bathy.nc, monai_input_wave.txt   the inputs, byte for byte;
+  monai_wave_gauges_02s.txt, monai_inversion_loss_history.txt
+                                          the inversion's observations and
+                                          history, byte for byte;
   monai_hmax_res008.nc                    the 0.08 m run's hmax.nc, byte for byte;
   monai.npz   gauge records (integers in units of 1e-6, as printed; the 0.01 m
               run keeps its first 1000 steps and every 5th after that), the
@@ -69,6 +78,9 @@ def main():
     shutil.copyfile(r1 / "Monai_Bathy.nc", HERE / "monai_bathy.nc")
     shutil.copyfile(r1 / "MonaiValley_InputWave.txt", HERE / "monai_input_wave.txt")
     shutil.copyfile(r8 / "hmax.nc", HERE / "monai_hmax_res008.nc")
+    inv = ex.parent / "inverse" / "Monai_Valley" / "Monai_Velley_BC_inversion_spline_lr002"
+    shutil.copyfile(inv / "MonaiValley_WaveGages_30s_02s.txt", HERE / "monai_wave_gauges_02s.txt")
+    shutil.copyfile(inv / "inverse_results_spline" / "loss_history.txt", HERE / "monai_inversion_loss_history.txt")
     out = {}
     for tag, folder, full, stride in (("r1", r1, 1000, 5), ("r8", r8, None, 1)):
         xy, series = gauges(folder)
