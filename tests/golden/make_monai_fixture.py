@@ -18,13 +18,16 @@ The inversion example (Examples/inverse/Monai_Valley/
 Monai_Velley_BC_inversion_spline_lr002, bc_inversion_train.py) contributes
 its laboratory gauge record (MonaiValley_WaveGages_30s_02s.txt, metres every
 0.2 s) and the per-epoch history its A100 run wrote
-(inverse_results_spline/loss_history.txt).
+(inverse_results_spline/loss_history.txt); its six-gauge variant
+(..._lr002_6simg: simulated gauges in sim_gauges.txt, five of them for
+training) contributes the same two files.
 
 Written next to this script:
   monai_bathy.nc, monai_input_wave.txt   the inputs, byte for byte;
-  monai_wave_gauges_02s.txt, monai_inversion_loss_history.txt
-                                          the inversion's observations and
-                                          history, byte for byte;
+  monai_wave_gauges_02s.txt, monai_inversion_loss_history.txt,
+  monai_sim_gauges.txt, monai_inversion_6simg_loss_history.txt
+                                          the two inversions' observations and
+                                          histories, byte for byte;
   monai_hmax_res008.nc                    the 0.08 m run's hmax.nc, byte for byte;
   monai.npz   gauge records (integers in units of 1e-6, as printed; the 0.01 m
               run keeps its first 1000 steps and every 5th after that), the
@@ -81,6 +84,10 @@ def main():
     inv = ex.parent / "inverse" / "Monai_Valley" / "Monai_Velley_BC_inversion_spline_lr002"
     shutil.copyfile(inv / "MonaiValley_WaveGages_30s_02s.txt", HERE / "monai_wave_gauges_02s.txt")
     shutil.copyfile(inv / "inverse_results_spline" / "loss_history.txt", HERE / "monai_inversion_loss_history.txt")
+    inv6 = inv.parent / "Monai_Velley_BC_inversion_spline_lr002_6simg"
+    shutil.copyfile(inv6 / "sim_gauges.txt", HERE / "monai_sim_gauges.txt")
+    shutil.copyfile(inv6 / "inverse_results_spline" / "loss_history.txt",
+                    HERE / "monai_inversion_6simg_loss_history.txt")
     out = {}
     for tag, folder, full, stride in (("r1", r1, 1000, 5), ("r8", r8, None, 1)):
         xy, series = gauges(folder)

@@ -13,8 +13,10 @@ early stopping (:625-766).  Its A100 run wrote one row per epoch
 (inverse_results_spline/loss_history.txt: losses, penalties, gradient norm,
 learning rate, seconds).
 
-Here: ``inversion.spline_inversion`` on the device-driven chunked run with the
-hand-written adjoint reproduces that history row for row; and one epoch of the
+The same script's six-gauge variant (..._lr002_6simg: simulated gauges,
+five for training) wrote a second history.  Here: ``inversion.spline_inversion``
+on the device-driven chunked run with the hand-written adjoint reproduces both
+histories row for row; and one epoch of the
 reference's own loop (per-chunk ``model(...)`` calls, gradient hooks on the
 state) through this package's public API gives the same loss and gradient.
 """
@@ -31,12 +33,22 @@ pytestmark = pytest.mark.gpu
 
 GOLDEN = Path(__file__).resolve().parent / "golden"
 DT_OUT, N_CHUNKS, N_CONTROLS = 0.2, 149, 20
-GAUGES = {"gauge_0": (4.521, 1.196), "gauge_1": (4.521, 1.696), "gauge_2": (4.521, 2.196)}
+GAUGES = {"gauge_0": (4.521, 1.196), "gauge_1": (4.521, 1.696), "gauge_2": (4.521, 2.196),
+          "gauge_3": (1.000, 1.696), "gauge_4": (2.000, 1.696), "gauge_5": (3.000, 1.696)}
+# scenario a: laboratory gauges 0-2, train on 0 and 1; scenario b (the _6simg variant): simulated gauges
+# 0-5, train on all but 2; gauge 2 is held out in both
+SCENARIOS = {
+    "lab": dict(obs="monai_wave_gauges_02s.txt", history="monai_inversion_loss_history.txt", n_gauges=3,
+                train=(0, 1), test=(2,)),
+    "sim6": dict(obs="monai_sim_gauges.txt", history="monai_inversion_6simg_loss_history.txt", n_gauges=6,
+                 train=(0, 1, 3, 4, 5), test=(2,)),
+}
 BOUNDS = (-0.02, 0.05)
 MAX_EPOCHS = 500  # the script's limit; its run stopped early, at epoch 219
 
 
-def setup(device):
+def setup(device, scenario="lab"):
+    sc = SCENARIOS[scenario]
     grid = swe.ModelGrid.from_bounds(xmin=0.0, xmax=5.44, ymin=0.0, ymax=3.36, resolution=0.08,
                                      dtype=torch.float64, device=device)
     model, _, _, _ = swe.load_dynamic_model(
@@ -48,14 +60,15 @@ def setup(device):
     model.clear_dynamic_boundary()
     model.train()
     cells = []
-    for name, (gx, gy) in GAUGES.items():
+    for name, (gx, gy) in list(GAUGES.items())[:sc["n_gauges"]]:
         ix = int(torch.argmin(torch.abs(grid.x - gx)))
         iy = int(torch.argmin(torch.abs(grid.y - gy)))
         model.gauge_indices[name] = (iy, ix)
         cells.append((iy, ix))
-    data = np.loadtxt(GOLDEN / "monai_wave_gauges_02s.txt", skiprows=1)
+    data = np.loadtxt(GOLDEN / sc["obs"], skiprows=1)
     model_times = np.arange(1, N_CHUNKS + 1, dtype=np.float64) * DT_OUT
-    target = torch.tensor(np.stack([np.interp(model_times, data[:, 0], data[:, c]) for c in (1, 2, 3)], axis=1),
+    cols = range(1, sc["n_gauges"] + 1)
+    target = torch.tensor(np.stack([np.interp(model_times, data[:, 0], data[:, c]) for c in cols], axis=1),
                           dtype=torch.float64, device=device)
     bed = model.bed.detach()
     depth = torch.clamp(0.0 - bed, min=0.0)
@@ -66,29 +79,34 @@ def setup(device):
     return model, cells, target, U0, basis, raw0
 
 
-def reference_history():
-    h = np.loadtxt(GOLDEN / "monai_inversion_loss_history.txt", skiprows=1)
+def reference_history(scenario="lab"):
+    h = np.loadtxt(GOLDEN / SCENARIOS[scenario]["history"], skiprows=1)
     return {c: h[:, i] for i, c in enumerate(inversion.HISTORY_COLUMNS)}
 
 
-def test_monai_inversion_reproduces_the_published_history(cuda_device):
+@pytest.mark.parametrize("scenario", ["lab", "sim6"])
+def test_monai_inversion_reproduces_the_published_history(scenario, cuda_device):
     """The whole published optimisation, epoch for epoch.
 
     Bars: the run stops at the same epoch by the same early-stopping rule, keeps
     the same best epoch, follows the same learning-rate schedule exactly, and
     every loss, penalty and gradient norm agrees within 1e-6 relative.
-    Measured on MI355X: 219 epochs, best epoch 209, learning rate 0.02 down to
-    2.5e-3 on the same epochs; every column within 4.3e-11 relative (the
-    history is printed to 11 significant digits); 0.27 s per epoch against the
-    published 84.7 s.
+    Measured on MI355X, laboratory gauges: 219 epochs, best epoch 209,
+    learning rate 0.02 down to 2.5e-3 on the same epochs; every column within
+    4.3e-11 relative (the history is printed to 11 significant digits); 0.27 s
+    per epoch against the published 84.7 s.  Simulated six gauges: 166
+    epochs, best epoch 156, every column within 4.8e-11; 0.27 s per epoch
+    against 81.4 s.
     """
-    model, cells, target, U0, basis, raw0 = setup(cuda_device)
+    sc = SCENARIOS[scenario]
+    model, cells, target, U0, basis, raw0 = setup(cuda_device, scenario)
     problem = inversion.GaugeProblem(model, U0, cells, DT_OUT, N_CHUNKS)
     lines = []
-    best_raw, best_epoch, hist = inversion.spline_inversion(problem, basis, target, raw0, train=(0, 1), test=(2,),
-                                                            bounds=BOUNDS, max_epochs=MAX_EPOCHS, log=lines.append)
+    best_raw, best_epoch, hist = inversion.spline_inversion(problem, basis, target, raw0, train=sc["train"],
+                                                            test=sc["test"], bounds=BOUNDS, max_epochs=MAX_EPOCHS,
+                                                            log=lines.append)
     print("\n".join(lines[:3] + ["..."] + lines[-2:]))
-    ref = reference_history()
+    ref = reference_history(scenario)
     n = len(hist)
     print(f"  stopped after {n} epochs (published {ref['epoch'].size}), best epoch {best_epoch}")
     assert n == ref["epoch"].size
