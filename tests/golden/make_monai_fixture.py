@@ -1,4 +1,4 @@
-"""Pack the reference's Monai Valley forward examples (and the CEA C1 hmax field) as fixtures.
+"""Pack the reference's Monai Valley forward and inversion examples as test fixtures.
 
 Run in the development container only (it reads the reference tree):
 
@@ -10,9 +10,8 @@ _res008 (driver example_run.py in each): the NetCDF-4 bathymetry
 water level on the left boundary (MonaiValley_InputWave.txt), and what the
 FP64 runs of 30 s on an A100 wrote: per-step gauge records (gauge_*.txt,
 %.6f), the maximum depth field (hmax.nc) and the final maximum depth
-(diffswe_run.log).  Also the CEA C1 run's maximum depth field
-(Cea2008/Cea2008_C1_new/hmax_n_0012.nc).  Only data files are read, through
-this package's own NetCDF reader; nothing is run.
+(diffswe_run.log).  Only data files are read, through this package's own
+NetCDF reader; nothing is run.
 
 The inversion example (Examples/inverse/Monai_Valley/
 Monai_Velley_BC_inversion_spline_lr002, bc_inversion_train.py) contributes
@@ -30,9 +29,10 @@ Written next to this script:
                                           histories, byte for byte;
   monai_hmax_res008.nc                    the 0.08 m run's hmax.nc, byte for byte;
   monai.npz   gauge records (integers in units of 1e-6, as printed; the 0.01 m
-              run keeps its first 1000 steps and every 5th after that), the
-              0.01 m and CEA C1 hmax fields quantised to 1e-9 m, the final
-              maximum depth of each run.
+              run keeps its first 1000 steps and every 5th after that), every
+              second row and column of the 0.01 m run's hmax field (quantised
+              to 1e-7 m) with the full field's sum, maximum, count above
+              1e-6 m and sum of squares, the final maximum depth of each run.
 """
 from __future__ import annotations
 
@@ -101,10 +101,9 @@ def main():
         out[f"{tag}_depth_u"] = np.stack([s[keep, 2] for s in series], axis=1)
         out[f"{tag}_level_u"] = np.stack([s[keep, 3] for s in series], axis=1)
         out[f"{tag}_final_max_u"] = np.int64(final_max_depth(folder))
-    for tag, path in (("r1", r1 / "hmax.nc"), ("cea", ex / "Cea2008" / "Cea2008_C1_new" / "hmax_n_0012.nc")):
-        h = hmax_field(path)
-        out[f"{tag}_hmax_sub_n"] = np.round(h[::2, ::2] * 1e7).astype(np.int32)
-        out[f"{tag}_hmax_stats"] = np.array([h.sum(), h.max(), float((h > 1e-6).sum()), float(np.sum(h * h))])
+    h = hmax_field(r1 / "hmax.nc")
+    out["r1_hmax_sub_n"] = np.round(h[::2, ::2] * 1e7).astype(np.int32)
+    out["r1_hmax_stats"] = np.array([h.sum(), h.max(), float((h > 1e-6).sum()), float(np.sum(h * h))])
     np.savez_compressed(HERE / "monai.npz", **out)
     print({k: v.shape for k, v in out.items()})
 

@@ -133,7 +133,7 @@ def test_model_grid_forms_agree():
 def cea_files(tmp_path):
     d = np.load(GOLDEN / "cea_c1.npz")
     topo = tmp_path / "topo.asc"
-    topo.write_bytes(gzip.decompress((GOLDEN / "cea_c1_topo.asc.gz").read_bytes()))
+    topo.write_bytes(gzip.decompress((GOLDEN / "cea_topo.asc.gz").read_bytes()))
     rain = tmp_path / "rain.txt"
     np.savetxt(rain, d["rain"])
     return d, topo, rain
