@@ -13,6 +13,7 @@
 // so the kernels use no LDS and no barriers.  State is SoA planes h, hu, hv.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <math.h>
 #include <stdint.h>
 #include <string.h>
@@ -103,6 +104,33 @@ __global__ __launch_bounds__(256) void stage_kernel(StageArgs<T> a) {
 }
 
 // ------------------------------------------------------------------ control kernels
+// Water volume sum h * dx * dy of every member, in FP64 and deterministic: each block
+// sums a fixed stride of the member's depth plane, one wave per member adds the
+// block partials in index order.
+template <class T>
+__global__ __launch_bounds__(256) void volume_partial_kernel(const T* __restrict__ U, int64_t plane, int nblk,
+                                                             double* __restrict__ partial) {
+  const int b = blockIdx.y;
+  const T* h = U + (int64_t)b * 3 * plane;
+  double s = 0.0;
+  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < plane; i += (int64_t)nblk * 256) s += (double)h[i];
+  for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+  __shared__ double wave_sum[4];
+  if ((threadIdx.x & 63) == 0) wave_sum[threadIdx.x >> 6] = s;
+  __syncthreads();
+  if (threadIdx.x == 0)
+    partial[(int64_t)b * nblk + blockIdx.x] = (wave_sum[0] + wave_sum[1]) + (wave_sum[2] + wave_sum[3]);
+}
+
+__global__ __launch_bounds__(64) void volume_finish_kernel(const double* __restrict__ partial, int nblk,
+                                                           double cell_area, double* __restrict__ out) {
+  const int b = blockIdx.x;
+  double s = 0.0;
+  for (int i = threadIdx.x; i < nblk; i += 64) s += partial[(int64_t)b * nblk + i];
+  for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+  if (threadIdx.x == 0) out[b] = s * cell_area;
+}
+
 __global__ void zero_words(uint32_t* p, int64_t n) {
   int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (i < n) p[i] = 0u;
@@ -484,6 +512,23 @@ int advance_chunked(const swe_grid* g, const swe_config* c, const T* bed, const 
 }  // namespace
 
 // ===================================================================== C ABI
+template <class T>
+int water_volume_impl(const swe_grid* g, const T* U, double* volume, void* ws, void* stream) {
+  if (check_grid(g)) return g_last_error;
+  if (!U || !volume || !ws) return fail(SWE_ERR_ARG, "U, volume and ws are required");
+  const int64_t plane = (int64_t)g->ny * g->nx;
+  // partials borrow the stage-tendency region of the workspace (3 * plane * sizeof(T) bytes per member),
+  // which holds nothing between steps
+  const int64_t room = 3 * plane * (int64_t)sizeof(T) / 8;
+  const int nblk = (int)std::max<int64_t>(1, std::min<int64_t>({1024, room, (plane + 255) / 256}));
+  double* partial = (double*)ws_view<T>(ws).k1;
+  hipStream_t s = (hipStream_t)stream;
+  hipLaunchKernelGGL(volume_partial_kernel<T>, dim3(nblk, g->batch), dim3(256), 0, s, U, plane, nblk, partial);
+  hipLaunchKernelGGL(volume_finish_kernel, dim3(g->batch), dim3(64), 0, s, partial, nblk, g->dx * g->dy, volume);
+  hipError_t e = hipGetLastError();
+  return e == hipSuccess ? 0 : fail(SWE_ERR_HIP, hipGetErrorString(e));
+}
+
 extern "C" {
 
 int swe_abi_version(void) { return SWE_ABI_VERSION; }
@@ -689,6 +734,14 @@ int swe_advance_tape_f64(const swe_grid* g, const swe_config* c, const double* b
   return advance_impl<double>(g, c, bed, rough, level, level_stride, tape_U, nullptr, first_step, n_steps, hmax,
                               dt_series, dt_capacity, nullptr, 0, nullptr, nullptr, 0, ws, rows_hint, stream, tape_k1,
                               tape_k2, tape_first);
+}
+
+int swe_water_volume_f32(const swe_grid* g, const float* U, double* volume, void* ws, void* stream) {
+  return water_volume_impl<float>(g, U, volume, ws, stream);
+}
+
+int swe_water_volume_f64(const swe_grid* g, const double* U, double* volume, void* ws, void* stream) {
+  return water_volume_impl<double>(g, U, volume, ws, stream);
 }
 
 }  // extern "C"

@@ -403,6 +403,21 @@ class SWE2D(nn.Module):
         denominator = torch.clamp(ax / self.dx + ay / self.dy, min=self.cfg.epsilon)
         return self.cfg.cfl / denominator
 
+    def water_volume(self, U):
+        """Water volume sum(h) * dx * dy of each member, as an FP64 tensor [B] on U's device.
+
+        Summed on the device in FP64 in a fixed order (the same bits every
+        call); the reference has no such diagnostic.  For conservation checks."""
+        self._check_state(U, "U")
+        bed, rough = self._fields(U)
+        ws = self._workspace(U, bed, rough)
+        state = U.detach().contiguous()
+        out = torch.empty(U.shape[0], dtype=torch.float64, device=U.device)
+        fn = getattr(_lib.load(), f"swe_water_volume_{_DTYPES[U.dtype]}")
+        _lib.check(fn(ws["grid"], _lib.ptr(state), _lib.ptr(out), _lib.ptr(ws["buf"]), _lib.stream_ptr(U.device)),
+                   "water_volume")
+        return out
+
     # ------------------------------------------------------------------ time stepping
     @torch._dynamo.disable
     def step(self, U, model_time, max_dt, infiltration=None, boundary_level=None):

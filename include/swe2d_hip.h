@@ -326,6 +326,16 @@ int swe_segment_adjoint_chunked_f64(const swe_grid* grid, const swe_config* cfg,
  * the stage kernels' instruction counts are priced. */
 int swe_valu_probe(float* out, int blocks, int iters, void* stream);
 
+/* ---- water volume (FP64) -------------------------------------------------
+ * volume[b] = sum over the interior cells of member b of h * dx * dy, summed
+ * in FP64 in a fixed order (the same bits on every call).  volume is a device
+ * array of grid->batch doubles.  Uses the workspace's stage-tendency region
+ * as scratch, so it must not overlap a step on the same workspace.  No
+ * counterpart in the reference (which has no volume diagnostic); it is the
+ * FP64 mass-conservation sum used by the conservation tests and bench.py. */
+int swe_water_volume_f32(const swe_grid* grid, const float* U, double* volume, void* ws, void* stream);
+int swe_water_volume_f64(const swe_grid* grid, const double* U, double* volume, void* ws, void* stream);
+
 /* ---- ensemble of forward runs (FP32) -------------------------------------
  * Uneven grids, each with its own adaptive dt, advanced in lockstep by one
  * launch sequence (the reference batches members on one grid with one shared

@@ -345,3 +345,45 @@ def test_stable_dt_matches_golden(golden, cuda_device, name):
     got = float(m.stable_dt(U, boundary_level=torch.tensor(STEP_CASES[1]["level"], dtype=DT[name])))
     print(f"{name}: stable dt {got:.9e} vs {ref:.9e}")
     assert abs(got - ref) <= (1e-6 if name == "f32" else 1e-13) * ref
+
+
+@pytest.mark.parametrize("name", ["f32", "f64"])
+def test_water_volume_fp64_sum(cuda_device, name):
+    """SWE2D.water_volume: FP64 sum of h * dx * dy per member, the same bits on every call."""
+    dtype = DT[name]
+    g = torch.Generator().manual_seed(11)
+    for B, ny, nx in ((2, 37, 53), (1, 700, 1000), (3, 2, 2)):
+        bed = torch.zeros(1, 1, ny, nx, dtype=dtype)
+        m = swe.SWE2D(bed, torch.zeros_like(bed), 0.3, 0.7, train_dem=False).to(cuda_device)
+        U = torch.rand(B, 3, ny, nx, generator=g, dtype=dtype).to(cuda_device)
+        v1, v2 = m.water_volume(U), m.water_volume(U)
+        ref = U[:, 0].double().cpu().numpy().sum(axis=(1, 2)) * 0.3 * 0.7
+        err = float(np.max(np.abs(v1.cpu().numpy() - ref) / ref))
+        print(f"{name} {B}x{ny}x{nx}: volume rel err {err:.2e}")
+        assert v1.dtype == torch.float64 and v1.shape == (B,)
+        assert torch.equal(v1, v2) and err <= 1e-13
+
+
+@pytest.mark.parametrize("name,bar", [("f64", 1e-12), ("f32", 1e-6)])
+def test_volume_is_conserved_on_a_periodic_basin(cuda_device, name, bar):
+    """All-periodic boundaries, no dry cells, no friction: the finite-volume update conserves
+    the water volume up to rounding (the FP64 sum measures it)."""
+    dtype = DT[name]
+    n = 96
+    y, x = torch.meshgrid(torch.arange(n, dtype=torch.float64), torch.arange(n, dtype=torch.float64), indexing="ij")
+    bed = (0.2 * torch.sin(2 * math.pi * x / n) * torch.cos(2 * math.pi * y / n))[None, None]
+    h = 1.0 + 0.3 * torch.exp(-((x - 40) ** 2 + (y - 50) ** 2) / 60.0)[None, None] - bed
+    U = torch.cat((h, 0.05 * torch.ones_like(h), torch.zeros_like(h)), dim=1).to(dtype)
+    cfg = swe.SWEConfig(boundary_left="periodic", boundary_right="periodic", boundary_top="periodic",
+                        boundary_bottom="periodic", apply_friction=False)
+    m = swe.SWE2D(bed.to(dtype), torch.zeros_like(bed, dtype=dtype), 0.5, 0.5, config=cfg,
+                  train_dem=False).to(cuda_device)
+    U = U.to(cuda_device)
+    v0 = float(m.water_volume(U)[0])
+    with torch.no_grad():
+        U, _ = m(U, t_end=2.0)
+    v1 = float(m.water_volume(U)[0])
+    steps = len(m.last_dt_series)
+    print(f"{name}: {steps} steps, volume {v0:.15e} -> {v1:.15e}, rel change {abs(v1 - v0) / v0:.2e}")
+    assert steps > 50 and float(U[:, 0].min()) > 0.1
+    assert abs(v1 - v0) <= bar * v0
