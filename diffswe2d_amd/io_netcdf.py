@@ -1,9 +1,10 @@
-"""NetCDF-4 DEM, roughness and rainfall inputs remapped to a model grid (src/io_netcdf.py).
+"""NetCDF DEM, roughness and rainfall inputs remapped to a model grid (src/io_netcdf.py).
 
 The reference opens files with xarray (``decode_cf=True, mask_and_scale=True``)
 and interpolates with ``DataArray.interp``.  Neither xarray nor netCDF4 is
-installed here, so files are read by ``diffswe2d_amd.hdf5`` (NetCDF-4 is
-HDF5 underneath) and decoded the way xarray decodes them:
+installed here, so NetCDF-4 files are read by ``diffswe2d_amd.hdf5`` (NetCDF-4
+is HDF5 underneath), NetCDF-3 files by scipy's classic-format reader, and both
+are decoded the way xarray decodes them:
 
 * variables are the root group's datasets other than pure dimensions; a
   variable's dimension names come from its ``_Netcdf4Coordinates`` dimension
@@ -83,12 +84,40 @@ class Variable:
 
 
 class Dataset:
-    """Root-group variables of a NetCDF-4 file, CF-decoded (xarray's mask_and_scale)."""
+    """Root-group variables of a NetCDF-4 or NetCDF-3 file, CF-decoded (xarray's mask_and_scale)."""
 
     def __init__(self, path):
         self.path = Path(path)
         if not self.path.exists():
             raise FileNotFoundError(f"NetCDF file not found: {self.path}")
+        with open(self.path, "rb") as fh:
+            magic = fh.read(4)
+        if magic[:3] == b"CDF":
+            self._open_classic()
+        else:
+            self._open_hdf5()
+        for v in self.variables.values():
+            v.coords = self.coords
+        self.data_vars = {n: v for n, v in self.variables.items() if n not in self.coords}
+
+    def _open_classic(self):
+        """NetCDF-3 (classic / 64-bit offset) through scipy's reader."""
+        from scipy.io import netcdf_file
+
+        def text(v):
+            return v.decode("utf-8", "replace") if isinstance(v, bytes) else v
+
+        with netcdf_file(self.path, "r", mmap=False, maskandscale=False) as f:
+            self.attrs = {k: text(v) for k, v in f._attributes.items()}
+            self.variables = {}
+            for name, var in f.variables.items():
+                attrs = {k: text(v) for k, v in var._attributes.items()}
+                values = np.array(var.data, copy=True)
+                self.variables[name] = Variable(name, self._decode(values, attrs), var.dimensions, attrs)
+        self.coords = {n: v.values for n, v in self.variables.items() if v.dims == (n,)}
+
+    def _open_hdf5(self):
+        """NetCDF-4: HDF5 datasets of the root group."""
         with hdf5.File(self.path) as f:
             self.attrs = dict(f.root_attrs)
             raw = {name: f.dataset(name) for name in f.links}
@@ -118,10 +147,7 @@ class Dataset:
             for name in dim_names:
                 if name in self.variables:
                     coords[name] = self.variables[name].values
-        for v in self.variables.values():
-            v.coords = coords
         self.coords = coords
-        self.data_vars = {n: v for n, v in self.variables.items() if n not in coords}
 
     @staticmethod
     def _dims_by_size(name, shape, dim_names):

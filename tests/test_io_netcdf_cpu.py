@@ -1,4 +1,4 @@
-"""NetCDF-4 input path (src/io_netcdf.py, src/solver.py:149-238, model_loader.py:29-64) on the CPU.
+"""NetCDF input path (src/io_netcdf.py, src/solver.py:149-238, model_loader.py:29-64) on the CPU.
 
 The HDF5 reader is pinned by the reference's own files: the Monai bathymetry
 (chunked, shuffle + deflate) must reproduce the range its ``actual_range``
@@ -7,7 +7,10 @@ imply at nine gauge cells of two grids; the 0.08 m run's hmax.nc (contiguous)
 must equal, at each gauge cell, the largest depth printed in that gauge's
 per-step record of the same run.  Decoding and remapping rules are pinned by
 closed forms (a plane survives linear remapping; conservative remapping keeps
-the total; CF masks and scales).
+the total; CF masks and scales), on files of both NetCDF-4 and NetCDF-3 (written
+here with scipy's classic-format writer), and the HDF5 filter pieces the
+shipped files do not use (Fletcher-32, filter masks, version-1 pipeline
+messages) on hand-built bytes.
 """
 from pathlib import Path
 
@@ -144,3 +147,84 @@ def test_from_netcdf_constructor_and_errors(tmp_path):
     not_h5.write_bytes(b"CDF\x01" + bytes(60))
     with pytest.raises(hdf5.HDF5Error):
         hdf5.File(not_h5)
+
+
+def write_classic(path, variables, dims, attrs=None):
+    """A NetCDF-3 file through scipy's writer: variables = {name: (dims, array, attrs)}."""
+    from scipy.io import netcdf_file
+
+    with netcdf_file(path, "w") as f:
+        for k, v in (attrs or {}).items():
+            setattr(f, k, v)
+        for name, size in dims.items():
+            f.createDimension(name, size)
+        for name, (vdims, arr, vattrs) in variables.items():
+            var = f.createVariable(name, arr.dtype.char, vdims)
+            var[:] = arr
+            for k, v in vattrs.items():
+                setattr(var, k, v)
+    return path
+
+
+def test_classic_netcdf_dem_roughness_and_rainfall(tmp_path):
+    x = np.linspace(0.0, 4.0, 9)
+    y = np.linspace(3.0, 0.0, 7)  # stored north to south
+    z = (0.3 + 0.5 * x[None, :] - 0.2 * y[:, None]).astype(np.float32)
+    n = np.full((7, 9), 0.02, dtype=np.float32)
+    dem = write_classic(tmp_path / "dem.nc", {"x": (("x",), x, {}), "y": (("y",), y, {}),
+                                              "elevation": (("y", "x"), z, {"units": "m"}),
+                                              "n": (("y", "x"), n, {})}, {"x": 9, "y": 7})
+    ds = io_netcdf.open_dataset(dem)
+    assert sorted(ds.data_vars) == ["elevation", "n"] and ds["elevation"].dims == ("y", "x")
+    grid = ModelGrid.from_bounds(0.5, 3.5, 0.5, 2.5, 0.25)
+    g = io_netcdf.load_dem_and_roughness_to_grid(dem, grid, roughness_variable="n")
+    tx, ty = grid.x.numpy(), grid.y.numpy()
+    plane = 0.3 + 0.5 * tx[None, :] - 0.2 * ty[:, None]
+    assert np.allclose(g.bed.numpy(), plane, atol=2e-7, rtol=0)  # float32 source values
+    assert np.allclose(g.roughness_length.numpy(), 0.02, rtol=1e-6) and bool(g.valid_mask.all())
+    m = swe.SWE2D.from_netcdf(dem, model_grid=grid, roughness_variable="n", train_dem=False)
+    assert torch.equal(m.bed_reference[0, 0], g.bed) and torch.allclose(m.roughness_length, g.roughness_length)
+    hole = z.copy()
+    hole[3, 4] = -9999.0
+    bad = write_classic(tmp_path / "hole.nc", {"x": (("x",), x, {}), "y": (("y",), y, {}),
+                                               "z": (("y", "x"), hole, {"_FillValue": np.float32(-9999.0)})},
+                        {"x": 9, "y": 7})
+    with pytest.raises(ValueError, match="missing values"):
+        io_netcdf.load_dem_and_roughness_to_grid(bad, grid)
+    # gridded rainfall: 2 records, mm/h, times in hours since a date; conservative remap keeps the total
+    rx, ry = (np.arange(4) + 0.5) * 1.0, (np.arange(3) + 0.5) * 1.0
+    rain = np.stack([np.full((3, 4), 3.6), np.arange(12.0).reshape(3, 4)]).astype(np.float64)
+    rf = write_classic(tmp_path / "rain.nc", {"x": (("x",), rx, {}), "y": (("y",), ry, {}),
+                                              "time": (("time",), np.array([1.0, 1.5]),
+                                                       {"units": "hours since 2000-01-01"}),
+                                              "rainfall": (("time", "y", "x"), rain, {"units": "mm/h"})},
+                       {"x": 4, "y": 3, "time": 2})
+    target = ModelGrid.from_bounds(0.0, 4.0, 0.0, 3.0, 0.5)
+    forcing = io_netcdf.load_rainfall_to_grid(rf, target)
+    assert forcing.times.tolist() == [0.0, 1800.0] and tuple(forcing.rates.shape) == (2, 1, 6, 8)
+    assert torch.allclose(forcing.rates[0], torch.tensor(1e-6, dtype=torch.float64))
+    vol = float(forcing.rates[1].sum()) * 0.25
+    assert abs(vol - rain[1].sum() * 1e-3 / 3600) < 1e-15
+
+
+def test_hdf5_filter_pipeline_pieces():
+    import zlib
+
+    data = np.arange(40, dtype="<f8") * 0.37
+    raw = data.tobytes()
+    shuffled = np.frombuffer(raw, dtype=np.uint8).reshape(-1, 8).T.tobytes()
+    packed = zlib.compress(shuffled) + b"\x01\x02\x03\x04"
+    filters = [(2, [8]), (1, [6]), (3, [])]  # shuffle, deflate, Fletcher-32, applied in that order on write
+    out = hdf5.File._unfilter(packed, filters, 0, 8)
+    assert np.array_equal(np.frombuffer(out, dtype="<f8"), data)
+    # a set mask bit skips that filter (here Fletcher-32 was not applied)
+    assert hdf5.File._unfilter(zlib.compress(shuffled), filters, 0b100, 8) == raw
+    with pytest.raises(NotImplementedError, match="filter id 32001"):
+        hdf5.File._unfilter(raw, [(32001, [])], 0, 8)
+
+    class Stub:
+        buf = memoryview(bytes([1, 2, 0, 0, 0, 0, 0, 0,            # version 1, two filters, reserved
+                                2, 0, 0, 0, 1, 0, 1, 0, 8, 0, 0, 0, 0, 0, 0, 0,   # shuffle: 1 value, padded
+                                1, 0, 8, 0, 1, 0, 1, 0]) + b"deflate\0" + bytes([4, 0, 0, 0, 0, 0, 0, 0]))
+
+    assert hdf5.File._filter_pipeline(Stub(), 0) == [(2, [8]), (1, [4])]
