@@ -45,7 +45,50 @@ def parse():
                     help="steps of the adjoint leg (taped forward, reverse sweep, checkpointed backward); 0 to skip")
     ap.add_argument("--copy-probe", action="store_true",
                     help="also run a 1 GiB device copy (calibrates rocprof FETCH_SIZE / WRITE_SIZE)")
+    ap.add_argument("--small-grids", action="store_true",
+                    help="instead of the basin: time forward steps at the reference examples' grid sizes")
     return ap.parse_args()
+
+
+SMALL_GRIDS = ((42, 68, "f64", "Monai Valley 0.08 m size"), (254, 200, "f64", "CEA 2008 size"),
+               (344, 544, "f32", "coastal run-up"), (1024, 1024, "f32", "1024 x 1024"))
+
+
+def small_grids(dev):
+    """Forward steps of the run-up scenario at small grid sizes, device-driven loop, event-timed.
+
+    At these sizes a launch holds a few dozen to a few thousand wavefronts, so a
+    step costs about the latency of one wavefront's strip, not bandwidth."""
+    import torch
+
+    import diffswe2d_amd as swe
+    from diffswe2d_amd import scenarios
+
+    rows = []
+    for ny, nx, name, label in SMALL_GRIDS:
+        dtype = torch.float64 if name == "f64" else torch.float32
+        dx = 5.44 / nx
+        sc = scenarios.coastal_runup(nx=nx, ny=ny, dx=dx)
+        cfg = swe.SWEConfig(boundary_left="water_level", boundary_right="wall", boundary_top="wall",
+                            boundary_bottom="wall")
+        m = swe.SWE2D(torch.tensor(sc["bed"], dtype=dtype), torch.tensor(sc["rough"], dtype=dtype), dx, dx,
+                      config=cfg, train_dem=False).to(dev)
+        U = torch.tensor(sc["U"], dtype=dtype, device=dev)
+        with torch.no_grad():
+            m(U, t_end=3.0, boundary_level=0.01)  # same length: first-use kernel loads happen here
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            torch.cuda.synchronize()
+            e0.record()
+            m(U, t_end=3.0, boundary_level=0.01)
+            e1.record()
+            torch.cuda.synchronize()
+        steps = len(m.last_dt_series)
+        ms = e0.elapsed_time(e1)
+        rows.append({"grid": f"{nx}x{ny}", "dtype": name, "case": label, "steps": steps,
+                     "us_per_step": round(1e3 * ms / steps, 2),
+                     "gcell_updates_per_s": round(nx * ny * steps / (ms * 1e-3) / 1e9, 4)})
+        print(f"# {rows[-1]}", file=sys.stderr, flush=True)
+    return rows
 
 
 def basin_model(n, device):
@@ -246,6 +289,9 @@ def main():
         print(json.dumps({"error": "no GPU visible"}))
         sys.exit(1)
     dev = torch.device("cuda:0")
+    if a.small_grids:
+        print(json.dumps({"small_grids": small_grids(dev)}))
+        return
     model, U, sc, cfg = basin_model(a.n, dev)
     tuned = None
     if a.rows:
