@@ -152,6 +152,14 @@ class File:
         with open(self.path, "rb") as fh:
             self._map = mmap.mmap(fh.fileno(), 0, access=mmap.ACCESS_READ)
         self.buf = memoryview(self._map)
+        self._datasets = {}
+        try:
+            self._open()
+        except BaseException:
+            self.close()
+            raise
+
+    def _open(self):
         if bytes(self.buf[:8]) != _SIGNATURE:
             raise HDF5Error(f"{self.path} is not an HDF5 file")
         version = self.buf[8]
@@ -165,9 +173,10 @@ class File:
         self.root = _u(self.buf, 36, 8)
         self.links = self._group_links(self.root)
         self.root_attrs = dict(self._attribute(off) for t, off, _ in self._messages(self.root) if t == _ATTRIBUTE)
-        self._datasets = {}
 
     def close(self):
+        if self._map.closed:
+            return
         self.buf.release()
         self._map.close()
 

@@ -52,8 +52,9 @@ class GridData:
 class Variable:
     """A decoded NetCDF variable: ``values`` (numpy), ``dims`` (names), ``attrs``."""
 
-    def __init__(self, name, values, dims, attrs):
+    def __init__(self, name, values, dims, attrs, coords=None):
         self.name, self.values, self.dims, self.attrs = name, values, tuple(dims), dict(attrs)
+        self.coords = {} if coords is None else coords
         if self.values.ndim != len(self.dims):
             raise ValueError(f"variable {name!r}: {self.values.ndim} axes but dimensions {self.dims}")
 
