@@ -21,7 +21,6 @@ the feature, rather than returning wrong data.
 from __future__ import annotations
 
 import mmap
-import struct
 import zlib
 from pathlib import Path
 from typing import Dict, Optional
@@ -32,7 +31,7 @@ _SIGNATURE = b"\x89HDF\r\n\x1a\n"
 _UNDEF = 0xFFFFFFFFFFFFFFFF
 
 # message types (format spec IV.A.2)
-_DATASPACE, _LINK_INFO, _DATATYPE, _FILL_OLD, _FILL = 0x01, 0x02, 0x03, 0x04, 0x05
+_DATASPACE, _LINK_INFO, _DATATYPE = 0x01, 0x02, 0x03
 _LINK, _LAYOUT, _FILTERS, _ATTRIBUTE, _CONTINUATION = 0x06, 0x08, 0x0B, 0x0C, 0x10
 _SYMBOL_TABLE, _ATTRIBUTE_INFO = 0x11, 0x15
 
@@ -145,7 +144,11 @@ class Dataset:
 
 
 class File:
-    """An HDF5 file opened read-only; ``datasets`` maps root-group link names to Dataset objects."""
+    """An HDF5 file opened read-only (memory-mapped).
+
+    ``links`` maps the root group's link names to object-header addresses,
+    ``dataset(name)`` returns that object as a Dataset, ``root_attrs`` holds the
+    root group's attributes."""
 
     def __init__(self, path):
         self.path = Path(path)
