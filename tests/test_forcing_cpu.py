@@ -45,3 +45,15 @@ def test_dem_regularization_closed_form():
     assert abs(float(m.dem_regularization().detach()) - prior) < 1e-4
     assert abs(float(m.dem_regularization(0.0, 1.0, 0.0).detach()) - (1.0 + 16.0)) < 1e-5
     assert float(m.dem_regularization(0.0, 0.0, 1.0).detach()) == 0.0
+
+
+def test_setup_logger_writes_terminal_and_file(tmp_path, capsys):
+    log = swe.setup_logger(str(tmp_path / "run.log"))
+    log.info("Step: 1")
+    log = swe.setup_logger(str(tmp_path / "run.log"))  # a second call replaces the handlers
+    log.info("Step: 2")
+    for h in log.handlers:
+        h.flush()
+    text = (tmp_path / "run.log").read_text().splitlines()
+    assert len(text) == 1 and text[0].endswith(" - Step: 2") and text[0][4] == "-"
+    assert len(log.handlers) == 2
