@@ -242,3 +242,47 @@ def spline_inversion(problem: GaugeProblem, basis: torch.Tensor, target: torch.T
     if best_raw is None:
         raise RuntimeError("Training did not produce a valid best state.")
     return best_raw, best_epoch, history
+
+
+MONAI_GAUGES = ((4.521, 1.196), (4.521, 1.696), (4.521, 2.196), (1.000, 1.696), (2.000, 1.696), (3.000, 1.696))
+
+
+def monai_valley_setup(bathymetry, observations, device, n_gauges=3, dtype=torch.float64):
+    """The set-up of the reference's Monai Valley boundary inversion (bc_inversion_train.py:84-131,
+    253-321, 576-631): the NetCDF bathymetry on 68 x 42 cells of 0.08 m, a water-level left
+    boundary and walls elsewhere (Manning n = 0.01), still water at 0 m, the first ``n_gauges``
+    gauge cells, the observation columns interpolated to the 149 chunk ends of 0.2 s, and a
+    clamped cubic B-spline over 20 controls started at a level of 0 within (-0.02, 0.05).
+
+    Returns a dict with model, problem (GaugeProblem), target [149, n_gauges], basis, raw0, bounds."""
+    from . import load_dynamic_model
+    from .grid import ModelGrid
+
+    chunk, n_chunks, n_controls, bounds = 0.2, 149, 20, (-0.02, 0.05)
+    grid = ModelGrid.from_bounds(xmin=0.0, xmax=5.44, ymin=0.0, ymax=3.36, resolution=0.08, dtype=dtype,
+                                 device=device)
+    model, _, _, _ = load_dynamic_model(dem_filepath=str(bathymetry), model_grid=grid, device=device,
+                                        boundary_left="water_level", boundary_right="wall", boundary_top="wall",
+                                        boundary_bottom="wall", frictionmodel="manning", roughness_var_name=None,
+                                        default_roughness_value=0.01)
+    model = model.to(device=device, dtype=dtype)
+    model.dem_correction_parameter.requires_grad_(False)
+    model.clear_dynamic_boundary()
+    model.train()
+    cells = []
+    for g, (gx, gy) in enumerate(MONAI_GAUGES[:n_gauges]):
+        ix = int(torch.argmin(torch.abs(grid.x - gx)))
+        iy = int(torch.argmin(torch.abs(grid.y - gy)))
+        model.gauge_indices[f"gauge_{g}"] = (iy, ix)
+        cells.append((iy, ix))
+    data = np.loadtxt(observations, skiprows=1)
+    ends = np.arange(1, n_chunks + 1, dtype=np.float64) * chunk
+    target = torch.tensor(np.stack([np.interp(ends, data[:, 0], data[:, c]) for c in range(1, n_gauges + 1)],
+                                   axis=1), dtype=dtype, device=device)
+    depth = torch.clamp(0.0 - model.bed.detach(), min=0.0)
+    U0 = torch.cat((depth, torch.zeros_like(depth), torch.zeros_like(depth)), dim=1)
+    times = torch.arange(n_chunks, dtype=dtype, device=device) * chunk
+    basis = clamped_bspline_basis(times, n_controls, 3)
+    raw0 = inverse_bounded(0.0, *bounds, torch.zeros(n_controls, dtype=dtype, device=device))
+    return dict(model=model, problem=GaugeProblem(model, U0, cells, chunk, n_chunks), cells=cells, U0=U0,
+                target=target, basis=basis, raw0=raw0, bounds=bounds)

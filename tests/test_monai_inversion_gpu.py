@@ -26,15 +26,12 @@ import numpy as np
 import pytest
 import torch
 
-import diffswe2d_amd as swe
 from diffswe2d_amd import inversion
 
 pytestmark = pytest.mark.gpu
 
 GOLDEN = Path(__file__).resolve().parent / "golden"
 DT_OUT, N_CHUNKS, N_CONTROLS = 0.2, 149, 20
-GAUGES = {"gauge_0": (4.521, 1.196), "gauge_1": (4.521, 1.696), "gauge_2": (4.521, 2.196),
-          "gauge_3": (1.000, 1.696), "gauge_4": (2.000, 1.696), "gauge_5": (3.000, 1.696)}
 # scenario a: laboratory gauges 0-2, train on 0 and 1; scenario b (the _6simg variant): simulated gauges
 # 0-5, train on all but 2; gauge 2 is held out in both
 SCENARIOS = {
@@ -49,34 +46,8 @@ MAX_EPOCHS = 500  # the script's limit; its run stopped early, at epoch 219
 
 def setup(device, scenario="lab"):
     sc = SCENARIOS[scenario]
-    grid = swe.ModelGrid.from_bounds(xmin=0.0, xmax=5.44, ymin=0.0, ymax=3.36, resolution=0.08,
-                                     dtype=torch.float64, device=device)
-    model, _, _, _ = swe.load_dynamic_model(
-        dem_filepath=str(GOLDEN / "monai_bathy.nc"), model_grid=grid, device=device, boundary_left="water_level",
-        boundary_right="wall", boundary_top="wall", boundary_bottom="wall", frictionmodel="manning",
-        roughness_var_name=None, default_roughness_value=0.01)
-    model = model.to(device=device, dtype=torch.float64)
-    model.dem_correction_parameter.requires_grad_(False)
-    model.clear_dynamic_boundary()
-    model.train()
-    cells = []
-    for name, (gx, gy) in list(GAUGES.items())[:sc["n_gauges"]]:
-        ix = int(torch.argmin(torch.abs(grid.x - gx)))
-        iy = int(torch.argmin(torch.abs(grid.y - gy)))
-        model.gauge_indices[name] = (iy, ix)
-        cells.append((iy, ix))
-    data = np.loadtxt(GOLDEN / sc["obs"], skiprows=1)
-    model_times = np.arange(1, N_CHUNKS + 1, dtype=np.float64) * DT_OUT
-    cols = range(1, sc["n_gauges"] + 1)
-    target = torch.tensor(np.stack([np.interp(model_times, data[:, 0], data[:, c]) for c in cols], axis=1),
-                          dtype=torch.float64, device=device)
-    bed = model.bed.detach()
-    depth = torch.clamp(0.0 - bed, min=0.0)
-    U0 = torch.cat((depth, torch.zeros_like(depth), torch.zeros_like(depth)), dim=1)
-    times = torch.arange(N_CHUNKS, dtype=torch.float64, device=device) * DT_OUT
-    basis = inversion.clamped_bspline_basis(times, N_CONTROLS, 3)
-    raw0 = inversion.inverse_bounded(0.0, *BOUNDS, torch.zeros(N_CONTROLS, dtype=torch.float64, device=device))
-    return model, cells, target, U0, basis, raw0
+    d = inversion.monai_valley_setup(GOLDEN / "monai_bathy.nc", GOLDEN / sc["obs"], device, n_gauges=sc["n_gauges"])
+    return d["model"], d["cells"], d["target"], d["U0"], d["basis"], d["raw0"]
 
 
 def reference_history(scenario="lab"):
