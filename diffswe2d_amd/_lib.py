@@ -114,6 +114,7 @@ _SIGNATURES = {
     "swe_segment_adjoint_chunked_f64": ([_G, _C, _P, _P, _P, _P, _P, _P, _P, _I, _I, _P, _I, _P, _P, _P, _P, _P, _P,
                                          _P, _I, _P], _I),
     "swe_valu_probe": ([_P, _I, _I, _P], _I),
+    "swe_division_probe": ([_P, _P, ctypes.c_longlong, _P, _P, _P, _P], _I),
     "swe_water_volume_f32": ([_G, _P, _P, _P, _P], _I),
     "swe_water_volume_f64": ([_G, _P, _P, _P, _P], _I),
     "swe_ensemble_workspace_bytes": ([_I], ctypes.c_size_t),

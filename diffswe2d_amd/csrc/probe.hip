@@ -1,5 +1,7 @@
-// VALU issue-rate probe: the ceiling the stage kernels are measured against.
+// Probes: the VALU issue rate the stage kernels are measured against, and the
+// kernels' FP32 division on the device next to the hardware IEEE division.
 //
+// VALU issue-rate probe:
 // Each lane runs 8 independent FP32 multiply/add chains (no memory traffic in
 // the loop; 16 non-packed VALU instructions per iteration in the gfx950 ISA),
 // so with every SIMD holding several wavefronts the launch time is set by how
@@ -7,6 +9,7 @@
 #include <hip/hip_runtime.h>
 
 #include "../../include/swe2d_hip.h"
+#include "swe_math.h"
 
 namespace {
 
@@ -25,7 +28,25 @@ __global__ __launch_bounds__(256) void valu_probe_kernel(float* out, int iters, 
   out[(int64_t)blockIdx.x * blockDim.x + threadIdx.x] = a0 + a1 + a2 + a3 + a4 + a5 + a6 + a7;
 }
 
+__global__ __launch_bounds__(256) void division_probe_kernel(const float* a, const float* b, int64_t n, float* q_rn,
+                                                             float* q_by, float* q_ieee) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float x = a[i], y = b[i];
+  q_rn[i] = swe::div_rn(x, y);
+  q_by[i] = swe::div_by(x, swe::Divisor<float>{y, (float)(1.0 / (double)y)});
+  q_ieee[i] = x / y;
+}
+
 }  // namespace
+
+extern "C" int swe_division_probe(const float* a, const float* b, long long n, float* q_rn, float* q_by,
+                                  float* q_ieee, void* stream) {
+  if (!a || !b || !q_rn || !q_by || !q_ieee || n <= 0) return SWE_ERR_ARG;
+  hipLaunchKernelGGL(division_probe_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, (hipStream_t)stream, a,
+                     b, (int64_t)n, q_rn, q_by, q_ieee);
+  return hipGetLastError() == hipSuccess ? 0 : SWE_ERR_HIP;
+}
 
 extern "C" int swe_valu_probe(float* out, int blocks, int iters, void* stream) {
   if (!out || blocks <= 0 || iters <= 0) return SWE_ERR_ARG;

@@ -17,8 +17,8 @@ namespace swe {
 
 // a / b inside a gradient expression.  FP32 device code multiplies by the
 // hardware reciprocal (v_rcp_f32, about 1 ulp) instead of the correctly rounded
-// division sequence; divisions that recompute forward values keep IEEE
-// division, so every branch of the reverse is taken exactly as in the forward.
+// division sequence; divisions that recompute forward values use the forward's
+// div_rn, so every branch of the reverse is taken exactly as in the forward.
 // FP64 and host code divide.
 template <class T> SWE_HD inline T gdiv(T a, T b) {
 #if defined(__HIP_DEVICE_COMPILE__)
@@ -125,7 +125,7 @@ SWE_HD Incr<T> increments_r(const Cell<T>& m, const Cell<T>& c, const Cell<T>& p
     reset = (r.deta * r.deta > r.dz * r.dz) && ((bed_minus > m.eta) || (bed_plus > p.eta));
     if (reset) r.deta = r.dh + r.dz;
   }
-  r.q = (T(0.5) * r.dh) / clamp_lo(c.h, k.floor);
+  r.q = div_rn(T(0.5) * r.dh, clamp_lo(c.h, k.floor));
   return r;
 }
 
@@ -182,8 +182,8 @@ SWE_HD void face_flux_bwd(const Side<T>& L, T hL_cell, T zL_cell, const Side<T>&
   T hL = clamp_lo(hcL, T(0)), hR = clamp_lo(hcR, T(0));
   bool wetL = hL > k.dry, wetR = hR > k.dry;
   T sfL = clamp_lo(hL, k.floor), sfR = clamp_lo(hR, k.floor);
-  T nL = wetL ? qnL / sfL : T(0), tL = wetL ? qtL / sfL : T(0), cL = wetL ? sqrt(k.g * sfL) : T(0);
-  T nR = wetR ? qnR / sfR : T(0), tR = wetR ? qtR / sfR : T(0), cR = wetR ? sqrt(k.g * sfR) : T(0);
+  T nL = wetL ? div_rn(qnL, sfL) : T(0), tL = wetL ? div_rn(qtL, sfL) : T(0), cL = wetL ? sqrt(k.g * sfL) : T(0);
+  T nR = wetR ? div_rn(qnR, sfR) : T(0), tR = wetR ? div_rn(qtR, sfR) : T(0), cR = wetR ? sqrt(k.g * sfR) : T(0);
   T um = (T(0.5) * (nL + nR) + cL) - cR;
   T cm = T(0.5) * (cL + cR) + T(0.25) * (nL - nR);
   bool dryL = hL <= k.dry, dryR = hR <= k.dry;
@@ -201,7 +201,7 @@ SWE_HD void face_flux_bwd(const Side<T>& L, T hL_cell, T zL_cell, const Side<T>&
   T dmn = hR * nR - hL * nL;
   T num_m = (SR * mL - SL * mR) + slsr * dh_;
   T num_p = (SR * pL - SL * pR) + slsr * dmn;
-  T fh = SL >= T(0) ? mL : (SR <= T(0) ? mR : num_m / den);
+  T fh = SL >= T(0) ? mL : (SR <= T(0) ? mR : div_rn(num_m, den));
   bool both = dryL && dryR;
 
   // ---- reverse
@@ -352,7 +352,7 @@ SWE_HD void clean_and_limit_bwd(T oh, T ou, T ov, T ch, T cu, T cv, const SweCon
   T s = T(0), q = T(0);
   if (positivity && dh < T(0)) {
     s = clamp_lo(-dh, k.eps);
-    q = oh / s;
+    q = div_rn(oh, s);
     theta = clamp_hi(q, T(1));
   }
   T c2h = positivity ? oh + theta * dh : ch;
@@ -400,7 +400,7 @@ SWE_HD void friction_bwd(T h_in, T hu, T hv, T coef, T dt, const SweConsts<T>& k
   bool wet = h > k.dry;
   T safe = clamp_lo(h, k.floor);
   T um = wet ? hu : T(0), vm = wet ? hv : T(0);
-  T u = um / safe, v = vm / safe;
+  T u = div_rn(um, safe), v = div_rn(vm, safe);
   T s2 = u * u + v * v;
   bool moving = wet && (s2 > T(0));
   T ss2 = clamp_lo(s2, k.fric_floor2);
@@ -411,21 +411,21 @@ SWE_HD void friction_bwd(T h_in, T hu, T hv, T coef, T dt, const SweConsts<T>& k
     T n = clamp_lo(coef, T(0));
     T num = k.manning_g * (n * n);
     T p = pow_third(sh, k.third);
-    cf = num / p;
+    cf = div_rn(num, p);
     // d/dsh [num / sh^(1/3)] = -num / p^2 * (1/3) sh^(1/3 - 1), with sh^(1/3 - 1) = p / sh
     T dp = gdiv(k.third * p, sh);
     if (h >= k.fric_floor) gcf_dh = -(num / (p * p)) * dp;
   } else {
     T fl = k.fric_floor;
     T z0 = clamp_lo(coef, fl);
-    T ratio = clamp_lo(h, fl) / z0;
+    T ratio = div_rn(clamp_lo(h, fl), z0);
     T dratio_dh = h >= fl ? T(1) / z0 : T(0);
     T x = T(0.46) * ratio;
-    T shallow = T(1) / clamp_lo(x, fl);
+    T shallow = div_rn(T(1), clamp_lo(x, fl));
     T r = clamp_lo(ratio, fl);
-    T lt = T(2.5) * ((log_t(r) - T(1)) + (T(1) / r) * T(1.359));
+    T lt = T(2.5) * ((log_t(r) - T(1)) + div_rn(T(1), r) * T(1.359));
     T lt2 = lt * lt;
-    T deep = T(1) / clamp_lo(lt2, fl);
+    T deep = div_rn(T(1), clamp_lo(lt2, fl));
     if (ratio < T(2.718281828459045)) {
       cf = shallow;
       T ds = x >= fl ? -(shallow * shallow) * T(0.46) : T(0);
@@ -440,9 +440,9 @@ SWE_HD void friction_bwd(T h_in, T hu, T hv, T coef, T dt, const SweConsts<T>& k
   }
   T a = cf * speed;
   T b = a * dt;
-  T c = b / safe;
+  T c = div_rn(b, safe);
   T den = T(1) + c;
-  T f = T(1) / den;
+  T f = div_rn(T(1), den);
   T gf = wet ? (gu_out * hu + gv_out * hv) : T(0);
   T ghu = wet ? gu_out * f : T(0);
   T ghv = wet ? gv_out * f : T(0);

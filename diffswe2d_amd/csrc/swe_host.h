@@ -27,6 +27,8 @@ template <class T> inline SweConsts<T> host_consts(const swe_config* c, double d
   k.cfl = (T)c->cfl;
   k.dx = (T)dx;
   k.dy = (T)dy;
+  k.ddx = Divisor<T>{k.dx, (T)(1.0 / (double)k.dx)};
+  k.ddy = Divisor<T>{k.dy, (T)(1.0 / (double)k.dy)};
   k.cval = (T)c->constant_value;
   k.manning_g = (T)9.81;
   k.third = (T)(1.0 / 3.0);

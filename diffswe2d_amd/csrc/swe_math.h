@@ -30,6 +30,51 @@ template <class T> SWE_HD T clamp_hi(T x, T hi) { return __builtin_elementwise_m
 template <class T> SWE_HD T clamp2(T x, T lo, T hi) { return clamp_hi(clamp_lo(x, lo), hi); }
 template <class T> SWE_HD T tabs(T x) { return __builtin_fabs(x); }
 
+// ---------------------------------------------------------------- division
+// FP32 a / b rounded like IEEE division.  The sequence is the one the compiler
+// emits for '/' on gfx950 -- a reciprocal estimate, one Newton step on it, then
+// two remainder corrections of the quotient, all fused multiply-adds -- without
+// the operand range scaling (v_div_scale / v_div_fmas) and the special-value
+// fix-up (v_div_fixup) around it.  Those act only when b is denormal or beyond
+// 2^126, when the quotient leaves the normal range, or when |a| < 2^-103; the
+// step's depths, momenta and fluxes reach none of these short of vanishing
+// values, where the result is still within an ulp (and a -0 numerator gives
+// +0).  8 instructions instead of 11.  FP64 keeps the plain division.
+template <class T> SWE_HD T div_rn(T a, T b) { return a / b; }
+
+SWE_HD float rcp_estimate(float b) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return __builtin_amdgcn_rcpf(b);
+#else
+  return 1.0f / b;
+#endif
+}
+
+// the sequence from a reciprocal estimate r within an ulp of 1 / b
+SWE_HD float div_rn_from(float a, float b, float r) {
+  r = __builtin_fmaf(__builtin_fmaf(-b, r, 1.0f), r, r);
+  float q = a * r;
+  q = __builtin_fmaf(__builtin_fmaf(-b, q, a), r, q);
+  return __builtin_fmaf(__builtin_fmaf(-b, q, a), r, q);
+}
+
+template <> SWE_HD float div_rn<float>(float a, float b) { return div_rn_from(a, b, rcp_estimate(b)); }
+
+// a / b for a divisor fixed for the whole run (dx, dy): its reciprocal comes
+// rounded from the host, so only the two remainder corrections remain
+// (5 instructions), with the same rounding as div_rn.
+template <class T> struct Divisor {
+  T b, r;
+};
+
+template <class T> SWE_HD T div_by(T a, const Divisor<T>& d) { return a / d.b; }
+
+template <> SWE_HD float div_by<float>(float a, const Divisor<float>& d) {
+  float q = a * d.r;
+  q = __builtin_fmaf(__builtin_fmaf(-d.b, q, a), d.r, q);
+  return __builtin_fmaf(__builtin_fmaf(-d.b, q, a), d.r, q);
+}
+
 // Constants of one run, folded on the host in double then cast to T once.
 template <class T> struct SweConsts {
   T g;            // float(gravity)                      riemann.py:36
@@ -43,6 +88,7 @@ template <class T> struct SweConsts {
   T theta;        // limiter theta                       numerics.py:16
   T cfl;
   T dx, dy;       // float(dx), float(dy)
+  Divisor<T> ddx, ddy;  // the same with their rounded reciprocals
   T cval;         // constant-boundary value
   T manning_g;    // 9.81 (hard-coded in friction.py:66)
   T third;        // 1.0 / 3.0                           friction.py:66
@@ -79,8 +125,8 @@ SWE_HD void primitive_cell(T h_raw, T hu, T hv, T z, const SweConsts<T>& k, T& h
   h = clamp_lo(h_raw, T(0));
   bool wet = h > k.dry;
   T safe = clamp_lo(h, k.floor);
-  u = wet ? hu / safe : T(0);
-  v = wet ? hv / safe : T(0);
+  u = wet ? div_rn(hu, safe) : T(0);
+  v = wet ? div_rn(hv, safe) : T(0);
   eta = h + z;
 }
 
@@ -108,7 +154,7 @@ SWE_HD Incr<T> increments(const Cell<T>& m, const Cell<T>& c, const Cell<T>& p, 
     bool reset = (r.deta * r.deta > r.dz * r.dz) && ((bed_minus > m.eta) || (bed_plus > p.eta));
     if (reset) r.deta = r.dh + r.dz;
   }
-  r.q = (T(0.5) * r.dh) / clamp_lo(c.h, k.floor);
+  r.q = div_rn(T(0.5) * r.dh, clamp_lo(c.h, k.floor));
   return r;
 }
 
@@ -170,11 +216,11 @@ SWE_HD FaceFlux<T> face_flux(const Side<T>& L, T hL_cell, T zL_cell, const Side<
   T hL = clamp_lo(hcL, T(0)), hR = clamp_lo(hcR, T(0));
   bool wetL = hL > k.dry, wetR = hR > k.dry;
   T sfL = clamp_lo(hL, k.floor), sfR = clamp_lo(hR, k.floor);
-  T nL = wetL ? qnL / sfL : T(0);
-  T tL = wetL ? qtL / sfL : T(0);
+  T nL = wetL ? div_rn(qnL, sfL) : T(0);
+  T tL = wetL ? div_rn(qtL, sfL) : T(0);
   T cL = wetL ? sqrt(k.g * sfL) : T(0);
-  T nR = wetR ? qnR / sfR : T(0);
-  T tR = wetR ? qtR / sfR : T(0);
+  T nR = wetR ? div_rn(qnR, sfR) : T(0);
+  T tR = wetR ? div_rn(qtR, sfR) : T(0);
   T cR = wetR ? sqrt(k.g * sfR) : T(0);
 
   T um = (T(0.5) * (nL + nR) + cL) - cR;
@@ -189,8 +235,8 @@ SWE_HD FaceFlux<T> face_flux(const Side<T>& L, T hL_cell, T zL_cell, const Side<
   T pR = hR * (nR * nR + k.half_g * hR);
   T den = clamp_lo(SR - SL, k.hll_floor);
   T slsr = SL * SR;
-  T m_star = ((SR * mL - SL * mR) + slsr * (hR - hL)) / den;
-  T p_star = ((SR * pL - SL * pR) + slsr * (hR * nR - hL * nL)) / den;
+  T m_star = div_rn((SR * mL - SL * mR) + slsr * (hR - hL), den);
+  T p_star = div_rn((SR * pL - SL * pR) + slsr * (hR * nR - hL * nL), den);
 
   T fh, fn;
   if (SL >= T(0)) {
@@ -225,7 +271,7 @@ SWE_HD void clean_and_limit(T oh, T ou, T ov, T& ch, T& cu, T& cv, const SweCons
   if (positivity) {
     T dh = ch - oh, du = cu - ou, dv = cv - ov;
     T theta = T(1);
-    if (dh < T(0)) theta = clamp_hi(oh / clamp_lo(-dh, k.eps), T(1));
+    if (dh < T(0)) theta = clamp_hi(div_rn(oh, clamp_lo(-dh, k.eps)), T(1));
     ch = oh + theta * dh;
     cu = ou + theta * du;
     cv = ov + theta * dv;
@@ -252,8 +298,8 @@ SWE_HD void friction(T& h_io, T& hu, T& hv, T coef, T dt, const SweConsts<T>& k,
   T h = clamp_lo(h_io, T(0));
   bool wet = h > k.dry;
   T safe = clamp_lo(h, k.floor);
-  T u = (wet ? hu : T(0)) / safe;
-  T v = (wet ? hv : T(0)) / safe;
+  T u = div_rn(wet ? hu : T(0), safe);
+  T v = div_rn(wet ? hv : T(0), safe);
   T s2 = u * u + v * v;
   bool moving = wet && (s2 > T(0));
   T speed = moving ? sqrt(clamp_lo(s2, k.fric_floor2)) : T(0);
@@ -261,17 +307,17 @@ SWE_HD void friction(T& h_io, T& hu, T& hv, T coef, T dt, const SweConsts<T>& k,
   if (model == 0) {
     T sh = clamp_lo(h, k.fric_floor);
     T n = clamp_lo(coef, T(0));
-    cf = (k.manning_g * (n * n)) / pow_third(sh, k.third);
+    cf = div_rn(k.manning_g * (n * n), pow_third(sh, k.third));
   } else {
     T fl = k.fric_floor;
-    T ratio = clamp_lo(h, fl) / clamp_lo(coef, fl);
-    T shallow = T(1) / clamp_lo(T(0.46) * ratio, fl);
+    T ratio = div_rn(clamp_lo(h, fl), clamp_lo(coef, fl));
+    T shallow = div_rn(T(1), clamp_lo(T(0.46) * ratio, fl));
     T r = clamp_lo(ratio, fl);
-    T lt = T(2.5) * ((log_t(r) - T(1)) + (T(1) / r) * T(1.359));  // scalar / tensor = reciprocal * scalar
-    T deep = T(1) / clamp_lo(lt * lt, fl);
+    T lt = T(2.5) * ((log_t(r) - T(1)) + div_rn(T(1), r) * T(1.359));  // scalar / tensor = reciprocal * scalar
+    T deep = div_rn(T(1), clamp_lo(lt * lt, fl));
     cf = ratio < T(2.718281828459045) ? shallow : deep;
   }
-  T factor = T(1) / (T(1) + ((cf * speed) * dt) / safe);
+  T factor = div_rn(T(1), T(1) + div_rn((cf * speed) * dt, safe));
   h_io = h;
   hu = wet ? hu * factor : T(0);
   hv = wet ? hv * factor : T(0);

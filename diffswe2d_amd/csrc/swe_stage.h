@@ -155,9 +155,9 @@ __device__ inline void stage_strip(const StageArgs<T>& a, int strip, int seg, in
         if (it >= 4) {
           // output row o = r - 2 (its state is P2)
           const int o = r - 2;
-          T ry_h = (gfh - G.fh) / k.dy;
-          T ry_u = (gft - G.ft) / k.dy;
-          T ry_v = (gfn - (G.fn - G.sL)) / k.dy;
+          T ry_h = div_by(gfh - G.fh, k.ddy);
+          T ry_u = div_by(gft - G.ft, k.ddy);
+          T ry_v = div_by(gfn - (G.fn - G.sL), k.ddy);
           // x sweep of row o
           Cell<T> cx = P2.x();
           Cell<T> cm = shup(cx), cp = shdn(cx);
@@ -168,9 +168,9 @@ __device__ inline void stage_strip(const StageArgs<T>& a, int strip, int seg, in
           FaceFlux<T> F = face_flux(left_of_face, cx.h, cx.z, nxt, cp.h, cp.z, k);
           if (xface_ok) ax = tmax(ax, F.speed);
           T pfh = shup(F.fh), pft = shup(F.ft), pfn = shup(F.fn - F.sR);
-          T rx_h = (pfh - F.fh) / k.dx;
-          T rx_u = (pfn - (F.fn - F.sL)) / k.dx;
-          T rx_v = (pft - F.ft) / k.dx;
+          T rx_h = div_by(pfh - F.fh, k.ddx);
+          T rx_u = div_by(pfn - (F.fn - F.sL), k.ddx);
+          T rx_v = div_by(pft - F.ft, k.ddx);
           T th = rx_h + ry_h, tu = rx_u + ry_u, tv = rx_v + ry_v;
           if (lane_out) {
             int64_t i = mbase + (int64_t)o * f.nx + col;

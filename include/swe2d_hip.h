@@ -326,6 +326,15 @@ int swe_segment_adjoint_chunked_f64(const swe_grid* grid, const swe_config* cfg,
  * the stage kernels' instruction counts are priced. */
 int swe_valu_probe(float* out, int blocks, int iters, void* stream);
 
+/* Division probe: for n operand pairs (device arrays), q_rn = the kernels'
+ * FP32 division (reciprocal estimate, one Newton step, two remainder
+ * corrections), q_by = the same from a host-rounded reciprocal (the form used
+ * for dx and dy), q_ieee = the compiler's IEEE division.  For operands in the
+ * normal range the three agree bit for bit.  No counterpart in the reference;
+ * a test hook. */
+int swe_division_probe(const float* a, const float* b, long long n, float* q_rn, float* q_by, float* q_ieee,
+                       void* stream);
+
 /* ---- water volume (FP64) -------------------------------------------------
  * volume[b] = sum over the interior cells of member b of h * dx * dy, summed
  * in FP64 in a fixed order (the same bits on every call).  volume is a device

@@ -4,9 +4,11 @@
 // (swe_math.h, swe_adjoint_math.h, swe_common.h) in plain loops over a padded
 // grid, so the CPU test suite can compare them with the oracle and with the
 // oracle's autograd without a GPU.  Nothing in the package loads this file.
+#include <math.h>
 #include <stdint.h>
 #include <string.h>
 
+#include <random>
 #include <vector>
 
 #include "swe_adjoint_math.h"
@@ -398,4 +400,32 @@ extern "C" void hc_faces_f32(const swe_config* cfg, int ny, int nx, double dx, d
       float* o = out + ((int64_t)line * (len + 1) + j + 1) * 6;
       o[0] = F.fh; o[1] = F.fn; o[2] = F.ft; o[3] = F.sL; o[4] = F.sR; o[5] = F.speed;
     }
+}
+
+// div_rn_from / div_by (swe_math.h) against IEEE division on n random operand
+// pairs: numerators log-uniform over [2^-100, 2^40], divisors over [2^-30, 2^30],
+// both signs, one pair in 64 with a zero numerator.  The reciprocal estimate is
+// taken exact and one ulp either side (the hardware estimate is within one ulp).
+// mismatch[0..2]: div_rn_from with the three estimates; mismatch[3..5]: div_by.
+extern "C" void hc_div_check(long long n, unsigned seed, long long* mismatch) {
+  std::mt19937_64 rng(seed);
+  std::uniform_real_distribution<double> unit(1.0, 2.0), ea(-100.0, 40.0), eb(-30.0, 30.0);
+  for (int i = 0; i < 6; ++i) mismatch[i] = 0;
+  for (long long i = 0; i < n; ++i) {
+    float a = (float)ldexp(unit(rng), (int)ea(rng));
+    float b = (float)ldexp(unit(rng), (int)eb(rng));
+    if (rng() & 1) a = -a;
+    if (rng() & 1) b = -b;
+    if ((rng() & 63) == 0) a = 0.0f;
+    const float want = a / b;
+    const float r = 1.0f / b;
+    const float est[3] = {r, nextafterf(r, 0.0f), nextafterf(r, 2.0f * r)};
+    for (int j = 0; j < 3; ++j) {
+      float got = div_rn_from(a, b, est[j]);
+      if (memcmp(&got, &want, 4) != 0 && !(got == 0.0f && want == 0.0f)) ++mismatch[j];
+      Divisor<float> d{b, est[j]};
+      float got2 = div_by(a, d);
+      if (memcmp(&got2, &want, 4) != 0 && !(got2 == 0.0f && want == 0.0f)) ++mismatch[3 + j];
+    }
+  }
 }

@@ -387,3 +387,34 @@ def test_volume_is_conserved_on_a_periodic_basin(cuda_device, name, bar):
     print(f"{name}: {steps} steps, volume {v0:.15e} -> {v1:.15e}, rel change {abs(v1 - v0) / v0:.2e}")
     assert steps > 50 and float(U[:, 0].min()) > 0.1
     assert abs(v1 - v0) <= bar * v0
+
+
+def test_device_division_equals_ieee_division(cuda_device):
+    """The stage kernels' FP32 division (v_rcp_f32 estimate, one Newton step, two remainder
+    corrections; and its host-reciprocal form for dx, dy) against the compiler's IEEE division
+    on the device: bit for bit over 16 M random normal-range operand pairs (numerators
+    log-uniform over [2^-100, 2^40], divisors over [2^-30, 2^30], both signs, some zeros)."""
+    from diffswe2d_amd import _lib
+
+    n = 1 << 24
+    g = torch.Generator().manual_seed(3)
+    ea = torch.randint(-100, 40, (n,), generator=g)
+    eb = torch.randint(-30, 30, (n,), generator=g)
+    a = (1 + torch.rand(n, generator=g, dtype=torch.float64)) * torch.pow(2.0, ea.double())
+    b = (1 + torch.rand(n, generator=g, dtype=torch.float64)) * torch.pow(2.0, eb.double())
+    a = torch.where(torch.rand(n, generator=g) < 0.5, -a, a)
+    b = torch.where(torch.rand(n, generator=g) < 0.5, -b, b)
+    a[torch.rand(n, generator=g) < 1 / 64] = 0.0
+    a = a.float().to(cuda_device)
+    b = b.float().to(cuda_device)
+    q_rn, q_by, q_ieee = (torch.empty_like(a) for _ in range(3))
+    _lib.check(_lib.load().swe_division_probe(_lib.ptr(a), _lib.ptr(b), n, _lib.ptr(q_rn), _lib.ptr(q_by),
+                                              _lib.ptr(q_ieee), _lib.stream_ptr(cuda_device)), "division probe")
+    torch.cuda.synchronize()
+    bits = lambda t: t.view(torch.int32)  # noqa: E731
+    zero = q_ieee == 0
+    bad_rn = int(((bits(q_rn) != bits(q_ieee)) & ~zero).sum())
+    bad_by = int(((bits(q_by) != bits(q_ieee)) & ~zero).sum())
+    print(f"division: {n} pairs, div_rn mismatches {bad_rn}, div_by mismatches {bad_by}")
+    assert bad_rn == 0 and bad_by == 0
+    assert bool((q_rn[zero] == 0).all()) and bool((q_by[zero] == 0).all())

@@ -131,3 +131,16 @@ def test_per_member_levels(golden):
     gU, gl = hc.step_bwd(cfg, U, bed, rough, 0.1, 0.1, lvl, dt, lam)
     assert relerr(gU, gU_ref) <= 1e-12
     assert np.allclose(gl, gl_ref.numpy(), rtol=1e-10, atol=1e-14)
+
+
+def test_division_sequence_rounds_like_ieee_division():
+    """div_rn / div_by (the FP32 division of the kernels) equal IEEE division bit for bit over
+    random normal-range operands, with the reciprocal estimate exact or one ulp off either way
+    (the hardware estimate's bound); zero numerators compare as values."""
+    import ctypes
+
+    lib = hc.lib()
+    out = (ctypes.c_longlong * 6)()
+    lib.hc_div_check(ctypes.c_longlong(2_000_000), ctypes.c_uint(7), out)
+    print("mismatches (div_rn exact/-1/+1 ulp estimate, div_by exact/-1/+1):", list(out))
+    assert list(out) == [0] * 6
