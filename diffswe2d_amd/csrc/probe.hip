@@ -1,5 +1,6 @@
 // Probes: the VALU issue rate the stage kernels are measured against, and the
-// kernels' FP32 division on the device next to the hardware IEEE division.
+// kernels' FP32 division and square root on the device next to the compiler's
+// IEEE forms.
 //
 // VALU issue-rate probe:
 // Each lane runs 8 independent FP32 multiply/add chains (no memory traffic in
@@ -38,7 +39,21 @@ __global__ __launch_bounds__(256) void division_probe_kernel(const float* a, con
   q_ieee[i] = x / y;
 }
 
+__global__ __launch_bounds__(256) void sqrt_probe_kernel(const float* x, int64_t n, float* s_rn, float* s_ieee) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  s_rn[i] = swe::sqrt_rn(x[i]);
+  s_ieee[i] = sqrtf(x[i]);
+}
+
 }  // namespace
+
+extern "C" int swe_sqrt_probe(const float* x, long long n, float* s_rn, float* s_ieee, void* stream) {
+  if (!x || !s_rn || !s_ieee || n <= 0) return SWE_ERR_ARG;
+  hipLaunchKernelGGL(sqrt_probe_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, (hipStream_t)stream, x,
+                     (int64_t)n, s_rn, s_ieee);
+  return hipGetLastError() == hipSuccess ? 0 : SWE_ERR_HIP;
+}
 
 extern "C" int swe_division_probe(const float* a, const float* b, long long n, float* q_rn, float* q_by,
                                   float* q_ieee, void* stream) {

@@ -182,8 +182,8 @@ SWE_HD void face_flux_bwd(const Side<T>& L, T hL_cell, T zL_cell, const Side<T>&
   T hL = clamp_lo(hcL, T(0)), hR = clamp_lo(hcR, T(0));
   bool wetL = hL > k.dry, wetR = hR > k.dry;
   T sfL = clamp_lo(hL, k.floor), sfR = clamp_lo(hR, k.floor);
-  T nL = wetL ? div_rn(qnL, sfL) : T(0), tL = wetL ? div_rn(qtL, sfL) : T(0), cL = wetL ? sqrt(k.g * sfL) : T(0);
-  T nR = wetR ? div_rn(qnR, sfR) : T(0), tR = wetR ? div_rn(qtR, sfR) : T(0), cR = wetR ? sqrt(k.g * sfR) : T(0);
+  T nL = wetL ? div_rn(qnL, sfL) : T(0), tL = wetL ? div_rn(qtL, sfL) : T(0), cL = wetL ? sqrt_rn(k.g * sfL) : T(0);
+  T nR = wetR ? div_rn(qnR, sfR) : T(0), tR = wetR ? div_rn(qtR, sfR) : T(0), cR = wetR ? sqrt_rn(k.g * sfR) : T(0);
   T um = (T(0.5) * (nL + nR) + cL) - cR;
   T cm = T(0.5) * (cL + cR) + T(0.25) * (nL - nR);
   bool dryL = hL <= k.dry, dryR = hR <= k.dry;
@@ -404,7 +404,7 @@ SWE_HD void friction_bwd(T h_in, T hu, T hv, T coef, T dt, const SweConsts<T>& k
   T s2 = u * u + v * v;
   bool moving = wet && (s2 > T(0));
   T ss2 = clamp_lo(s2, k.fric_floor2);
-  T speed = moving ? sqrt(ss2) : T(0);
+  T speed = moving ? sqrt_rn(ss2) : T(0);
   T sh = clamp_lo(h, k.fric_floor);
   T cf, gcf_dh = T(0);  // d cf / d h (through the clamp of h)
   if (model == 0) {

@@ -138,7 +138,7 @@ SWE_HD void apply_ghost(const GhostMap& g, T& h, T& hu, T& hv, T mirror_normal, 
     case BC_ABSORBING: {
       T h_ref = clamp_lo(level - z_ghost, T(0));
       T h_c = h_ref > T(1e-4) ? h_ref : h;
-      T c = sqrt(T(9.81) * clamp_lo(h_c, T(1e-6)));
+      T c = sqrt_rn(T(9.81) * clamp_lo(h_c, T(1e-6)));
       bool outward_pos = g.side == RIGHT || g.side == BOTTOM;
       T flow = outward_pos ? c * (h - h_ref) : (-c) * (h - h_ref);
       if (x_side) hu = flow;
@@ -191,7 +191,7 @@ SWE_HD void ghost_bwd(const GhostMap& g, T h_edge, T z_ghost, T level, T gh, T g
       bool use_ref = h_ref > T(1e-4);
       T h_c = use_ref ? h_ref : h_edge;
       T cc = clamp_lo(h_c, T(1e-6));
-      T c = sqrt(T(9.81) * cc);
+      T c = sqrt_rn(T(9.81) * cc);
       T diff = h_edge - h_ref;
       bool outward_pos = g.side == RIGHT || g.side == BOTTOM;
       T sgn = outward_pos ? T(1) : T(-1);

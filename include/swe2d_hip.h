@@ -335,6 +335,12 @@ int swe_valu_probe(float* out, int blocks, int iters, void* stream);
 int swe_division_probe(const float* a, const float* b, long long n, float* q_rn, float* q_by, float* q_ieee,
                        void* stream);
 
+/* Square-root probe: s_rn = the kernels' FP32 square root (hardware estimate
+ * and a pick among it and its neighbours), s_ieee = the compiler's IEEE square
+ * root, for n device arguments; equal bit for bit for arguments >= 2^-96.  No
+ * counterpart in the reference; a test hook. */
+int swe_sqrt_probe(const float* x, long long n, float* s_rn, float* s_ieee, void* stream);
+
 /* ---- water volume (FP64) -------------------------------------------------
  * volume[b] = sum over the interior cells of member b of h * dx * dy, summed
  * in FP64 in a fixed order (the same bits on every call).  volume is a device

@@ -429,3 +429,21 @@ extern "C" void hc_div_check(long long n, unsigned seed, long long* mismatch) {
     }
   }
 }
+
+// sqrt_rn_from (swe_math.h) against IEEE sqrt on n random arguments log-uniform
+// over [2^-96, 2^100], from the exact root and from one ulp either side of it
+// (the hardware estimate is within one ulp); mismatch[0..2] per estimate.
+extern "C" void hc_sqrt_check(long long n, unsigned seed, long long* mismatch) {
+  std::mt19937_64 rng(seed);
+  std::uniform_real_distribution<double> unit(1.0, 2.0), ex(-96.0, 100.0);
+  for (int i = 0; i < 3; ++i) mismatch[i] = 0;
+  for (long long i = 0; i < n; ++i) {
+    float x = (float)ldexp(unit(rng), (int)ex(rng));
+    const float want = sqrtf(x);
+    const float est[3] = {want, nextafterf(want, 0.0f), nextafterf(want, 2.0f * want)};
+    for (int j = 0; j < 3; ++j) {
+      float got = sqrt_rn_from(x, est[j]);
+      if (memcmp(&got, &want, 4) != 0) ++mismatch[j];
+    }
+  }
+}

@@ -418,3 +418,25 @@ def test_device_division_equals_ieee_division(cuda_device):
     print(f"division: {n} pairs, div_rn mismatches {bad_rn}, div_by mismatches {bad_by}")
     assert bad_rn == 0 and bad_by == 0
     assert bool((q_rn[zero] == 0).all()) and bool((q_by[zero] == 0).all())
+
+
+def test_device_square_root_equals_ieee_sqrt(cuda_device):
+    """The kernels' FP32 square root against the compiler's IEEE sqrt on the device: bit for bit over
+    16 M arguments log-uniform in [2^-96, 2^100] plus +-0, +inf and NaN."""
+    from diffswe2d_amd import _lib
+
+    n = 1 << 24
+    g = torch.Generator().manual_seed(9)
+    e = torch.randint(-96, 100, (n,), generator=g)
+    x = ((1 + torch.rand(n, generator=g, dtype=torch.float64)) * torch.pow(2.0, e.double())).float()
+    x[:4] = torch.tensor([0.0, -0.0, float("inf"), float("nan")])
+    x = x.to(cuda_device)
+    s_rn, s_ieee = torch.empty_like(x), torch.empty_like(x)
+    _lib.check(_lib.load().swe_sqrt_probe(_lib.ptr(x), n, _lib.ptr(s_rn), _lib.ptr(s_ieee),
+                                          _lib.stream_ptr(cuda_device)), "sqrt probe")
+    torch.cuda.synchronize()
+    bad = int((s_rn.view(torch.int32) != s_ieee.view(torch.int32))[4:].sum())
+    print(f"sqrt: {n} arguments, mismatches {bad}; specials {s_rn[:4].tolist()}")
+    assert bad == 0
+    assert s_rn[0].item() == 0.0 and str(s_rn[1].item()) == "-0.0" and s_rn[2].item() == float("inf")
+    assert math.isnan(s_rn[3].item())

@@ -144,3 +144,15 @@ def test_division_sequence_rounds_like_ieee_division():
     lib.hc_div_check(ctypes.c_longlong(2_000_000), ctypes.c_uint(7), out)
     print("mismatches (div_rn exact/-1/+1 ulp estimate, div_by exact/-1/+1):", list(out))
     assert list(out) == [0] * 6
+
+
+def test_square_root_pick_rounds_like_ieee_sqrt():
+    """sqrt_rn_from (the FP32 square root of the kernels) equals IEEE sqrt over random arguments in
+    [2^-96, 2^100], whether the estimate is the exact root or one ulp off either way."""
+    import ctypes
+
+    lib = hc.lib()
+    out = (ctypes.c_longlong * 3)()
+    lib.hc_sqrt_check(ctypes.c_longlong(2_000_000), ctypes.c_uint(5), out)
+    print("mismatches (exact / -1 ulp / +1 ulp estimate):", list(out))
+    assert list(out) == [0] * 3
