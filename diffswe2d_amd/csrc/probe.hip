@@ -1,6 +1,6 @@
 // Probes: the VALU issue rate the stage kernels are measured against, and the
-// kernels' FP32 division and square root on the device next to the compiler's
-// IEEE forms.
+// kernels' FP32 division, square root and friction power on the device next to
+// the compiler's IEEE forms and powf.
 //
 // VALU issue-rate probe:
 // Each lane runs 8 independent FP32 multiply/add chains (no memory traffic in
@@ -46,7 +46,22 @@ __global__ __launch_bounds__(256) void sqrt_probe_kernel(const float* x, int64_t
   s_ieee[i] = sqrtf(x[i]);
 }
 
+__global__ __launch_bounds__(256) void pow_probe_kernel(const float* x, int64_t n, float third, float* y_fast,
+                                                        float* y_powf) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  y_fast[i] = swe::pow_third(x[i], third);
+  y_powf[i] = powf(x[i], third);
+}
+
 }  // namespace
+
+extern "C" int swe_pow_probe(const float* x, long long n, float third, float* y_fast, float* y_powf, void* stream) {
+  if (!x || !y_fast || !y_powf || n <= 0) return SWE_ERR_ARG;
+  hipLaunchKernelGGL(pow_probe_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, (hipStream_t)stream, x,
+                     (int64_t)n, third, y_fast, y_powf);
+  return hipGetLastError() == hipSuccess ? 0 : SWE_ERR_HIP;
+}
 
 extern "C" int swe_sqrt_probe(const float* x, long long n, float* s_rn, float* s_ieee, void* stream) {
   if (!x || !s_rn || !s_ieee || n <= 0) return SWE_ERR_ARG;

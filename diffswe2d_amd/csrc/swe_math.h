@@ -310,8 +310,43 @@ SWE_HD void clean_and_limit(T oh, T ou, T ov, T& ch, T& cu, T& cv, const SweCons
 }
 
 // Semi-implicit Manning friction (friction.py:12-38, 62-66, 102-131).
+// h^third with third = float(1/3) = 1/3 + 9.93e-9 (friction.py:66).  FP32: for
+// a positive normal h (the friction floor keeps h >= 1e-6) the cube root is
+// taken from a hardware log2 / exp2 estimate by two Newton steps on y^3 = h in
+// FP64, then multiplied by h^(third - 1/3) = exp(t) ~ 1 + t + t^2 / 2 with
+// t = (third - 1/3) ln h in FP64; the result is h^third rounded to nearest
+// but for arguments within ~1e-14 of a rounding boundary, which land one ulp
+// away.  This replaces a generic powf of ~160 instructions by ~30.  Other
+// arguments go to powf; FP64 keeps pow.
+SWE_HD float cbrt_estimate(float x, float& log2x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  log2x = __builtin_amdgcn_logf(x);
+  return __builtin_amdgcn_exp2f(log2x * (1.0f / 3.0f));
+#else
+  log2x = log2f(x);
+  return exp2f(log2x * (1.0f / 3.0f));
+#endif
+}
+
+SWE_HD float pow_third_from(float x, float third, float y0, float log2x) {
+  const double X = x;
+  double Y = y0;
+  const double r = (double)rcp_estimate(3.0f * y0 * y0);  // Newton slope; its error only scales the step
+  Y = __builtin_fma(-__builtin_fma(Y * Y, Y, -X), r, Y);
+  Y = __builtin_fma(-__builtin_fma(Y * Y, Y, -X), r, Y);
+  const double t = ((double)third - 1.0 / 3.0) * ((double)log2x * 0.6931471805599453);
+  return (float)(Y * __builtin_fma(t, __builtin_fma(t, 0.5, 1.0), 1.0));
+}
+
 template <class T> SWE_HD T pow_third(T x, T third);
-template <> SWE_HD float pow_third<float>(float x, float third) { return powf(x, third); }
+
+template <> SWE_HD float pow_third<float>(float x, float third) {
+  if (!(x >= 1.17549435e-38f && x <= 3.40282347e+38f)) return powf(x, third);
+  float log2x;
+  const float y0 = cbrt_estimate(x, log2x);
+  return pow_third_from(x, third, y0, log2x);
+}
+
 template <> SWE_HD double pow_third<double>(double x, double third) { return pow(x, third); }
 
 template <class T> SWE_HD T log_t(T x);

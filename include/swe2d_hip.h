@@ -341,6 +341,11 @@ int swe_division_probe(const float* a, const float* b, long long n, float* q_rn,
  * counterpart in the reference; a test hook. */
 int swe_sqrt_probe(const float* x, long long n, float* s_rn, float* s_ieee, void* stream);
 
+/* Friction-power probe: y_fast = the kernels' FP32 x^third (a cube root refined
+ * in FP64, times x^(third - 1/3)), y_powf = powf(x, third), for n device
+ * arguments.  No counterpart in the reference; a test hook. */
+int swe_pow_probe(const float* x, long long n, float third, float* y_fast, float* y_powf, void* stream);
+
 /* ---- water volume (FP64) -------------------------------------------------
  * volume[b] = sum over the interior cells of member b of h * dx * dy, summed
  * in FP64 in a fixed order (the same bits on every call).  volume is a device

@@ -447,3 +447,30 @@ extern "C" void hc_sqrt_check(long long n, unsigned seed, long long* mismatch) {
     }
   }
 }
+
+// pow_third_from (swe_math.h) against h^third rounded to nearest from a double
+// evaluation, for n arguments log-uniform over [1e-6, 1e4] (the friction's
+// depths); the log2 and cube-root estimates are taken exact and moved by
+// +-1e-6 relative (a bound for the hardware log2 / exp2).  out[0] = arguments
+// off by one ulp, out[1] = off by more, out[2] = the same for powf.
+extern "C" void hc_pow_check(long long n, unsigned seed, long long* out) {
+  std::mt19937_64 rng(seed);
+  std::uniform_real_distribution<double> lx(log(1e-6), log(1e4)), pert(-1e-6, 1e-6);
+  const float third = (float)(1.0 / 3.0);
+  out[0] = out[1] = out[2] = 0;
+  for (long long i = 0; i < n; ++i) {
+    const float x = (float)exp(lx(rng));
+    const float want = (float)pow((double)x, (double)third);
+    const float l2 = (float)(log2((double)x) * (1.0 + pert(rng)));
+    const float y0 = (float)(cbrt((double)x) * (1.0 + pert(rng)));
+    const float got = pow_third_from(x, third, y0, l2);
+    int32_t a, b;
+    memcpy(&a, &got, 4);
+    memcpy(&b, &want, 4);
+    const int32_t d = a > b ? a - b : b - a;
+    if (d == 1) ++out[0];
+    if (d > 1) ++out[1];
+    const float pf = powf(x, third);
+    if (memcmp(&pf, &want, 4) != 0) ++out[2];
+  }
+}

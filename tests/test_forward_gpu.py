@@ -4,6 +4,7 @@ Bars (FP32): tendency and states bitwise or within a few ulps of the oracle on
 small grids; after the full 500-step dam break rel-L2 <= 1e-5, identical
 wet/dry masks and dt series within 1e-6 relative.
 """
+import ctypes
 import math
 
 import numpy as np
@@ -440,3 +441,27 @@ def test_device_square_root_equals_ieee_sqrt(cuda_device):
     assert bad == 0
     assert s_rn[0].item() == 0.0 and str(s_rn[1].item()) == "-0.0" and s_rn[2].item() == float("inf")
     assert math.isnan(s_rn[3].item())
+
+
+def test_device_friction_power_rounds_to_nearest(cuda_device):
+    """The kernels' FP32 h^third (Manning friction) on the device against h^third rounded from an FP64
+    evaluation on the host, over 4 M depths log-uniform in [1e-6, 1e4]: at most one ulp away, and
+    for at most 1 in 10^5 arguments; powf's own count is reported beside it."""
+    from diffswe2d_amd import _lib
+
+    n = 1 << 22
+    g = torch.Generator().manual_seed(13)
+    x64 = torch.exp(torch.empty(n, dtype=torch.float64).uniform_(math.log(1e-6), math.log(1e4), generator=g))
+    x = x64.float()
+    third = float(np.float32(1.0 / 3.0))
+    want = torch.pow(x.double(), third).float()
+    xd = x.to(cuda_device)
+    y_fast, y_powf = torch.empty_like(xd), torch.empty_like(xd)
+    _lib.check(_lib.load().swe_pow_probe(_lib.ptr(xd), n, ctypes.c_float(third), _lib.ptr(y_fast), _lib.ptr(y_powf),
+                                         _lib.stream_ptr(cuda_device)), "pow probe")
+    torch.cuda.synchronize()
+    d_fast = (y_fast.cpu().view(torch.int32).long() - want.view(torch.int32).long()).abs()
+    d_powf = (y_powf.cpu().view(torch.int32).long() - want.view(torch.int32).long()).abs()
+    print(f"pow_third: {int((d_fast == 1).sum())} one ulp off, {int((d_fast > 1).sum())} further, of {n}; "
+          f"powf: {int((d_powf == 1).sum())} one ulp off, {int((d_powf > 1).sum())} further")
+    assert int((d_fast > 1).sum()) == 0 and int((d_fast == 1).sum()) <= n // 100_000

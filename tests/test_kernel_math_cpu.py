@@ -156,3 +156,17 @@ def test_square_root_pick_rounds_like_ieee_sqrt():
     lib.hc_sqrt_check(ctypes.c_longlong(2_000_000), ctypes.c_uint(5), out)
     print("mismatches (exact / -1 ulp / +1 ulp estimate):", list(out))
     assert list(out) == [0] * 3
+
+
+def test_friction_cube_root_rounds_to_nearest():
+    """pow_third_from (the FP32 h^third of the Manning friction) against h^third rounded from a double
+    evaluation over depths 1e-6..1e4, with log2 / cube-root estimates 1e-6 off: never more than one ulp
+    away, and one ulp away for at most 1 in 10^5 arguments."""
+    import ctypes
+
+    lib = hc.lib()
+    out = (ctypes.c_longlong * 3)()
+    n = 4_000_000
+    lib.hc_pow_check(ctypes.c_longlong(n), ctypes.c_uint(11), out)
+    print(f"pow_third: {out[0]} one ulp off, {out[1]} further, of {n} (host powf differs for {out[2]})")
+    assert out[1] == 0 and out[0] <= n // 100_000
