@@ -421,8 +421,9 @@ def main():
                 "unit": "T lane-instr/s", "frac": round(got / peak, 4),
                 "lane_instr_per_cell_update": round(per_cell, 1),
                 "source": "profiles/basin8192_fp32_traffic.json (SQ_INSTS_VALU); peak from swe_valu_probe",
-                "note": "peak is the measured issue rate of plain FP32 mul/add; a frac near or above 1 means "
-                        "the stage kernels issue VALU at that ceiling"}
+                "note": "peak is the measured issue rate of a plain FP32 mul/add stream; the stage kernels "
+                        "issue above it (frac > 1) with reciprocal / square-root instructions overlapping the "
+                        "rest, and their time follows their instruction count"}
         print(f"# VALU: {got / 1e12:.2f} of {peak / 1e12:.2f} T lane-instr/s measured by the probe "
               f"({got / peak:.2f})", file=sys.stderr, flush=True)
         traffic_src = "profiles/basin8192_fp32_traffic.json (FETCH_SIZE x2.0 + WRITE_SIZE x1.0, copy-calibrated)"
