@@ -117,6 +117,7 @@ _SIGNATURES = {
     "swe_division_probe": ([_P, _P, ctypes.c_longlong, _P, _P, _P, _P], _I),
     "swe_sqrt_probe": ([_P, ctypes.c_longlong, _P, _P, _P], _I),
     "swe_pow_probe": ([_P, ctypes.c_longlong, ctypes.c_float, _P, _P, _P], _I),
+    "swe_lane_shift_probe": ([_P, ctypes.c_longlong, _P, _P, _P, _P, _P], _I),
     "swe_water_volume_f32": ([_G, _P, _P, _P, _P], _I),
     "swe_water_volume_f64": ([_G, _P, _P, _P, _P], _I),
     "swe_ensemble_workspace_bytes": ([_I], ctypes.c_size_t),

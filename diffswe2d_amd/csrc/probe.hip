@@ -1,6 +1,6 @@
-// Probes: the VALU issue rate the stage kernels are measured against, and the
+// Probes: the VALU issue rate the stage kernels are measured against, the
 // kernels' FP32 division, square root and friction power on the device next to
-// the compiler's IEEE forms and powf.
+// the compiler's IEEE forms and powf, and their one-lane wavefront shifts.
 //
 // VALU issue-rate probe:
 // Each lane runs 8 independent FP32 multiply/add chains (no memory traffic in
@@ -10,6 +10,7 @@
 #include <hip/hip_runtime.h>
 
 #include "../../include/swe2d_hip.h"
+#include "swe_fetch.h"
 #include "swe_math.h"
 
 namespace {
@@ -54,7 +55,25 @@ __global__ __launch_bounds__(256) void pow_probe_kernel(const float* x, int64_t 
   y_powf[i] = powf(x[i], third);
 }
 
+__global__ __launch_bounds__(256) void lane_shift_probe_kernel(const double* x, double* up64, double* dn64,
+                                                               float* up32, float* dn32) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const double v = x[i];
+  up64[i] = swe::shup(v);
+  dn64[i] = swe::shdn(v);
+  up32[i] = swe::shup((float)v);
+  dn32[i] = swe::shdn((float)v);
+}
+
 }  // namespace
+
+extern "C" int swe_lane_shift_probe(const double* x, long long n, double* up64, double* dn64, float* up32,
+                                    float* dn32, void* stream) {
+  if (!x || !up64 || !dn64 || !up32 || !dn32 || n <= 0 || n % 256 != 0) return SWE_ERR_ARG;
+  hipLaunchKernelGGL(lane_shift_probe_kernel, dim3((unsigned)(n / 256)), dim3(256), 0, (hipStream_t)stream, x, up64,
+                     dn64, up32, dn32);
+  return hipGetLastError() == hipSuccess ? 0 : SWE_ERR_HIP;
+}
 
 extern "C" int swe_pow_probe(const float* x, long long n, float third, float* y_fast, float* y_powf, void* stream) {
   if (!x || !y_fast || !y_powf || n <= 0) return SWE_ERR_ARG;

@@ -5,17 +5,46 @@
 
 namespace swe {
 
-template <class T> __device__ inline T shup(T v) { return __shfl_up(v, 1, kWave); }
-template <class T> __device__ inline T shdn(T v) { return __shfl_down(v, 1, kWave); }
+// One-lane shifts across the whole 64-lane wavefront.  DPP = true: single DPP
+// moves in the VALU (wave_shr:1 = control 0x138: lane i reads lane i - 1;
+// wave_shl:1 = 0x130: lane i reads lane i + 1); the lane with no source (0 for
+// shup, 63 for shdn) reads 0 (bound_ctrl).  DPP = false: __shfl_up / __shfl_down
+// by 1, an LDS-routed ds_bpermute that keeps the VALU free and leaves the
+// sourceless lane its own value.  Each kernel takes the form measured faster for
+// it.  The kernels' halo lanes absorb the sourceless lane's value: no stored
+// result and no cross-lane reduction depends on a value shifted into lane 0 or
+// lane 63.  64-bit values move as two 32-bit halves.
+template <int CTRL> __device__ inline int dpp_shift(int x) {
+  return __builtin_amdgcn_mov_dpp(x, CTRL, 0xf, 0xf, true);
+}
+template <int CTRL, class T> __device__ inline T dpp_shift_value(T v) {
+  if constexpr (sizeof(T) == 4) {
+    return __builtin_bit_cast(T, dpp_shift<CTRL>(__builtin_bit_cast(int, v)));
+  } else {
+    static_assert(sizeof(T) == 8, "32- or 64-bit lane values");
+    long long w = __builtin_bit_cast(long long, v);
+    int lo = dpp_shift<CTRL>((int)(w & 0xffffffffLL));
+    int hi = dpp_shift<CTRL>((int)(w >> 32));
+    return __builtin_bit_cast(T, (long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
+  }
+}
+template <bool DPP = true, class T> __device__ inline T shup(T v) {
+  if constexpr (DPP) return dpp_shift_value<0x138>(v);
+  else return __shfl_up(v, 1, kWave);
+}
+template <bool DPP = true, class T> __device__ inline T shdn(T v) {
+  if constexpr (DPP) return dpp_shift_value<0x130>(v);
+  else return __shfl_down(v, 1, kWave);
+}
 
-template <class T> __device__ inline Cell<T> shup(const Cell<T>& c) {
-  return Cell<T>{shup(c.h), shup(c.un), shup(c.ut), shup(c.eta), shup(c.z)};
+template <bool DPP = true, class T> __device__ inline Cell<T> shup(const Cell<T>& c) {
+  return Cell<T>{shup<DPP>(c.h), shup<DPP>(c.un), shup<DPP>(c.ut), shup<DPP>(c.eta), shup<DPP>(c.z)};
 }
-template <class T> __device__ inline Cell<T> shdn(const Cell<T>& c) {
-  return Cell<T>{shdn(c.h), shdn(c.un), shdn(c.ut), shdn(c.eta), shdn(c.z)};
+template <bool DPP = true, class T> __device__ inline Cell<T> shdn(const Cell<T>& c) {
+  return Cell<T>{shdn<DPP>(c.h), shdn<DPP>(c.un), shdn<DPP>(c.ut), shdn<DPP>(c.eta), shdn<DPP>(c.z)};
 }
-template <class T> __device__ inline Side<T> shdn(const Side<T>& s) {
-  return Side<T>{shdn(s.hlin), shdn(s.eta), shdn(s.z), shdn(s.un), shdn(s.ut)};
+template <bool DPP = true, class T> __device__ inline Side<T> shdn(const Side<T>& s) {
+  return Side<T>{shdn<DPP>(s.hlin), shdn<DPP>(s.eta), shdn<DPP>(s.z), shdn<DPP>(s.un), shdn<DPP>(s.ut)};
 }
 
 // eta = h + z is formed where a Cell is built (the value primitive_cell gives),

@@ -126,6 +126,8 @@ __device__ inline void stage_strip(const StageArgs<T>& a, int strip, int seg, in
   const int lane = threadIdx.x & (kWave - 1);
   const Fields<T>& f = a.f;
   const SweConsts<T>& k = a.k;
+  // stage 1 runs faster with LDS-routed lane shifts, stage 2 with DPP moves
+  constexpr bool kDpp = STAGE == 2;
   const int x0 = strip * kOut;
   const int y0 = seg * a.S;
   const int col = x0 - kHalo + lane;
@@ -160,14 +162,14 @@ __device__ inline void stage_strip(const StageArgs<T>& a, int strip, int seg, in
           T ry_v = div_by(gfn - (G.fn - G.sL), k.ddy);
           // x sweep of row o
           Cell<T> cx = P2.x();
-          Cell<T> cm = shup(cx), cp = shdn(cx);
+          Cell<T> cm = shup<kDpp>(cx), cp = shdn<kDpp>(cx);
           Incr<T> xi = increments(cm, cx, cp, k, a.wsr != 0);
           Side<T> right_of_face = side_minus(cx, xi, k);
           Side<T> left_of_face = side_plus(cx, xi, k);
-          Side<T> nxt = shdn(right_of_face);
+          Side<T> nxt = shdn<kDpp>(right_of_face);
           FaceFlux<T> F = face_flux(left_of_face, cx.h, cx.z, nxt, cp.h, cp.z, k);
           if (xface_ok) ax = tmax(ax, F.speed);
-          T pfh = shup(F.fh), pft = shup(F.ft), pfn = shup(F.fn - F.sR);
+          T pfh = shup<kDpp>(F.fh), pft = shup<kDpp>(F.ft), pfn = shup<kDpp>(F.fn - F.sR);
           T rx_h = div_by(pfh - F.fh, k.ddx);
           T rx_u = div_by(pfn - (F.fn - F.sL), k.ddx);
           T rx_v = div_by(pft - F.ft, k.ddx);

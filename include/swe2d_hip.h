@@ -346,6 +346,13 @@ int swe_sqrt_probe(const float* x, long long n, float* s_rn, float* s_ieee, void
  * arguments.  No counterpart in the reference; a test hook. */
 int swe_pow_probe(const float* x, long long n, float third, float* y_fast, float* y_powf, void* stream);
 
+/* Lane-shift probe: the kernels' one-lane wavefront shifts on n device values
+ * (n a multiple of 256; 64-lane wavefronts over consecutive elements):
+ * up[i] = x[i - 1] and dn[i] = x[i + 1] within a wavefront, 0 in the lane with
+ * no source; as FP64 (two 32-bit halves) and as FP32 of (float)x.  No counterpart in the reference; a test hook. */
+int swe_lane_shift_probe(const double* x, long long n, double* up64, double* dn64, float* up32, float* dn32,
+                         void* stream);
+
 /* ---- water volume (FP64) -------------------------------------------------
  * volume[b] = sum over the interior cells of member b of h * dx * dy, summed
  * in FP64 in a fixed order (the same bits on every call).  volume is a device

@@ -465,3 +465,30 @@ def test_device_friction_power_rounds_to_nearest(cuda_device):
     print(f"pow_third: {int((d_fast == 1).sum())} one ulp off, {int((d_fast > 1).sum())} further, of {n}; "
           f"powf: {int((d_powf == 1).sum())} one ulp off, {int((d_powf > 1).sum())} further")
     assert int((d_fast > 1).sum()) == 0 and int((d_fast == 1).sum()) <= n // 100_000
+
+
+@pytest.mark.gpu
+def test_device_lane_shifts(cuda_device):
+    """The kernels' one-lane wavefront shifts (DPP moves) against the shift they stand for, per 64-lane
+    wavefront of consecutive elements: up[i] = x[i - 1], dn[i] = x[i + 1], 0 in the lane with no source;
+    FP64 values bit for bit (both 32-bit halves move) and FP32 values of (float)x."""
+    from diffswe2d_amd import _lib
+
+    n = 1024
+    g = torch.Generator().manual_seed(5)
+    x = torch.randn(n, dtype=torch.float64, generator=g) * 1e3
+    xd = x.to(cuda_device)
+    up64, dn64 = torch.empty_like(xd), torch.empty_like(xd)
+    up32 = torch.empty(n, dtype=torch.float32, device=cuda_device)
+    dn32 = torch.empty_like(up32)
+    _lib.check(_lib.load().swe_lane_shift_probe(_lib.ptr(xd), n, _lib.ptr(up64), _lib.ptr(dn64), _lib.ptr(up32),
+                                                _lib.ptr(dn32), _lib.stream_ptr(cuda_device)), "lane shift probe")
+    torch.cuda.synchronize()
+    lane = torch.arange(n) % 64
+    idx = torch.arange(n)
+    want_up = torch.where(lane > 0, x[(idx - 1).clamp(min=0)], 0.0)
+    want_dn = torch.where(lane < 63, x[(idx + 1).clamp(max=n - 1)], 0.0)
+    assert torch.equal(up64.cpu().view(torch.int64), want_up.view(torch.int64))
+    assert torch.equal(dn64.cpu().view(torch.int64), want_dn.view(torch.int64))
+    assert torch.equal(up32.cpu(), want_up.float())
+    assert torch.equal(dn32.cpu(), want_dn.float())
