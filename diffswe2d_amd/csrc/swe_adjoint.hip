@@ -43,18 +43,20 @@ template <class T> struct AdjArgs {
   const T* U;            // state at step start
   const T* k1;           // first-stage tendency (stage 2 rebuilds the predictor from it)
   T half_dt;
-  const T* lam;          // adjoint of the stage tendency [B,3,ny,nx]
-  const T* base;         // added to the interior output (nullable)
+  const T* lam;          // adjoint of the stage tendency [B,3,ny,nx] (unused when lam_P is set)
+  const T* base;         // added to the interior output (nullable; required when lam_P is set)
   T* out;                // adjoint of the stage input state [B,3,ny,nx]
   T* ghost;              // [B][4 sides][2 layers][lmax][3]
   int lmax;
   // elementwise reverses
   const T* k2;           // second-stage tendency
   const T* lam_out;      // adjoint of the step output
-  const T* lam_P;        // adjoint of the predictor state
+  const T* lam_P;        // adjoint of the predictor state: when set, stage-1 L^T reverses the
+                         // predictor clean-up itself (lam of each row from lam_P, k1 and U; the
+                         // clean-up's direct adjoint of U added to its output)
   T dt;
   int friction, friction_model;
-  T* acc;                // adjoint reaching U directly (written by the corrector, updated by the predictor)
+  T* acc;                // adjoint reaching U directly past the corrector (written by the corrector)
   T* part;               // per-block level-gradient partials of the ghost fold
   int part_blocks;       // blocks per side in the ghost fold
 };
@@ -149,6 +151,15 @@ __device__ inline void predictor_bwd(const AdjArgs<T>& a, int64_t i, Lam3<T>& la
   lam = Lam3<T>{half * gc_h, half * gc_u, half * gc_v};
 }
 
+// adjoint of k1 at padded cell (r, c): the predictor clean-up reversed on interior
+// cells, 0 outside the domain (the stage tendency exists on interior cells only)
+template <class T> __device__ inline Lam3<T> predictor_lam(const AdjArgs<T>& a, int b, int r, int c) {
+  if (r < 0 || r >= a.f.ny || c < 0 || c >= a.f.nx) return Lam3<T>{T(0), T(0), T(0)};
+  Lam3<T> lam, acc;
+  predictor_bwd(a, (int64_t)b * 3 * a.f.plane + (int64_t)r * a.f.nx + c, lam, acc);
+  return lam;
+}
+
 // one thread per interior cell: lam_k2 = adjoint of k2, acc = adjoint reaching U
 template <class T>
 __global__ __launch_bounds__(256) void adj_corrector_kernel(AdjArgs<T> a, T* lam_k2, int64_t n) {
@@ -165,24 +176,6 @@ __global__ __launch_bounds__(256) void adj_corrector_kernel(AdjArgs<T> a, T* lam
   lam_k2[i] = lam.h;
   lam_k2[i + P] = lam.u;
   lam_k2[i + 2 * P] = lam.v;
-}
-
-// one thread per interior cell: lam_k1 = adjoint of k1, acc += adjoint reaching U
-template <class T>
-__global__ __launch_bounds__(256) void adj_predictor_kernel(AdjArgs<T> a, T* lam_k1, int64_t n) {
-  int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  if (t >= n) return;
-  int b = (int)(t / a.f.plane);
-  int64_t cell = t - (int64_t)b * a.f.plane;
-  int64_t i = (int64_t)b * 3 * a.f.plane + cell, P = a.f.plane;
-  Lam3<T> lam, acc;
-  predictor_bwd(a, i, lam, acc);
-  a.acc[i] += acc.h;
-  a.acc[i + P] += acc.u;
-  a.acc[i + 2 * P] += acc.v;
-  lam_k1[i] = lam.h;
-  lam_k1[i + P] = lam.u;
-  lam_k1[i + 2 * P] = lam.v;
 }
 
 // x sweep of one row (all lanes take part): adjoint of the primitives of each
@@ -251,7 +244,7 @@ void adj_L_kernel(AdjArgs<T> a) {
   for (int it = 0; it < rows + 6; ++it) {
     const int n = y0 - kHaloA + it;
     Prim<T> Pn = fetch<T, STAGE, AdjArgs<T>>(a, b, n, col, half);
-    Lam3<T> Ln = load_lam(a, b, n, col);
+    Lam3<T> Ln = STAGE == 1 && a.lam_P != nullptr ? predictor_lam(a, b, n, col) : load_lam(a, b, n, col);
     Incr<T> yi_cur{};
     bool reset_cur = false;
     PrimGrad<T> An{T(0), T(0), T(0), T(0)};
@@ -290,9 +283,18 @@ void adj_L_kernel(AdjArgs<T> a) {
         if (r >= 0 && r < f.ny && col >= 0 && col < f.nx) {
           int64_t i = (int64_t)b * 3 * f.plane + (int64_t)r * f.nx + col;
           if (a.base != nullptr) {
-            lh += a.base[i];
-            lu += a.base[i + f.plane];
-            lv += a.base[i + 2 * f.plane];
+            T bh = a.base[i], bu = a.base[i + f.plane], bv = a.base[i + 2 * f.plane];
+            if (STAGE == 1 && a.lam_P != nullptr) {
+              // the predictor clean-up's direct adjoint of U, after the corrector's
+              Lam3<T> pl, pacc;
+              predictor_bwd(a, i, pl, pacc);
+              bh += pacc.h;
+              bu += pacc.u;
+              bv += pacc.v;
+            }
+            lh += bh;
+            lu += bu;
+            lv += bv;
           }
           a.out[i] = lh;
           a.out[i + f.plane] = lu;
@@ -461,7 +463,7 @@ constexpr int kGhostThreads = 256;
 constexpr int kGhostOneBlock = 2048;  // longest side folded by one block per member
 
 template <class T> struct AdjWs {
-  T *lam_acc, *lam_P, *lam_k, *ghost, *part;  // lam_k holds lam_k2, then lam_k1
+  T *lam_acc, *lam_P, *lam_k, *ghost, *part;  // lam_k holds lam_k2
   int lmax, part_blocks;
 };
 
@@ -581,12 +583,11 @@ int step_adjoint_impl(const swe_grid* g, const swe_config* c, const T* U, const 
   a2.out = w.lam_P;
   hipLaunchKernelGGL((adj_L_kernel<T, 2>), lg, lb, 0, s, a2);
   ghost_fold(a2, 2, g->batch, lam_level, s);
-  // stage 1: predictor clean-up reversed, then L^T at U
+  // stage 1: predictor clean-up reversed inside L^T at U (rows of lam formed from
+  // lam_P, k1 and U as the sweep reaches them), plus the corrector's direct part
   AdjArgs<T> a1 = a;
   a1.lam_P = w.lam_P;
-  a1.acc = w.lam_acc;
-  hipLaunchKernelGGL((adj_predictor_kernel<T>), eg, eb, 0, s, a1, w.lam_k, n);
-  a1.lam = w.lam_k;
+  a1.lam = nullptr;
   a1.base = w.lam_acc;
   a1.out = lam_U;
   hipLaunchKernelGGL((adj_L_kernel<T, 1>), lg, lb, 0, s, a1);
