@@ -76,19 +76,55 @@ __device__ inline void load_state(const Args& a, int64_t base, int r, int c, T h
   }
 }
 
+// A stencil cell's loads (its source cell's state, stage-1 tendency and bed),
+// issued apart from the arithmetic on them so that a kernel can do unrelated
+// work while they are in flight.
+template <class T> struct CellLoads {
+  GhostMap g;
+  T h, hu, hv, k1h, k1u, k1v, z;
+};
+
 template <class T, int STAGE, class Args>
-__device__ inline Prim<T> fetch(const Args& a, int b, int r, int c, T half_dt) {
+__device__ inline CellLoads<T> fetch_loads(const Args& a, int b, int r, int c) {
   const Fields<T>& f = a.f;
-  GhostMap g = ghost_map(r, c, f.ny, f.nx, f.bc);
-  int64_t base = (int64_t)b * 3 * f.plane;
-  T h, hu, hv;
-  load_state<T, STAGE, Args>(a, base, g.r, g.c, half_dt, h, hu, hv);
-  T z = f.bed[(int64_t)b * f.bed_bstride + (int64_t)g.r * f.nx + g.c];
+  CellLoads<T> w;
+  w.g = ghost_map(r, c, f.ny, f.nx, f.bc);
+  const int64_t i = (int64_t)b * 3 * f.plane + (int64_t)w.g.r * f.nx + w.g.c;
+  const int64_t P = f.plane;
+  w.h = a.U[i];
+  w.hu = a.U[i + P];
+  w.hv = a.U[i + 2 * P];
+  if (STAGE == 2) {
+    w.k1h = a.k1[i];
+    w.k1u = a.k1[i + P];
+    w.k1v = a.k1[i + 2 * P];
+  }
+  w.z = f.bed[(int64_t)b * f.bed_bstride + (int64_t)w.g.r * f.nx + w.g.c];
+  return w;
+}
+
+// the cell from its loads: the predictor clean-up (stage 2), the ghost rule,
+// the primitives -- the arithmetic of load_state and fetch, in their order
+template <class T, int STAGE, class Args>
+__device__ inline Prim<T> fetch_finish(const Args& a, int b, const CellLoads<T>& w, T half_dt) {
+  const Fields<T>& f = a.f;
+  const GhostMap& g = w.g;
+  T h = w.h, hu = w.hu, hv = w.hv;
+  if (STAGE == 2) {
+    T ch = h + half_dt * w.k1h;
+    T cu = hu + half_dt * w.k1u;
+    T cv = hv + half_dt * w.k1v;
+    clean_and_limit(h, hu, hv, ch, cu, cv, a.k, a.positivity != 0);
+    h = ch;
+    hu = cu;
+    hv = cv;
+  }
+  const T z = w.z;
   if (g.kind >= 0) {
     T mirror = T(0);
     if (g.kind == BC_WALL) {
       T mh, mu, mv;
-      load_state<T, STAGE, Args>(a, base, g.mr, g.mc, half_dt, mh, mu, mv);
+      load_state<T, STAGE, Args>(a, (int64_t)b * 3 * f.plane, g.mr, g.mc, half_dt, mh, mu, mv);
       mirror = (g.side == LEFT || g.side == RIGHT) ? mu : mv;
     }
     T level = (g.kind == BC_WATER_LEVEL || g.kind == BC_ABSORBING) ? f.level[b * f.level_stride] : T(0);
@@ -102,6 +138,11 @@ __device__ inline Prim<T> fetch(const Args& a, int b, int r, int c, T half_dt) {
   p.qu = hu;
   p.qv = hv;
   return p;
+}
+
+template <class T, int STAGE, class Args>
+__device__ inline Prim<T> fetch(const Args& a, int b, int r, int c, T half_dt) {
+  return fetch_finish<T, STAGE, Args>(a, b, fetch_loads<T, STAGE, Args>(a, b, r, c), half_dt);
 }
 
 }  // namespace swe

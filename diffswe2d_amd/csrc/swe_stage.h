@@ -144,7 +144,25 @@ __device__ inline void stage_strip(const StageArgs<T>& a, int strip, int seg, in
 
   for (int it = 0; it < rows + 4; ++it) {
     const int r = y0 - kHalo + it;
-    Prim<T> Pn = fetch<T, STAGE>(a, b, r, col, half_dt);
+    // row r's loads go out first; the x sweep of row o = r - 2 does not need them
+    CellLoads<T> raw = fetch_loads<T, STAGE>(a, b, r, col);
+    T rx_h = T(0), rx_u = T(0), rx_v = T(0);
+    if (it >= 4) {
+      // x sweep of row o (its state is P2)
+      Cell<T> cx = P2.x();
+      Cell<T> cm = shup<kDpp>(cx), cp = shdn<kDpp>(cx);
+      Incr<T> xi = increments(cm, cx, cp, k, a.wsr != 0);
+      Side<T> right_of_face = side_minus(cx, xi, k);
+      Side<T> left_of_face = side_plus(cx, xi, k);
+      Side<T> nxt = shdn<kDpp>(right_of_face);
+      FaceFlux<T> F = face_flux(left_of_face, cx.h, cx.z, nxt, cp.h, cp.z, k);
+      if (xface_ok) ax = tmax(ax, F.speed);
+      T pfh = shup<kDpp>(F.fh), pft = shup<kDpp>(F.ft), pfn = shup<kDpp>(F.fn - F.sR);
+      rx_h = div_by(pfh - F.fh, k.ddx);
+      rx_u = div_by(pfn - (F.fn - F.sL), k.ddx);
+      rx_v = div_by(pft - F.ft, k.ddx);
+    }
+    Prim<T> Pn = fetch_finish<T, STAGE>(a, b, raw, half_dt);
     if (it >= 2) {
       // y increments of row r-1 and its two face states
       Cell<T> c1 = P1.y();
@@ -155,24 +173,11 @@ __device__ inline void stage_strip(const StageArgs<T>& a, int strip, int seg, in
         FaceFlux<T> G = face_flux(carryL, P2.h, P2.z, below, P1.h, P1.z, k);
         if (ycol_ok && r - 2 >= -1 && r - 2 <= f.ny - 1) ay = tmax(ay, G.speed);
         if (it >= 4) {
-          // output row o = r - 2 (its state is P2)
+          // output row o = r - 2
           const int o = r - 2;
           T ry_h = div_by(gfh - G.fh, k.ddy);
           T ry_u = div_by(gft - G.ft, k.ddy);
           T ry_v = div_by(gfn - (G.fn - G.sL), k.ddy);
-          // x sweep of row o
-          Cell<T> cx = P2.x();
-          Cell<T> cm = shup<kDpp>(cx), cp = shdn<kDpp>(cx);
-          Incr<T> xi = increments(cm, cx, cp, k, a.wsr != 0);
-          Side<T> right_of_face = side_minus(cx, xi, k);
-          Side<T> left_of_face = side_plus(cx, xi, k);
-          Side<T> nxt = shdn<kDpp>(right_of_face);
-          FaceFlux<T> F = face_flux(left_of_face, cx.h, cx.z, nxt, cp.h, cp.z, k);
-          if (xface_ok) ax = tmax(ax, F.speed);
-          T pfh = shup<kDpp>(F.fh), pft = shup<kDpp>(F.ft), pfn = shup<kDpp>(F.fn - F.sR);
-          T rx_h = div_by(pfh - F.fh, k.ddx);
-          T rx_u = div_by(pfn - (F.fn - F.sL), k.ddx);
-          T rx_v = div_by(pft - F.ft, k.ddx);
           T th = rx_h + ry_h, tu = rx_u + ry_u, tv = rx_v + ry_v;
           if (lane_out) {
             int64_t i = mbase + (int64_t)o * f.nx + col;
