@@ -243,7 +243,11 @@ void adj_L_kernel(AdjArgs<T> a) {
 
   for (int it = 0; it < rows + 6; ++it) {
     const int n = y0 - kHaloA + it;
-    Prim<T> Pn = fetch<T, STAGE, AdjArgs<T>>(a, b, n, col, half);
+    // row n's loads go out first; the x sweep of row n-3 does not need them
+    CellLoads<T> raw = fetch_loads<T, STAGE, AdjArgs<T>>(a, b, n, col);
+    PrimGrad<T> gx{T(0), T(0), T(0), T(0)};
+    if (it >= 6 && n - 3 >= 0 && n - 3 < f.ny) gx = x_sweep_bwd(a, P3, L3);  // x-faces exist on interior rows only
+    Prim<T> Pn = fetch_finish<T, STAGE, AdjArgs<T>>(a, b, raw, half);
     Lam3<T> Ln = STAGE == 1 && a.lam_P != nullptr ? predictor_lam(a, b, n, col) : load_lam(a, b, n, col);
     Incr<T> yi_cur{};
     bool reset_cur = false;
@@ -272,8 +276,6 @@ void adj_L_kernel(AdjArgs<T> a) {
     if (it >= 6) {
       // row n-3 is complete in y; add its x sweep and emit
       const int r = n - 3;
-      PrimGrad<T> gx{T(0), T(0), T(0), T(0)};
-      if (r >= 0 && r < f.ny) gx = x_sweep_bwd(a, P3, L3);  // x-faces exist on interior rows only
       T gh = A3.h + gx.h, geta = A3.eta + gx.eta;
       T gu = A3.ut + gx.un;  // y sweep: un = v, ut = u
       T gv = A3.un + gx.ut;
