@@ -146,6 +146,15 @@ __device__ inline void stage_strip(const StageArgs<T>& a, int strip, int seg, in
     const int r = y0 - kHalo + it;
     // row r's loads go out first; the x sweep of row o = r - 2 does not need them
     CellLoads<T> raw = fetch_loads<T, STAGE>(a, b, r, col);
+    // stage 2: the output row's step-start state and roughness, needed at its update
+    T oh = T(0), ou = T(0), ov = T(0), coef = T(0);
+    if (STAGE == 2 && it >= 4 && lane_out) {
+      const int64_t cell = (int64_t)(r - 2) * f.nx + col;
+      oh = a.U[mbase + cell];
+      ou = a.U[mbase + cell + f.plane];
+      ov = a.U[mbase + cell + 2 * f.plane];
+      if (a.friction) coef = f.rough[(int64_t)b * f.rough_bstride + cell];
+    }
     T rx_h = T(0), rx_u = T(0), rx_v = T(0);
     if (it >= 4) {
       // x sweep of row o (its state is P2)
@@ -195,13 +204,9 @@ __device__ inline void stage_strip(const StageArgs<T>& a, int strip, int seg, in
                 a.k2_out[i + f.plane] = tu;
                 a.k2_out[i + 2 * f.plane] = tv;
               }
-              T oh = a.U[i], ou = a.U[i + f.plane], ov = a.U[i + 2 * f.plane];
               T nh = oh + dt * th, nu = ou + dt * tu, nv = ov + dt * tv;
               clean_and_limit(oh, ou, ov, nh, nu, nv, k, a.positivity != 0);
-              if (a.friction) {
-                T coef = f.rough[(int64_t)b * f.rough_bstride + cell];
-                swe::friction(nh, nu, nv, coef, dt, k, a.friction_model);
-              }
+              if (a.friction) swe::friction(nh, nu, nv, coef, dt, k, a.friction_model);
               a.out[i] = nh;
               a.out[i + f.plane] = nu;
               a.out[i + 2 * f.plane] = nv;
