@@ -137,27 +137,44 @@ __device__ inline void corrector_bwd(const AdjArgs<T>& a, int b, int64_t cell, i
   lam = Lam3<T>{dt * gc_h, dt * gc_u, dt * gc_v};
 }
 
-// predictor clean-up reversed at interior cell i (solver.py:639)
+// predictor clean-up reversed at one interior cell (solver.py:639) from its state o*,
+// stage-1 tendency k* and the predictor-state adjoint g*
 template <class T>
-__device__ inline void predictor_bwd(const AdjArgs<T>& a, int64_t i, Lam3<T>& lam, Lam3<T>& acc) {
-  const int64_t P = a.f.plane;
+__device__ inline void predictor_core(const AdjArgs<T>& a, T oh, T ou, T ov, T kh, T ku, T kv, T gh, T gu, T gv,
+                                      Lam3<T>& lam, Lam3<T>& acc) {
   const T half = a.half_dt;
-  T oh = a.U[i], ou = a.U[i + P], ov = a.U[i + 2 * P];
-  T ch = oh + half * a.k1[i], cu = ou + half * a.k1[i + P], cv = ov + half * a.k1[i + 2 * P];
+  T ch = oh + half * kh, cu = ou + half * ku, cv = ov + half * kv;
   T go_h = 0, go_u = 0, go_v = 0, gc_h, gc_u, gc_v;
-  clean_and_limit_bwd(oh, ou, ov, ch, cu, cv, a.k, a.positivity != 0, a.lam_P[i], a.lam_P[i + P], a.lam_P[i + 2 * P],
-                      go_h, go_u, go_v, gc_h, gc_u, gc_v);
+  clean_and_limit_bwd(oh, ou, ov, ch, cu, cv, a.k, a.positivity != 0, gh, gu, gv, go_h, go_u, go_v, gc_h, gc_u, gc_v);
   acc = Lam3<T>{go_h + gc_h, go_u + gc_u, go_v + gc_v};
   lam = Lam3<T>{half * gc_h, half * gc_u, half * gc_v};
 }
 
-// adjoint of k1 at padded cell (r, c): the predictor clean-up reversed on interior
-// cells, 0 outside the domain (the stage tendency exists on interior cells only)
-template <class T> __device__ inline Lam3<T> predictor_lam(const AdjArgs<T>& a, int b, int r, int c) {
-  if (r < 0 || r >= a.f.ny || c < 0 || c >= a.f.nx) return Lam3<T>{T(0), T(0), T(0)};
-  Lam3<T> lam, acc;
-  predictor_bwd(a, (int64_t)b * 3 * a.f.plane + (int64_t)r * a.f.nx + c, lam, acc);
-  return lam;
+// the same at interior cell i
+template <class T>
+__device__ inline void predictor_bwd(const AdjArgs<T>& a, int64_t i, Lam3<T>& lam, Lam3<T>& acc) {
+  const int64_t P = a.f.plane;
+  predictor_core(a, a.U[i], a.U[i + P], a.U[i + 2 * P], a.k1[i], a.k1[i + P], a.k1[i + 2 * P], a.lam_P[i],
+                 a.lam_P[i + P], a.lam_P[i + 2 * P], lam, acc);
+}
+
+// loads of the predictor reverse at padded cell (r, c), issued ahead of their use
+template <class T> struct PredLoads {
+  T kh, ku, kv, gh, gu, gv;
+  bool in;  // interior cell (the stage tendency exists on interior cells only)
+};
+template <class T> __device__ inline PredLoads<T> predictor_loads(const AdjArgs<T>& a, int b, int r, int c) {
+  PredLoads<T> p{T(0), T(0), T(0), T(0), T(0), T(0), r >= 0 && r < a.f.ny && c >= 0 && c < a.f.nx};
+  if (p.in) {
+    const int64_t P = a.f.plane, i = (int64_t)b * 3 * P + (int64_t)r * a.f.nx + c;
+    p.kh = a.k1[i];
+    p.ku = a.k1[i + P];
+    p.kv = a.k1[i + 2 * P];
+    p.gh = a.lam_P[i];
+    p.gu = a.lam_P[i + P];
+    p.gv = a.lam_P[i + 2 * P];
+  }
+  return p;
 }
 
 // one thread per interior cell: lam_k2 = adjoint of k2, acc = adjoint reaching U
@@ -245,10 +262,23 @@ void adj_L_kernel(AdjArgs<T> a) {
     const int n = y0 - kHaloA + it;
     // row n's loads go out first; the x sweep of row n-3 does not need them
     CellLoads<T> raw = fetch_loads<T, STAGE, AdjArgs<T>>(a, b, n, col);
+    const bool fused_pred = STAGE == 1 && a.lam_P != nullptr;
+    PredLoads<T> pl{};
+    if (fused_pred) pl = predictor_loads(a, b, n, col);
     PrimGrad<T> gx{T(0), T(0), T(0), T(0)};
     if (it >= 6 && n - 3 >= 0 && n - 3 < f.ny) gx = x_sweep_bwd(a, P3, L3);  // x-faces exist on interior rows only
     Prim<T> Pn = fetch_finish<T, STAGE, AdjArgs<T>>(a, b, raw, half);
-    Lam3<T> Ln = STAGE == 1 && a.lam_P != nullptr ? predictor_lam(a, b, n, col) : load_lam(a, b, n, col);
+    Lam3<T> Ln{T(0), T(0), T(0)};
+    if (fused_pred) {
+      // lam of row n: the predictor clean-up reversed (on interior cells the stage-1
+      // fetch loaded the cell's own state)
+      if (pl.in) {
+        Lam3<T> pacc;
+        predictor_core(a, raw.h, raw.hu, raw.hv, pl.kh, pl.ku, pl.kv, pl.gh, pl.gu, pl.gv, Ln, pacc);
+      }
+    } else {
+      Ln = load_lam(a, b, n, col);
+    }
     Incr<T> yi_cur{};
     bool reset_cur = false;
     PrimGrad<T> An{T(0), T(0), T(0), T(0)};
