@@ -74,14 +74,6 @@ template <class T> __device__ inline Lam3<T> load_lam(const AdjArgs<T>& a, int b
 template <class T> __device__ inline SideGrad<T> shupg(const SideGrad<T>& g) {
   return SideGrad<T>{shup(g.hlin), shup(g.eta), shup(g.un), shup(g.ut), shup(g.hcell)};
 }
-template <class T> __device__ inline Side<T> shdns(const Side<T>& s) { return shdn(s); }
-
-template <class T> __device__ inline void addpg(PrimGrad<T>& a, const PrimGrad<T>& b) {
-  a.h += b.h;
-  a.eta += b.eta;
-  a.un += b.un;
-  a.ut += b.ut;
-}
 
 // ghost buffer slot of padded cell (r, c), or -1 for interior / corner / far cells
 __device__ inline int64_t ghost_slot(int b, int r, int c, int ny, int nx, int lmax) {
