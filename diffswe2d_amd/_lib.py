@@ -14,7 +14,7 @@ import torch
 HERE = Path(__file__).resolve().parent
 LIB_PATH = HERE / "libswe2d_hip.so"
 
-SWE_ABI_VERSION = 1
+SWE_ABI_VERSION = 2
 BC_CODES = {"transmissive": 0, "periodic": 1, "constant": 2, "water_level": 3, "wall": 4, "absorbing": 5}
 FRICTION_CODES = {"manning": 0}
 
@@ -33,7 +33,8 @@ class SweConfigC(ctypes.Structure):
                 ("cfl", ctypes.c_double), ("limiter_theta", ctypes.c_double),
                 ("constant_value", ctypes.c_double), ("boundary", ctypes.c_int32 * 4),
                 ("water_slope_reset", ctypes.c_int32), ("positivity_limiter", ctypes.c_int32),
-                ("apply_friction", ctypes.c_int32), ("friction_model", ctypes.c_int32)]
+                ("apply_friction", ctypes.c_int32), ("friction_model", ctypes.c_int32),
+                ("grad_bed", ctypes.c_int32), ("pad", ctypes.c_int32)]
 
 
 class SweScenario(ctypes.Structure):
@@ -95,6 +96,7 @@ _SIGNATURES = {
     "swe_advance_tape_f32": ([_G, _C, _P, _P, _P, _I, _P, _P, _P, _I, _I, _I, _P, _P, _I, _P, _I, _P], _I),
     "swe_advance_tape_f64": ([_G, _C, _P, _P, _P, _I, _P, _P, _P, _I, _I, _I, _P, _P, _I, _P, _I, _P], _I),
     "swe_adjoint_workspace_bytes": ([_G, _I], ctypes.c_size_t),
+    "swe_adjoint_bed_grad_offset": ([_G, _I], ctypes.c_size_t),
     "swe_adjoint_last_error_message": ([], ctypes.c_char_p),
     "swe_operator_adjoint_f32": ([_G, _C, _P, _P, _P, _I, _P, _P, _P, _P, _I, _P], _I),
     "swe_operator_adjoint_f64": ([_G, _C, _P, _P, _P, _I, _P, _P, _P, _P, _I, _P], _I),

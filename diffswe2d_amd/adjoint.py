@@ -18,6 +18,7 @@ correction) and roughness are treated as constants.
 """
 from __future__ import annotations
 
+import contextlib
 import ctypes
 import warnings
 
@@ -93,13 +94,55 @@ def step_adjoint(model, U, k1, k2, level, dt, lam_out, lam_level):
 
 
 def _warn_constant_bed(model):
-    """The adjoint treats the bed as a constant: say so once when the DEM correction is trainable."""
+    """The chunked device loop treats the bed as a constant: say so once when the DEM correction is trainable."""
     p = getattr(model, "dem_correction_parameter", None)
     if p is not None and p.requires_grad and not getattr(model, "_warned_constant_bed", False):
-        warnings.warn("the HIP adjoint differentiates with respect to the state and the boundary level only; "
-                      "dem_correction_parameter receives no gradient (construct the model with train_dem=False "
-                      "to silence this)", stacklevel=3)
+        warnings.warn("the chunked device loop differentiates with respect to the state and the boundary levels "
+                      "only; dem_correction_parameter receives no gradient there (construct the model with "
+                      "train_dem=False to silence this)", stacklevel=3)
         model._warned_constant_bed = True
+
+
+def _bed_input(model):
+    """The differentiable bed (SWE2D.bed, src/solver.py:134-147) when the DEM correction is trainable, else None."""
+    p = getattr(model, "dem_correction_parameter", None)
+    return model.bed if p is not None and p.requires_grad else None
+
+
+def _bed_region(model, U):
+    """The adjoint workspace's bed adjoint [B, 1, ny, nx] (accumulated by adjoint calls made with grad_bed set)."""
+    lib = _lib.load()
+    aws = _adj_ws(model, U)
+    B, _, ny, nx = U.shape
+    off = lib.swe_adjoint_bed_grad_offset(aws["grid"], U.element_size())
+    n = B * ny * nx * U.element_size()
+    return aws["buf"][off:off + n].view(U.dtype).view(B, 1, ny, nx)
+
+
+@contextlib.contextmanager
+def _bed_gradient(model, U, wanted):
+    """Clear the bed adjoint and let the adjoint calls inside accumulate into it."""
+    if not wanted:
+        yield None
+        return
+    region = _bed_region(model, U)
+    region.zero_()
+    model._grad_bed = True
+    try:
+        yield region
+    finally:
+        model._grad_bed = False
+
+
+def _bed_grad(region, bed_meta):
+    """The bed adjoint reduced to the bed's shape (summed over members sharing one bed)."""
+    shape, dtype, device = bed_meta
+    g = region.sum(dim=0, keepdim=True) if shape[0] == 1 and region.shape[0] > 1 else region
+    return g.reshape(shape).to(device=device, dtype=dtype).clone()
+
+
+def _meta(bed):
+    return None if bed is None else (tuple(bed.shape), bed.dtype, bed.device)
 
 
 def _level_grad(level, lam_level):
@@ -111,9 +154,10 @@ def _level_grad(level, lam_level):
 
 class _StepFn(torch.autograd.Function):
     @staticmethod
-    def forward(ctx, U, level, model, max_dt, forcing):
+    def forward(ctx, U, level, bed, model, max_dt, forcing):
         U_ = U.detach().contiguous()
         lvl = level.detach()
+        ctx.bed_meta = _meta(bed)
         # source terms enter k1 and k2 only; their adjoint with respect to U is zero
         out, k1, k2, dt = record_step(model, U_, lvl, max_dt, forcing=forcing)
         if not (dt == dt and abs(dt) != float("inf")):
@@ -130,15 +174,17 @@ class _StepFn(torch.autograd.Function):
     def backward(ctx, g_out, _g_dt):
         U, k1, k2, lvl = ctx.saved_tensors
         lam_level = torch.zeros(U.shape[0], dtype=U.dtype, device=U.device)
-        lam_U = step_adjoint(ctx.model, U, k1, k2, lvl, ctx.dt, g_out.contiguous(), lam_level)
+        want_bed = ctx.needs_input_grad[2]
+        with _bed_gradient(ctx.model, U, want_bed) as region:
+            lam_U = step_adjoint(ctx.model, U, k1, k2, lvl, ctx.dt, g_out.contiguous(), lam_level)
         g_level = _level_grad(lvl, lam_level) if ctx.needs_input_grad[1] else None
-        return lam_U, g_level, None, None, None
+        g_bed = _bed_grad(region, ctx.bed_meta) if want_bed else None
+        return lam_U, g_level, g_bed, None, None, None
 
 
 def differentiable_step(model, U, level, max_dt, forcing=None):
-    """SWE2D.step with gradients to U and the level; returns (U_new, float dt)."""
-    _warn_constant_bed(model)
-    out, dt = _StepFn.apply(U, level, model, max_dt, forcing)
+    """SWE2D.step with gradients to U, the level and (a trainable DEM correction) the bed; returns (U_new, dt)."""
+    out, dt = _StepFn.apply(U, level, _bed_input(model), model, max_dt, forcing)
     return out, float(dt)
 
 
@@ -242,9 +288,10 @@ def _checkpointed_run(model, U0, level, duration, max_steps, cap, K, budget=0):
 
 class _ForwardFn(torch.autograd.Function):
     @staticmethod
-    def forward(ctx, U0, level, model, duration, max_steps, cap):
+    def forward(ctx, U0, level, bed, model, duration, max_steps, cap):
         U0_ = U0.detach().contiguous()
         lvl = level.detach()
+        ctx.bed_meta = _meta(bed)
         K = int(getattr(model, "adjoint_checkpoint_every", 64))
         if K < 1:
             raise ValueError("adjoint_checkpoint_every must be at least 1")
@@ -267,14 +314,22 @@ class _ForwardFn(torch.autograd.Function):
         model, dts, segments = ctx.model, ctx.dts, ctx.segments
         lam = g_U.contiguous().clone()
         lam_level = torch.zeros(lam.shape[0], dtype=lam.dtype, device=lam.device)
+        want_bed = ctx.needs_input_grad[2]
+        region = None
         if segments:
-            if getattr(model, "adjoint_verify_replay", False):
-                lam = _replay_checked(model, segments, lvl, dts, lam, lam_level)
-            else:
-                lam = _reverse_segments(model, segments, lvl, dts, lam, lam_level)
+            with _bed_gradient(model, lam, want_bed) as region:
+                if getattr(model, "adjoint_verify_replay", False):
+                    lam = _replay_checked(model, segments, lvl, dts, lam, lam_level)
+                else:
+                    lam = _reverse_segments(model, segments, lvl, dts, lam, lam_level)
         ctx.segments = None
         g_level = _level_grad(lvl, lam_level) if ctx.needs_input_grad[1] else None
-        return lam, g_level, None, None, None, None
+        g_bed = None
+        if want_bed:
+            meta = ctx.bed_meta
+            g_bed = _bed_grad(region, meta) if region is not None else torch.zeros(meta[0], dtype=meta[1],
+                                                                                  device=meta[2])
+        return lam, g_level, g_bed, None, None, None, None
 
 
 def _reverse_segments(model, segments, lvl, dts, lam, lam_level):
@@ -331,22 +386,23 @@ def _replay_checked(model, segments, lvl, dts, lam, lam_level):
 
 
 def differentiable_forward(model, U0, duration, absolute_start, max_steps, boundary_level, cap):
-    """SWE2D.forward with gradients to U0 and an explicit boundary level."""
-    _warn_constant_bed(model)
+    """SWE2D.forward with gradients to U0, an explicit boundary level and (a trainable DEM correction) the bed."""
     if boundary_level is None and model.use_dynamic_bc:
         raise NotImplementedError("gradients through a dynamic boundary series are not supported by the adjoint")
     level = model._prepare_boundary_level(
         model.cfg.constant_level if boundary_level is None else boundary_level, U0)
     model._check_state(U0, "U0")
-    return _ForwardFn.apply(U0, level, model, float(duration), int(max_steps), cap)
+    return _ForwardFn.apply(U0, level, _bed_input(model), model, float(duration), int(max_steps), cap)
 
 
 class _OperatorFn(torch.autograd.Function):
     @staticmethod
-    def forward(ctx, U, level, model):
+    def forward(ctx, U, level, bed, model):
         U_ = U.detach().contiguous()
         lvl = level.detach()
-        rhs = model.spatial_operator(U_, boundary_level=lvl)
+        ctx.bed_meta = _meta(bed)
+        with torch.no_grad():
+            rhs = model.spatial_operator(U_, boundary_level=lvl)
         ctx.model = model
         ctx.save_for_backward(U_, lvl)
         return rhs
@@ -362,15 +418,19 @@ class _OperatorFn(torch.autograd.Function):
         l, stride = model._level_arg(lvl, U)
         lam_U = torch.empty_like(U)
         lam_level = torch.zeros(U.shape[0], dtype=U.dtype, device=U.device)
-        _lib.check(_fn(lib, "swe_operator_adjoint", U.dtype)(
-            ws["grid"], ws["cfg"], _lib.ptr(U), _lib.ptr(bed), _lib.ptr(l), stride, _lib.ptr(g.contiguous()),
-            _lib.ptr(lam_U), _lib.ptr(lam_level), _lib.ptr(aws["buf"]), int(model.rows_hint),
-            _lib.stream_ptr(U.device)), "operator_adjoint", _ADJ_MSG)
+        want_bed = ctx.needs_input_grad[2]
+        with _bed_gradient(model, U, want_bed) as region:
+            cfg = _lib.make_config(model.cfg)
+            cfg.grad_bed = int(want_bed)
+            _lib.check(_fn(lib, "swe_operator_adjoint", U.dtype)(
+                ws["grid"], cfg, _lib.ptr(U), _lib.ptr(bed), _lib.ptr(l), stride, _lib.ptr(g.contiguous()),
+                _lib.ptr(lam_U), _lib.ptr(lam_level), _lib.ptr(aws["buf"]), int(model.rows_hint),
+                _lib.stream_ptr(U.device)), "operator_adjoint", _ADJ_MSG)
         g_level = _level_grad(lvl, lam_level) if ctx.needs_input_grad[1] else None
-        return lam_U, g_level, None
+        g_bed = _bed_grad(region, ctx.bed_meta) if want_bed else None
+        return lam_U, g_level, g_bed, None
 
 
 def operator_adjoint(model, U, level):
-    """spatial_operator(U, boundary_level=level) with gradients to U and level."""
-    _warn_constant_bed(model)
-    return _OperatorFn.apply(U, level, model)
+    """spatial_operator(U, boundary_level=level) with gradients to U, the level and (a trainable DEM) the bed."""
+    return _OperatorFn.apply(U, level, _bed_input(model), model)

@@ -59,6 +59,7 @@ template <class T> struct AdjArgs {
   T* acc;                // adjoint reaching U directly past the corrector (written by the corrector)
   T* part;               // per-block level-gradient partials of the ghost fold
   int part_blocks;       // blocks per side in the ghost fold
+  T* bedgrad;            // adjoint of the bed, [B, ny, nx], accumulated (nullable)
 };
 
 template <class T> struct Lam3 {
@@ -72,8 +73,11 @@ template <class T> __device__ inline Lam3<T> load_lam(const AdjArgs<T>& a, int b
 }
 
 template <class T> __device__ inline SideGrad<T> shupg(const SideGrad<T>& g) {
-  return SideGrad<T>{shup(g.hlin), shup(g.eta), shup(g.un), shup(g.ut), shup(g.hcell)};
+  return SideGrad<T>{shup(g.hlin), shup(g.eta), shup(g.un), shup(g.ut), shup(g.hcell), shup(g.z), shup(g.zcell)};
 }
+
+// values per ghost slot: the adjoints of the ghost cell's h, hu, hv and bed
+constexpr int kGhostVals = 4;
 
 // ghost buffer slot of padded cell (r, c), or -1 for interior / corner / far cells
 __device__ inline int64_t ghost_slot(int b, int r, int c, int ny, int nx, int lmax) {
@@ -104,7 +108,7 @@ __device__ inline int64_t ghost_slot(int b, int r, int c, int ny, int nx, int lm
   } else {
     return -1;
   }
-  return ((((int64_t)b * 4 + side) * 2 + layer) * lmax + i) * 3;
+  return ((((int64_t)b * 4 + side) * 2 + layer) * lmax + i) * kGhostVals;
 }
 
 // friction and the corrector clean-up reversed at interior cell i (solver.py:646-661):
@@ -209,13 +213,14 @@ __device__ inline PrimGrad<T> x_sweep_bwd(const AdjArgs<T>& a, const Prim<T>& P,
   face_flux_bwd(plus, cx.h, cx.z, minus_next, cp.h, cp.z, k, gfh, gfn, gft, gsL, gsR, gL, gR);
   SideGrad<T> gm = shupg(gR);  // this cell as the right state of face (c-1 | c)
   CellGrad<T> cg = sides_bwd(cx, xi, reset, gL, gm, k);
-  PrimGrad<T> own{cg.h, cg.eta, cg.un, cg.ut};
-  PrimGrad<T> to_m{T(0), T(0), T(0), T(0)}, to_p{T(0), T(0), T(0), T(0)};
+  PrimGrad<T> own{cg.h, cg.eta, cg.un, cg.ut, cg.z};
+  PrimGrad<T> to_m{}, to_p{};
   increments_bwd(cm, cx, cp, cg, k, to_m, own, to_p);
   own.h += shdn(to_m.h) + shup(to_p.h);
   own.eta += shdn(to_m.eta) + shup(to_p.eta);
   own.un += shdn(to_m.un) + shup(to_p.un);
   own.ut += shdn(to_m.ut) + shup(to_p.ut);
+  own.z += shdn(to_m.z) + shup(to_p.z);
   return own;
 }
 
@@ -224,7 +229,9 @@ __device__ inline PrimGrad<T> x_sweep_bwd(const AdjArgs<T>& a, const Prim<T>& P,
 // latency the third wave hides.  FP64 keeps its natural allocation.
 template <class T> constexpr int adj_waves_per_eu() { return sizeof(T) == 4 ? 3 : 1; }
 
-template <class T, int STAGE>
+// BED: also accumulate the adjoint of the bed into a.bedgrad (interior cells) and
+// the ghost slots' fourth value (ghost cells).
+template <class T, int STAGE, bool BED>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(adj_waves_per_eu<T>())))
 void adj_L_kernel(AdjArgs<T> a) {
   const int lane = threadIdx.x & (kWave - 1);
@@ -257,7 +264,7 @@ void adj_L_kernel(AdjArgs<T> a) {
     const bool fused_pred = STAGE == 1 && a.lam_P != nullptr;
     PredLoads<T> pl{};
     if (fused_pred) pl = predictor_loads(a, b, n, col);
-    PrimGrad<T> gx{T(0), T(0), T(0), T(0)};
+    PrimGrad<T> gx{};
     if (it >= 6 && n - 3 >= 0 && n - 3 < f.ny) gx = x_sweep_bwd(a, P3, L3);  // x-faces exist on interior rows only
     Prim<T> Pn = fetch_finish<T, STAGE, AdjArgs<T>>(a, b, raw, half);
     Lam3<T> Ln{T(0), T(0), T(0)};
@@ -273,7 +280,7 @@ void adj_L_kernel(AdjArgs<T> a) {
     }
     Incr<T> yi_cur{};
     bool reset_cur = false;
-    PrimGrad<T> An{T(0), T(0), T(0), T(0)};
+    PrimGrad<T> An{};
     if (it >= 2) yi_cur = increments_r(P2.y(), P1.y(), Pn.y(), k, a.wsr != 0, reset_cur);  // row n-1
     if (it >= 3) {
       // face (n-2 | n-1): normal momentum hv, tangential hu
@@ -292,6 +299,7 @@ void adj_L_kernel(AdjArgs<T> a) {
       A2.eta += cg.eta;
       A2.un += cg.un;
       A2.ut += cg.ut;
+      A2.z += cg.z;
       increments_bwd(P3.y(), P2.y(), P1.y(), cg, k, A3, A2, A1);
       gm_pend = gR;
     }
@@ -303,6 +311,7 @@ void adj_L_kernel(AdjArgs<T> a) {
       T gv = A3.un + gx.ut;
       T lh, lu, lv;
       primitive_bwd(P3.hr, P3.qu, P3.qv, k, gh, gu, gv, geta, lh, lu, lv);
+      const T gz = (A3.z + gx.z) + geta;  // eta = h + z
       if (lane_out) {
         if (r >= 0 && r < f.ny && col >= 0 && col < f.nx) {
           int64_t i = (int64_t)b * 3 * f.plane + (int64_t)r * f.nx + col;
@@ -323,12 +332,14 @@ void adj_L_kernel(AdjArgs<T> a) {
           a.out[i] = lh;
           a.out[i + f.plane] = lu;
           a.out[i + 2 * f.plane] = lv;
+          if (BED) a.bedgrad[(int64_t)b * f.plane + (int64_t)r * f.nx + col] += gz;
         } else {
           int64_t gs = ghost_slot(b, r, col, f.ny, f.nx, a.lmax);
           if (gs >= 0) {
             a.ghost[gs] = lh;
             a.ghost[gs + 1] = lu;
             a.ghost[gs + 2] = lv;
+            a.ghost[gs + 3] = BED ? gz : T(0);
           }
         }
       }
@@ -365,7 +376,7 @@ __device__ inline T ghost_fold_at(const AdjArgs<T>& a, int stage, int side, int 
   const Fields<T>& f = a.f;
   const int64_t base = (int64_t)b * 3 * f.plane;
   GhostMap g[2];
-  T gh[2], gu[2], gv[2];
+  T gh[2], gu[2], gv[2], gz[2];
   for (int layer = 0; layer < 2; ++layer) {
     int r, c;
     if (side == LEFT) { r = i; c = -1 - layer; }
@@ -373,10 +384,11 @@ __device__ inline T ghost_fold_at(const AdjArgs<T>& a, int stage, int side, int 
     else if (side == TOP) { r = -1 - layer; c = i; }
     else { r = f.ny + layer; c = i; }
     g[layer] = ghost_map(r, c, f.ny, f.nx, f.bc);
-    int64_t gs = ((((int64_t)b * 4 + side) * 2 + layer) * a.lmax + i) * 3;
+    int64_t gs = ((((int64_t)b * 4 + side) * 2 + layer) * a.lmax + i) * kGhostVals;
     gh[layer] = a.ghost[gs];
     gu[layer] = a.ghost[gs + 1];
     gv[layer] = a.ghost[gs + 2];
+    gz[layer] = a.ghost[gs + 3];
   }
   const int kind = g[0].kind;  // both layers belong to the same side
   const bool uses_level = kind == BC_WATER_LEVEL || kind == BC_ABSORBING;
@@ -400,6 +412,11 @@ __device__ inline T ghost_fold_at(const AdjArgs<T>& a, int stage, int side, int 
       int64_t mm = base + (int64_t)m.mr * f.nx + m.mc;
       if (side == LEFT || side == RIGHT) a.out[mm + f.plane] += gmir;
       else a.out[mm + 2 * f.plane] += gmir;
+    }
+    if (a.bedgrad != nullptr) {
+      // a ghost cell's bed is its source cell's; the water-level and absorbing
+      // rules read it through level - z, so it takes minus the level adjoint too
+      a.bedgrad[(int64_t)b * f.plane + (int64_t)m.r * f.nx + m.c] += gz[layer] - glev;
     }
     glev_sum += glev;
   }
@@ -488,6 +505,7 @@ constexpr int kGhostOneBlock = 2048;  // longest side folded by one block per me
 
 template <class T> struct AdjWs {
   T *lam_acc, *lam_P, *lam_k, *ghost, *part;  // lam_k holds lam_k2
+  T* bedgrad;                                 // [B, ny, nx], accumulated over calls
   int lmax, part_blocks;
 };
 
@@ -496,21 +514,27 @@ inline int ghost_blocks(const swe_grid* g) {
   return (lmax + kGhostThreads - 1) / kGhostThreads;
 }
 
-// [lam_acc | lam_P | lam_k | ghost adjoints | level partials], each 256-byte aligned
-template <class T> size_t adj_ws_bytes(const swe_grid* g) {
+// [lam_acc | lam_P | lam_k | ghost adjoints | level partials | bed adjoint],
+// each 256-byte aligned
+template <class T> size_t adj_bed_offset(const swe_grid* g) {
   size_t state = (size_t)g->batch * 3 * (size_t)g->ny * g->nx * sizeof(T);
   int lmax = g->ny > g->nx ? g->ny : g->nx;
-  size_t ghost = (size_t)g->batch * 4 * 2 * lmax * 3 * sizeof(T);
+  size_t ghost = (size_t)g->batch * 4 * 2 * lmax * kGhostVals * sizeof(T);
   size_t part = (size_t)g->batch * 4 * ghost_blocks(g) * sizeof(T);
   return 3 * al256(state) + al256(ghost) + al256(part);
+}
+
+template <class T> size_t adj_ws_bytes(const swe_grid* g) {
+  return adj_bed_offset<T>(g) + al256((size_t)g->batch * (size_t)g->ny * g->nx * sizeof(T));
 }
 
 template <class T> AdjWs<T> adj_ws(void* ws, const swe_grid* g) {
   size_t state = al256((size_t)g->batch * 3 * (size_t)g->ny * g->nx * sizeof(T));
   int lmax = g->ny > g->nx ? g->ny : g->nx;
-  size_t ghost = al256((size_t)g->batch * 4 * 2 * lmax * 3 * sizeof(T));
+  size_t ghost = al256((size_t)g->batch * 4 * 2 * lmax * kGhostVals * sizeof(T));
   char* p = (char*)ws;
   AdjWs<T> w;
+  w.bedgrad = (T*)(p + adj_bed_offset<T>(g));
   w.lam_acc = (T*)p;
   w.lam_P = (T*)(p + state);
   w.lam_k = (T*)(p + 2 * state);
@@ -519,6 +543,14 @@ template <class T> AdjWs<T> adj_ws(void* ws, const swe_grid* g) {
   w.lmax = lmax;
   w.part_blocks = ghost_blocks(g);
   return w;
+}
+
+// L^T of one stage, with the bed adjoint when the call accumulates it
+template <class T, int STAGE> void launch_adj_L(const AdjArgs<T>& a, dim3 grid, dim3 block, hipStream_t s) {
+  if (a.bedgrad != nullptr)
+    hipLaunchKernelGGL((adj_L_kernel<T, STAGE, true>), grid, block, 0, s, a);
+  else
+    hipLaunchKernelGGL((adj_L_kernel<T, STAGE, false>), grid, block, 0, s, a);
 }
 
 // the ghost fold of one L^T sweep: four side launches, then the level reduction
@@ -573,6 +605,7 @@ int step_adjoint_impl(const swe_grid* g, const swe_config* c, const T* U, const 
   a.f.rough = rough;
   a.f.level = level;
   a.f.level_stride = level_stride;
+  a.bedgrad = c->grad_bed ? w.bedgrad : nullptr;
   a.S = adj_rows(g, rows_hint);
   a.strips = (g->nx + 4 + kOutA - 1) / kOutA;
   a.segs = (g->ny + 4 + a.S - 1) / a.S;
@@ -605,7 +638,7 @@ int step_adjoint_impl(const swe_grid* g, const swe_config* c, const T* U, const 
   a2.lam = w.lam_k;
   a2.base = nullptr;
   a2.out = w.lam_P;
-  hipLaunchKernelGGL((adj_L_kernel<T, 2>), lg, lb, 0, s, a2);
+  launch_adj_L<T, 2>(a2, lg, lb, s);
   ghost_fold(a2, 2, g->batch, lam_level, s);
   // stage 1: predictor clean-up reversed inside L^T at U (rows of lam formed from
   // lam_P, k1 and U as the sweep reaches them), plus the corrector's direct part
@@ -614,7 +647,7 @@ int step_adjoint_impl(const swe_grid* g, const swe_config* c, const T* U, const 
   a1.lam = nullptr;
   a1.base = w.lam_acc;
   a1.out = lam_U;
-  hipLaunchKernelGGL((adj_L_kernel<T, 1>), lg, lb, 0, s, a1);
+  launch_adj_L<T, 1>(a1, lg, lb, s);
   ghost_fold(a1, 1, g->batch, lam_level, s);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return afail(SWE_ERR_HIP, hipGetErrorString(e));
@@ -643,6 +676,7 @@ int operator_adjoint_impl(const swe_grid* g, const swe_config* c, const T* U, co
   a.f.rough = bed;
   a.f.level = level;
   a.f.level_stride = level_stride;
+  a.bedgrad = c->grad_bed ? w.bedgrad : nullptr;
   a.S = adj_rows(g, rows_hint);
   a.strips = (g->nx + 4 + kOutA - 1) / kOutA;
   a.segs = (g->ny + 4 + a.S - 1) / a.S;
@@ -658,7 +692,7 @@ int operator_adjoint_impl(const swe_grid* g, const swe_config* c, const T* U, co
   a.part_blocks = w.part_blocks;
   long long waves = (long long)a.strips * a.segs;
   dim3 lg((unsigned)((waves + kWavesPerBlock - 1) / kWavesPerBlock), (unsigned)g->batch);
-  hipLaunchKernelGGL((adj_L_kernel<T, 1>), lg, dim3(kWave * kWavesPerBlock), 0, s, a);
+  launch_adj_L<T, 1>(a, lg, dim3(kWave * kWavesPerBlock), s);
   ghost_fold(a, 1, g->batch, lam_level, s);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return afail(SWE_ERR_HIP, hipGetErrorString(e));
@@ -827,6 +861,11 @@ size_t swe_adjoint_workspace_bytes(const swe_grid* g, int dtype_bytes) {
 }
 
 const char* swe_adjoint_last_error_message(void) { return g_msg; }
+
+size_t swe_adjoint_bed_grad_offset(const swe_grid* g, int dtype_bytes) {
+  if (!g || g->batch <= 0 || g->ny < 2 || g->nx < 2) return 0;
+  return dtype_bytes == 8 ? adj_bed_offset<double>(g) : adj_bed_offset<float>(g);
+}
 
 int swe_operator_adjoint_f32(const swe_grid* g, const swe_config* c, const float* U, const float* bed,
                              const float* level, int level_stride, const float* lam_rhs, float* lam_U,

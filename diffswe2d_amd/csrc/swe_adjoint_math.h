@@ -8,7 +8,8 @@
 //   clamp             gradient passes on the inclusive side (x >= lo, x <= hi)
 //   where             the selected branch only
 //   sign              zero derivative (minmod's magnitude is |a|-based)
-//   bed terms         the bed is a constant (not differentiated)
+// The z components are the bed's adjoints; kernels that do not differentiate
+// the bed leave them unused.
 #pragma once
 
 #include "swe_math.h"
@@ -58,6 +59,7 @@ SWE_HD void limited_increment_bwd(T qm, T q, T qp, T theta, T g, T& gqm, T& gq, 
 // Adjoint of the reconstructed values one face takes from one cell.
 template <class T> struct SideGrad {
   T hlin, eta, un, ut, hcell;
+  T z, zcell;  // the face value of the bed and the cell bed (source term)
 };
 
 // Adjoints of one cell's increments, from both its faces (plus = its +1 face,
@@ -67,6 +69,7 @@ template <class T> struct SideGrad {
 template <class T> struct CellGrad {
   T h, eta, un, ut;        // direct adjoints of the cell's primitive values
   T dh, deta, dun, dut;    // adjoints of the raw limited increments
+  T z, dz;                 // the same for the bed
 };
 
 template <class T>
@@ -74,6 +77,8 @@ SWE_HD CellGrad<T> sides_bwd(const Cell<T>& c, const Incr<T>& d, bool reset, con
                              const SideGrad<T>& gm, const SweConsts<T>& k) {
   CellGrad<T> r;
   r.h = gp.hlin + gm.hlin + gp.hcell + gm.hcell;
+  r.z = gp.z + gm.z + gp.zcell + gm.zcell;
+  T gdz = T(0.5) * gp.z - T(0.5) * gm.z;
   r.eta = gp.eta + gm.eta;
   r.un = gp.un + gm.un;
   r.ut = gp.ut + gm.ut;
@@ -98,11 +103,13 @@ SWE_HD CellGrad<T> sides_bwd(const Cell<T>& c, const Incr<T>& d, bool reset, con
   // water-slope reset: deta' = reset ? dh + dz : deta
   if (reset) {
     gdh += gdeta;
+    gdz += gdeta;
     r.deta = T(0);
   } else {
     r.deta = gdeta;
   }
   r.dh = gdh;
+  r.dz = gdz;
   return r;
 }
 
@@ -132,7 +139,7 @@ SWE_HD Incr<T> increments_r(const Cell<T>& m, const Cell<T>& c, const Cell<T>& p
 // Spread the adjoints of a cell's raw increments onto the primitive values of
 // the cell and its two neighbours (gm: cell - 1, gc: cell, gp: cell + 1).
 template <class T> struct PrimGrad {
-  T h, eta, un, ut;
+  T h, eta, un, ut, z;
 };
 
 template <class T>
@@ -142,6 +149,7 @@ SWE_HD void increments_bwd(const Cell<T>& m, const Cell<T>& c, const Cell<T>& p,
   limited_increment_bwd(m.eta, c.eta, p.eta, k.theta, g.deta, gm.eta, gc.eta, gp.eta);
   limited_increment_bwd(m.un, c.un, p.un, k.theta, g.dun, gm.un, gc.un, gp.un);
   limited_increment_bwd(m.ut, c.ut, p.ut, k.theta, g.dut, gm.ut, gc.ut, gp.ut);
+  limited_increment_bwd(m.z, c.z, p.z, k.theta, g.dz, gm.z, gc.z, gp.z);
 }
 
 // ---------------------------------------------------------------- primitive variables
@@ -330,6 +338,11 @@ SWE_HD void face_flux_bwd(const Side<T>& L, T hL_cell, T zL_cell, const Side<T>&
   T gemin = g_zf * wmin(emin, zmax);
   gLeta += gemin * wmin(L.eta, R.eta);
   gReta += gemin * wmin(R.eta, L.eta);
+  T gzmax = g_zf * wmin(zmax, emin);
+  gL.z = gzmax * wmax(L.z, R.z);
+  gR.z = gzmax * wmax(R.z, L.z);
+  gL.zcell = gBL;  // s_L = half_g (h_L + h_cL) (z_L - z_face)
+  gR.zcell = gBR;
   gL.hlin = gLh;
   gL.eta = gLeta;
   gL.un = g_Lun;

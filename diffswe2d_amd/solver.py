@@ -251,6 +251,10 @@ class SWE2D(nn.Module):
                 "U and bed must have identical spatial dimensions. "
                 f"U has {tuple(U.shape[-2:])}; bed has {tuple(self.bed_reference.shape[-2:])}.")
 
+    def _bed_trainable(self):
+        """The DEM correction takes gradients (train_dem=True): the differentiable paths then carry the bed."""
+        return bool(self.dem_correction_parameter.requires_grad)
+
     def _fields(self, U):
         """Contiguous bed and roughness on U's device/dtype (cached between calls)."""
         p = self.dem_correction_parameter
@@ -293,6 +297,7 @@ class SWE2D(nn.Module):
             ws = dict(buf=buf, grid=grid, parity=0)
             self._ws[key] = ws
         ws["cfg"] = _lib.make_config(self.cfg)
+        ws["cfg"].grad_bed = int(getattr(self, "_grad_bed", False))  # set by the adjoint during a backward
         return ws
 
     def _level_arg(self, level, U):
@@ -348,7 +353,8 @@ class SWE2D(nn.Module):
         self._check_state(U)
         level = self._prepare_boundary_level(
             self.cfg.constant_level if boundary_level is None else boundary_level, U)
-        if (torch.is_grad_enabled() and (U.requires_grad or level.requires_grad) and rainfall_rate is None
+        if (torch.is_grad_enabled() and (U.requires_grad or level.requires_grad or self._bed_trainable())
+                and rainfall_rate is None
                 and infiltration is None):
             from .adjoint import operator_adjoint
             rhs = operator_adjoint(self, U, level)
@@ -435,7 +441,7 @@ class SWE2D(nn.Module):
             # time_float = float(model_time) of a tensor in U's dtype (:608-611)
             t = model_time.detach() if torch.is_tensor(model_time) else U.new_tensor(model_time)
             forcing = (infiltration, float(t.cpu()))
-        needs_grad = torch.is_grad_enabled() and (U.requires_grad or level.requires_grad)
+        needs_grad = torch.is_grad_enabled() and (U.requires_grad or level.requires_grad or self._bed_trainable())
         if needs_grad:
             from .adjoint import differentiable_step
             return differentiable_step(self, U, level, max_dt_value, forcing)
@@ -495,7 +501,7 @@ class SWE2D(nn.Module):
         self._check_state(U0, "U0")
         cap = None if max_dt_cap is None else self._scalar_float(max_dt_cap, "max_dt_cap")
         level_req = boundary_level is not None and torch.is_tensor(boundary_level) and boundary_level.requires_grad
-        needs_grad = torch.is_grad_enabled() and (U0.requires_grad or level_req)
+        needs_grad = torch.is_grad_enabled() and (U0.requires_grad or level_req or self._bed_trainable())
         forced = self.rainfall is not None or infiltration is not None
         if needs_grad and not forced:
             from .adjoint import differentiable_forward

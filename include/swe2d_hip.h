@@ -28,7 +28,7 @@
 extern "C" {
 #endif
 
-#define SWE_ABI_VERSION 1
+#define SWE_ABI_VERSION 2
 
 enum {
   SWE_OK = 0,
@@ -71,6 +71,9 @@ typedef struct swe_config {
   int32_t positivity_limiter;
   int32_t apply_friction;
   int32_t friction_model;
+  int32_t grad_bed;               /* adjoint calls: also accumulate the bed adjoint (see
+                                     swe_adjoint_bed_grad_offset); 0 elsewhere */
+  int32_t pad;
 } swe_config;
 
 /* Device control block at the head of the workspace (read back by the host). */
@@ -235,6 +238,13 @@ int swe_advance_tape_f64(const swe_grid* grid, const swe_config* cfg, const doub
  * ACCUMULATED into (+=), so a reverse sweep over many steps sums it. */
 size_t swe_adjoint_workspace_bytes(const swe_grid* grid, int dtype_bytes);
 const char* swe_adjoint_last_error_message(void);
+
+/* Byte offset, in the adjoint workspace, of the bed adjoint [batch, ny, nx]
+ * (the gradient with respect to SWE2D.bed, through which the reference's
+ * autograd reaches dem_correction_parameter, solver.py:100-110).  Every
+ * adjoint call made with cfg->grad_bed set ADDS its contribution there; the
+ * caller clears it before a backward pass and reads it after. */
+size_t swe_adjoint_bed_grad_offset(const swe_grid* grid, int dtype_bytes);
 
 /* L^T: adjoint of swe_spatial_operator with respect to U (and the level). */
 int swe_operator_adjoint_f32(const swe_grid* grid, const swe_config* cfg, const float* U, const float* bed,

@@ -255,15 +255,3 @@ def test_spline_gradient_central_differences(golden, cuda_device, ieee_oracle):
         ad = float(ctrl.grad[i])
         print(f"control {i}: adjoint {ad:.8e}, central FD (dt frozen) {fd:.8e}, rel {abs(ad - fd) / max(abs(fd), 1e-300):.2e}")
         assert abs(ad - fd) <= 1e-4 * max(abs(fd), 1e-3 * float(ctrl.grad.abs().max()))
-
-
-def test_trainable_dem_warns_that_the_bed_is_constant(cuda_device):
-    n = 16
-    m = swe.SWE2D(torch.zeros(1, 1, n, n), torch.zeros(1, 1, n, n), 0.1, 0.1).to(cuda_device)  # train_dem=True
-    U = torch.zeros(1, 3, n, n, device=cuda_device)
-    U[:, 0] = 1.0
-    U.requires_grad_(True)
-    with pytest.warns(UserWarning, match="dem_correction_parameter receives no gradient"):
-        out, _ = m.step(U, 0.0, 1e30)
-    out.sum().backward()
-    assert m.dem_correction_parameter.grad is None

@@ -1,0 +1,138 @@
+"""Gradient with respect to the bed (a trainable DEM correction) against oracle autograd.
+
+The reference's SWE2D makes dem_correction_parameter trainable by default
+(src/solver.py:50-147, bed = bed_reference + correction on valid cells), so its
+autograd carries the gradient of a loss into the bed.  Here the hand-written
+adjoint accumulates that gradient in its workspace (cfg.grad_bed) and the
+autograd Functions return it for the bed.  Bars: FP64 1e-10 relative to oracle
+autograd for one step and one operator application, 1e-9 through a many-step
+checkpointed or taped forward; FP32 1e-4.
+"""
+import pytest
+import torch
+
+import diffswe2d_amd as swe
+from oracle import swe_oracle as O
+from test_adjoint_gpu import STEP_CASES, relerr
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture
+def ieee_oracle():
+    """The oracle with correctly rounded sqrt: its forward values match the GPU's bit for bit."""
+    old = O.SQRT_MODE
+    O.SQRT_MODE = "ieee"
+    yield O
+    O.SQRT_MODE = old
+
+
+def case_setup(golden, case, dtype, dev, maximum=None):
+    d = golden("steps")
+    c = STEP_CASES[case]
+    cfg = swe.SWEConfig(boundary_left=c["bc"][0], boundary_right=c["bc"][1], boundary_top=c["bc"][2],
+                        boundary_bottom=c["bc"][3], apply_friction=c["friction"], constant_value=0.05)
+    bed = torch.tensor(d["bed"], dtype=dtype)
+    rough = torch.tensor(d["rough"], dtype=dtype)
+    m = swe.SWE2D(bed, rough, 0.1, 0.1, config=cfg, train_dem=True, maximum_dem_correction=maximum).to(dev)
+    U = torch.tensor(d["U0"], dtype=dtype)
+    return m, cfg, bed, rough, U, torch.tensor(c["level"], dtype=dtype)
+
+
+@pytest.mark.parametrize("case", range(len(STEP_CASES)))
+@pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
+def test_step_bed_gradient(golden, cuda_device, ieee_oracle, case, dtype):
+    """One step: the DEM correction's gradient equals oracle autograd's bed gradient; U's is unchanged."""
+    m, cfg, bed, rough, U, lvl = case_setup(golden, case, dtype, cuda_device)
+    lam = torch.randn(U.shape, dtype=dtype, generator=torch.Generator().manual_seed(40 + case))
+    Ur, br = U.clone().requires_grad_(True), bed.clone().requires_grad_(True)
+    out, _ = O.step(Ur, br, rough, cfg, 0.1, 0.1, 1e30, lvl)
+    gU_ref, gb_ref = torch.autograd.grad(out, (Ur, br), lam)
+    Ug = U.to(cuda_device).requires_grad_(True)
+    outg, _ = m.step(Ug, 0.0, 1e30, boundary_level=lvl.to(cuda_device))
+    gU, gp = torch.autograd.grad(outg, (Ug, m.dem_correction_parameter), lam.to(cuda_device))
+    e_b, e_U = relerr(gp, gb_ref), relerr(gU, gU_ref)
+    tol = 1e-10 if dtype == torch.float64 else 1e-4
+    print(f"case {case} {dtype}: bed gradient rel err {e_b:.2e} (|g| {float(gb_ref.norm()):.3e}), U {e_U:.2e}")
+    assert e_b <= tol and e_U <= tol
+
+
+@pytest.mark.parametrize("case", range(len(STEP_CASES)))
+def test_operator_bed_gradient_fp64(golden, cuda_device, case):
+    m, cfg, bed, rough, U, lvl = case_setup(golden, case, torch.float64, cuda_device)
+    lam = torch.randn(U.shape, dtype=torch.float64, generator=torch.Generator().manual_seed(60 + case))
+    br = bed.clone().requires_grad_(True)
+    gb_ref, = torch.autograd.grad(O.spatial_operator(U, br, cfg, 0.1, 0.1, lvl), (br,), lam)
+    rhs = m.spatial_operator(U.to(cuda_device), boundary_level=lvl.to(cuda_device))
+    gp, = torch.autograd.grad(rhs, (m.dem_correction_parameter,), lam.to(cuda_device))
+    e = relerr(gp, gb_ref)
+    print(f"case {case}: operator bed gradient rel err {e:.2e}")
+    assert e <= 1e-10
+
+
+def ripple_runup(dev, maximum=None, batch=1):
+    """The reduced coastal run-up with a rippled, slightly moving surface (no exact lake-at-rest ties)."""
+    from diffswe2d_amd import scenarios
+
+    sc = scenarios.coastal_runup(nx=40, ny=26, dx=0.136)
+    cfg = swe.SWEConfig(boundary_left="water_level", boundary_right="wall", boundary_top="wall",
+                        boundary_bottom="absorbing")
+    bed = torch.tensor(sc["bed"])
+    rough = torch.tensor(sc["rough"])
+    m = swe.SWE2D(bed, rough, 0.136, 0.136, config=cfg, train_dem=True, maximum_dem_correction=maximum).to(dev)
+    U0 = torch.tensor(sc["U"])
+    yy, xx = torch.meshgrid(torch.arange(26, dtype=torch.float64), torch.arange(40, dtype=torch.float64),
+                            indexing="ij")
+    wet = U0[0, 0] > 0
+    U0[0, 0] = torch.where(wet, U0[0, 0] + (2e-3 * torch.sin(0.7 * xx + 0.3) * torch.cos(0.45 * yy)).abs() + 1e-3,
+                           U0[0, 0])
+    U0[0, 1] = torch.where(wet, 1e-3 * torch.cos(0.3 * yy) * U0[0, 0], U0[0, 1])
+    if batch > 1:
+        U0 = torch.cat([U0, U0 * torch.tensor([1.0, 0.5, 1.0], dtype=torch.float64)[None, :, None, None]])
+    return m, cfg, bed, rough, U0
+
+
+@pytest.mark.parametrize("mode", ["tape", "replay"])
+def test_forward_bed_gradient_fp64(cuda_device, ieee_oracle, mode):
+    """A 0.3 s forward call (many steps, K = 5 checkpoints) with a trainable DEM against oracle autograd."""
+    m, cfg, bed, rough, U0 = ripple_runup(cuda_device)
+    m.adjoint_checkpoint_every = 5
+    m.adjoint_tape = mode == "tape"
+    lvl = torch.tensor(0.015, dtype=torch.float64)
+    lam = torch.randn(U0.shape, dtype=torch.float64, generator=torch.Generator().manual_seed(8))
+    br = bed.clone().requires_grad_(True)
+    out, _ = O.forward(U0, br, rough, cfg, 0.136, 0.136, 0.3, boundary_level=lvl)
+    gb_ref, = torch.autograd.grad(out, (br,), lam)
+    outg, _ = m(U0.to(cuda_device), t_end=0.3, boundary_level=lvl.to(cuda_device))
+    assert relerr(outg.detach(), out.detach()) < 1e-12
+    kinds = {kind for kind, _, _ in m.last_adjoint_segments}
+    assert kinds == {"tape" if mode == "tape" else "ckpt"}, kinds
+    gp, = torch.autograd.grad(outg, (m.dem_correction_parameter,), lam.to(cuda_device))
+    e = relerr(gp, gb_ref)
+    print(f"{mode}: {len(m.last_dt_series)} steps, bed gradient rel err {e:.2e}")
+    assert len(m.last_dt_series) > 10 and e <= 1e-9
+
+
+def test_shared_bed_two_members_and_bounded_correction(cuda_device, ieee_oracle):
+    """Two members over one bed: the bed gradient is the sum of the members'; with
+    maximum_dem_correction the chain through tanh is torch's own."""
+    m, cfg, bed, rough, U0 = ripple_runup(cuda_device, maximum=0.5, batch=2)
+    with torch.no_grad():
+        m.dem_correction_parameter.copy_(0.01 * torch.randn(bed.shape, generator=torch.Generator().manual_seed(2),
+                                                            dtype=torch.float64))
+    lvl = torch.tensor(0.015, dtype=torch.float64)
+    lam = torch.randn(U0.shape, dtype=torch.float64, generator=torch.Generator().manual_seed(9))
+    p = m.dem_correction_parameter.detach().cpu().clone().requires_grad_(True)
+    bed_o = bed + 0.5 * torch.tanh(p)
+    out = U0
+    for _ in range(6):
+        out, _ = O.step(out, bed_o, rough, cfg, 0.136, 0.136, 1e30, lvl)
+    gp_ref, = torch.autograd.grad(out, (p,), lam)
+    outg = U0.to(cuda_device)
+    for _ in range(6):
+        outg, _ = m.step(outg, 0.0, 1e30, boundary_level=lvl.to(cuda_device))
+    assert relerr(outg.detach(), out.detach()) < 1e-12
+    gp, = torch.autograd.grad(outg, (m.dem_correction_parameter,), lam.to(cuda_device))
+    e = relerr(gp, gp_ref)
+    print(f"two members, bounded correction: parameter gradient rel err {e:.2e}")
+    assert e <= 1e-10
