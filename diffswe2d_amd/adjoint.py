@@ -20,7 +20,6 @@ from __future__ import annotations
 
 import contextlib
 import ctypes
-import warnings
 
 import torch
 
@@ -91,16 +90,6 @@ def step_adjoint(model, U, k1, k2, level, dt, lam_out, lam_level):
         _lib.ptr(lvl), stride, float(dt), _lib.ptr(lam_out.contiguous()), _lib.ptr(lam_U), _lib.ptr(lam_level),
         _lib.ptr(aws["buf"]), int(model.rows_hint), _lib.stream_ptr(U.device)), "step_adjoint", _ADJ_MSG)
     return lam_U
-
-
-def _warn_constant_bed(model):
-    """The chunked device loop treats the bed as a constant: say so once when the DEM correction is trainable."""
-    p = getattr(model, "dem_correction_parameter", None)
-    if p is not None and p.requires_grad and not getattr(model, "_warned_constant_bed", False):
-        warnings.warn("the chunked device loop differentiates with respect to the state and the boundary levels "
-                      "only; dem_correction_parameter receives no gradient there (construct the model with "
-                      "train_dem=False to silence this)", stacklevel=3)
-        model._warned_constant_bed = True
 
 
 def _bed_input(model):

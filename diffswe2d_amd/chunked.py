@@ -22,7 +22,7 @@ import ctypes
 import torch
 
 from . import _lib
-from .adjoint import _ADJ_MSG, _adj_ws, _fn, _warn_constant_bed, tape_budget
+from .adjoint import _ADJ_MSG, _adj_ws, _bed_grad, _bed_gradient, _bed_input, _fn, _meta, tape_budget
 from .solver import sync_parity
 
 _TAPE_SEGMENT = 512     # steps per recorded segment when the tape fits
@@ -115,9 +115,10 @@ def _run(model, U0, levels, chunk, gauges, max_steps, K, budget):
 
 class _ChunkedFn(torch.autograd.Function):
     @staticmethod
-    def forward(ctx, U0, levels, model, chunk, gauges, max_steps):
+    def forward(ctx, U0, levels, bed, model, chunk, gauges, max_steps):
         U0_ = U0.detach().contiguous()
         lv = levels.detach().contiguous()
+        ctx.bed_meta = _meta(bed)
         K = int(getattr(model, "adjoint_checkpoint_every", 64))
         if K < 1:
             raise ValueError("adjoint_checkpoint_every must be at least 1")
@@ -138,47 +139,57 @@ class _ChunkedFn(torch.autograd.Function):
         some = segments[0][3]
         U0 = some[0][0] if segments[0][0] == "tape" else some
         dtype, dev = U0.dtype, U0.device
-        bed, rough = model._fields(U0)
-        ws = model._workspace(U0, bed, rough)
-        aws = _adj_ws(model, U0)
-        stream = _lib.stream_ptr(dev)
-        rows = int(model.rows_hint)
-        lam = g_U.contiguous().clone() if g_U is not None else torch.zeros_like(U0)
-        lam_tmp = torch.empty_like(lam)
-        level_grad = torch.zeros(lv.numel(), dtype=dtype, device=dev)
-        n_g = gauges.shape[0]
-        g_obs = (g_obs.contiguous() if g_obs is not None
-                 else torch.zeros(lv.numel(), max(n_g, 1), dtype=dtype, device=dev))
-        n = len(ctx.dts)
-        chunk = ctx.chunks
-        ends = [1 if k == n - 1 or chunk[k + 1] != chunk[k] else 0 for k in range(n)]
-        host_dts = (ctypes.c_double * n)(*ctx.dts)
-        host_chunk = (ctypes.c_int32 * n)(*chunk)
-        host_ends = (ctypes.c_int32 * n)(*ends)
-        scratch = None
-        for kind, first, m, data in reversed(segments):
-            dptr = ctypes.addressof(host_dts) + 8 * first
-            cptr = ctypes.addressof(host_chunk) + 4 * first
-            eptr = ctypes.addressof(host_ends) + 4 * first
-            if kind == "tape":
-                tU, tk1, tk2 = data
-                _lib.check(_fn(lib, "swe_segment_reverse_chunked", dtype)(
-                    ws["grid"], ws["cfg"], _lib.ptr(tU), _lib.ptr(tk1), _lib.ptr(tk2), _lib.ptr(bed),
-                    _lib.ptr(rough), _lib.ptr(lv), dptr, cptr, eptr, m, _lib.ptr(gauges), n_g, _lib.ptr(g_obs),
-                    _lib.ptr(lam), _lib.ptr(lam_tmp), _lib.ptr(level_grad), _lib.ptr(aws["buf"]), rows, stream),
-                    "segment_reverse_chunked", _ADJ_MSG)
-            else:
-                need = lib.swe_segment_tape_bytes(ws["grid"], m, U0.element_size()) // U0.element_size()
-                if scratch is None or scratch.numel() < need:
-                    scratch = torch.empty(need, dtype=dtype, device=dev)
-                _lib.check(_fn(lib, "swe_segment_adjoint_chunked", dtype)(
-                    ws["grid"], ws["cfg"], _lib.ptr(data), _lib.ptr(bed), _lib.ptr(rough), _lib.ptr(lv), dptr,
-                    cptr, eptr, m, ws["parity"], _lib.ptr(gauges), n_g, _lib.ptr(g_obs), _lib.ptr(lam),
-                    _lib.ptr(lam_tmp), _lib.ptr(level_grad), _lib.ptr(scratch), _lib.ptr(ws["buf"]),
-                    _lib.ptr(aws["buf"]), rows, stream), "segment_adjoint_chunked", _ADJ_MSG)
-                ws["parity"] ^= m & 1
+        want_bed = ctx.needs_input_grad[2]
+        with _bed_gradient(model, U0, want_bed) as region:
+            bed, rough = model._fields(U0)
+            ws = model._workspace(U0, bed, rough)
+            aws = _adj_ws(model, U0)
+            stream = _lib.stream_ptr(dev)
+            rows = int(model.rows_hint)
+            lam = g_U.contiguous().clone() if g_U is not None else torch.zeros_like(U0)
+            lam_tmp = torch.empty_like(lam)
+            level_grad = torch.zeros(lv.numel(), dtype=dtype, device=dev)
+            n_g = gauges.shape[0]
+            g_obs = (g_obs.contiguous() if g_obs is not None
+                     else torch.zeros(lv.numel(), max(n_g, 1), dtype=dtype, device=dev))
+            n = len(ctx.dts)
+            chunk = ctx.chunks
+            ends = [1 if k == n - 1 or chunk[k + 1] != chunk[k] else 0 for k in range(n)]
+            host_dts = (ctypes.c_double * n)(*ctx.dts)
+            host_chunk = (ctypes.c_int32 * n)(*chunk)
+            host_ends = (ctypes.c_int32 * n)(*ends)
+            scratch = None
+            for kind, first, m, data in reversed(segments):
+                dptr = ctypes.addressof(host_dts) + 8 * first
+                cptr = ctypes.addressof(host_chunk) + 4 * first
+                eptr = ctypes.addressof(host_ends) + 4 * first
+                if kind == "tape":
+                    tU, tk1, tk2 = data
+                    _lib.check(_fn(lib, "swe_segment_reverse_chunked", dtype)(
+                        ws["grid"], ws["cfg"], _lib.ptr(tU), _lib.ptr(tk1), _lib.ptr(tk2), _lib.ptr(bed),
+                        _lib.ptr(rough), _lib.ptr(lv), dptr, cptr, eptr, m, _lib.ptr(gauges), n_g, _lib.ptr(g_obs),
+                        _lib.ptr(lam), _lib.ptr(lam_tmp), _lib.ptr(level_grad), _lib.ptr(aws["buf"]), rows, stream),
+                        "segment_reverse_chunked", _ADJ_MSG)
+                else:
+                    need = lib.swe_segment_tape_bytes(ws["grid"], m, U0.element_size()) // U0.element_size()
+                    if scratch is None or scratch.numel() < need:
+                        scratch = torch.empty(need, dtype=dtype, device=dev)
+                    _lib.check(_fn(lib, "swe_segment_adjoint_chunked", dtype)(
+                        ws["grid"], ws["cfg"], _lib.ptr(data), _lib.ptr(bed), _lib.ptr(rough), _lib.ptr(lv), dptr,
+                        cptr, eptr, m, ws["parity"], _lib.ptr(gauges), n_g, _lib.ptr(g_obs), _lib.ptr(lam),
+                        _lib.ptr(lam_tmp), _lib.ptr(level_grad), _lib.ptr(scratch), _lib.ptr(ws["buf"]),
+                        _lib.ptr(aws["buf"]), rows, stream), "segment_adjoint_chunked", _ADJ_MSG)
+                    ws["parity"] ^= m & 1
         ctx.segments = None
-        return lam, level_grad, None, None, None, None
+        g_bed = None
+        if want_bed:
+            g_bed = _bed_grad(region, ctx.bed_meta)
+            if g_obs is not None and n_g > 0:
+                # observations are h + z at the gauge cells: their adjoint reaches the bed directly
+                gsum = g_obs[:, :n_g].sum(dim=0).to(g_bed.dtype)
+                gi = gauges.long().to(g_bed.device)
+                g_bed[0, 0].index_put_((gi[:, 0], gi[:, 1]), gsum.to(g_bed.device), accumulate=True)
+        return lam, level_grad, g_bed, None, None, None, None
 
 
 def gauge_run(model, U0, series, chunk, gauges, max_steps=10_000_000):
@@ -204,9 +215,8 @@ def gauge_run(model, U0, series, chunk, gauges, max_steps=10_000_000):
     levels = series.reshape(-1).to(dtype=U0.dtype)
     if not (chunk > 0):
         raise ValueError("chunk duration must be positive")
-    if torch.is_grad_enabled() and (U0.requires_grad or levels.requires_grad):
-        _warn_constant_bed(model)
-        return _ChunkedFn.apply(U0, levels, model, float(chunk), g, int(max_steps))
+    if torch.is_grad_enabled() and (U0.requires_grad or levels.requires_grad or model._bed_trainable()):
+        return _ChunkedFn.apply(U0, levels, _bed_input(model), model, float(chunk), g, int(max_steps))
     with torch.no_grad():
         obs, U, dts, _, _ = _run(model, U0.detach().contiguous(), levels.detach().contiguous(), float(chunk), g,
                                  int(max_steps), 0, None)

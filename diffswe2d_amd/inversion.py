@@ -109,16 +109,24 @@ class GaugeProblem:
         self._ix = torch.tensor([g[1] for g in self.gauges], device=self.U0.device)
         self._bed_g = bed[0, 0, self._iy, self._ix]
 
+    def _gauge_bed(self):
+        """The bed at the gauges: the current SWE2D.bed, differentiable when the DEM correction trains."""
+        if torch.is_grad_enabled() and self.model._bed_trainable():
+            bed = self.model.bed.to(device=self.U0.device, dtype=self.U0.dtype)
+            return bed[0, 0, self._iy, self._ix]
+        return self._bed_g
+
     def simulate(self, series: torch.Tensor) -> torch.Tensor:
         """Observations [n_chunks, n_gauges] for the dense boundary series."""
         if series.numel() < self.n_chunks:
             raise ValueError("series shorter than the number of chunks")
+        bed_g = self._gauge_bed()
         if self.fused and self.U0.shape[0] == 1 and not self.model.use_dynamic_bc:
             from .chunked import gauge_run
 
             obs, _ = gauge_run(self.model, self.U0, series[:self.n_chunks], self.chunk, self.gauges)
             self.stats["steps"] = len(self.model.last_dt_series)
-            return obs if self.observe == "water_level" else obs - self._bed_g
+            return obs if self.observe == "water_level" else obs - bed_g
         s = self.U0
         obs = []
         steps = 0
@@ -127,7 +135,7 @@ class GaugeProblem:
                               boundary_level=series[k])
             steps += len(getattr(self.model, "last_dt_series", []))
             depth = s[0, 0, self._iy, self._ix]
-            obs.append(depth + self._bed_g if self.observe == "water_level" else depth)
+            obs.append(depth + bed_g if self.observe == "water_level" else depth)
         self.stats["steps"] = steps
         return torch.stack(obs)
 
