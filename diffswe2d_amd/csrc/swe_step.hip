@@ -292,8 +292,11 @@ int pick_rows(const swe_grid* g, int rows_hint) {
   long long per_col = (long long)strips * g->batch;
   int S = (int)((long long)g->ny * per_col / target);
   // small grids: short strips, so that enough wavefronts run at once; each
-  // wavefront's rows form one dependent chain, whose latency sets the step time
-  if (S < 2) S = 2;
+  // wavefront's rows form one dependent chain, whose latency sets the step time.
+  // One-row strips when even they give at most one wavefront per SIMD (1,024 on
+  // the chip): the chain is then shortest (5 rows instead of 6) and nothing waits
+  // for a SIMD; two-row strips otherwise.
+  if (S < 2) S = (long long)g->ny * per_col <= 1024 ? 1 : 2;
   if (S > 64) S = 64;
   if (S > g->ny) S = g->ny;
   return S;
