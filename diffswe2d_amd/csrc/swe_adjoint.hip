@@ -574,7 +574,9 @@ int adj_rows(const swe_grid* g, int hint) {
   int strips = (g->nx + 4 + kOutA - 1) / kOutA;
   long long per = (long long)strips * g->batch;
   int S = (int)((long long)(g->ny + 4) * per / 4096);
-  if (S < 2) S = 2;  // small grids: more, shorter strips (latency-bound)
+  // small grids are latency-bound: shorter strips, one row when that still gives
+  // at most one wavefront per SIMD (1,024 on the chip)
+  if (S < 2) S = (long long)(g->ny + 4) * per <= 1024 ? 1 : 2;
   if (S > 64) S = 64;
   return S;
 }
