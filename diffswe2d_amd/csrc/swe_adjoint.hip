@@ -557,7 +557,11 @@ template <class T, int STAGE> void launch_adj_L(const AdjArgs<T>& a, dim3 grid, 
 template <class T>
 void ghost_fold(const AdjArgs<T>& a, int stage, int batch, T* lam_level, hipStream_t s) {
   if (a.lmax <= kGhostOneBlock) {
-    hipLaunchKernelGGL((adj_ghost_all_kernel<T>), dim3(batch), dim3(1024), 0, s, a, stage, lam_level);
+    // as many threads as the longest side needs (a power of two for the block sum):
+    // the fold's four phases and the sum then synchronise as few wavefronts as possible
+    int threads = 64;
+    while (threads < a.lmax && threads < 1024) threads <<= 1;
+    hipLaunchKernelGGL((adj_ghost_all_kernel<T>), dim3(batch), dim3(threads), 0, s, a, stage, lam_level);
     return;
   }
   dim3 grid((unsigned)a.part_blocks, (unsigned)batch);
