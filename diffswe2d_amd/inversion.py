@@ -22,7 +22,6 @@ Gradients come from the hand-written adjoint (``diffswe2d_amd.adjoint``).
 """
 from __future__ import annotations
 
-import math
 import time
 from dataclasses import dataclass, field
 

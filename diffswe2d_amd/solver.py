@@ -18,7 +18,7 @@ from __future__ import annotations
 import ctypes
 import math
 from numbers import Integral, Real
-from typing import Optional, Union
+from typing import Union
 
 import torch
 import torch.nn as nn
