@@ -503,7 +503,8 @@ class SWE2D(nn.Module):
         level_req = boundary_level is not None and torch.is_tensor(boundary_level) and boundary_level.requires_grad
         needs_grad = torch.is_grad_enabled() and (U0.requires_grad or level_req or self._bed_trainable())
         forced = self.rainfall is not None or infiltration is not None
-        per_step = callback is not None or self.track_gauges  # per-step work of the loop (src/solver.py:765-801)
+        # per-step work of the loop (src/solver.py:765-801), or a level interpolated per step (:391-415)
+        per_step = callback is not None or self.track_gauges or (boundary_level is None and self.use_dynamic_bc)
         if needs_grad and not forced and not per_step:
             from .adjoint import differentiable_forward
             return differentiable_forward(self, U0, duration, absolute_start, max_steps, boundary_level, cap)

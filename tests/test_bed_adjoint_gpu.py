@@ -204,3 +204,26 @@ def test_differentiable_forward_with_callback_and_gauges(cuda_device):
     assert n > 5 and len(calls) == n and len(m.ts_h["g0"]) == n
     assert torch.equal(out_h, out_d)
     assert relerr(gU_h, gU_d) <= 1e-14 and relerr(gb_h, gb_d) <= 1e-14
+
+
+def test_dynamic_boundary_series_gradients(cuda_device, ieee_oracle):
+    """With a dynamic boundary series (set_dynamic_boundary, levels interpolated at each step start,
+    src/solver.py:391-415) and gradients on, the forward steps through the per-step loop: gradients reach
+    U0, the bed and the series itself, as oracle autograd gives them."""
+    m, cfg, bed, rough, U0 = ripple_runup(cuda_device)
+    times = torch.tensor([0.0, 0.1, 0.2, 0.35], dtype=torch.float64)
+    wls = torch.tensor([0.010, 0.018, 0.012, 0.020], dtype=torch.float64)
+    wls_g = wls.clone().to(cuda_device).requires_grad_(True)
+    m.set_dynamic_boundary(times.to(cuda_device), wls_g)
+    lam = torch.randn(U0.shape, dtype=torch.float64, generator=torch.Generator().manual_seed(31))
+    Ug = U0.to(cuda_device).requires_grad_(True)
+    out, _ = m(Ug, t_end=0.3)
+    gU, gp, gw = torch.autograd.grad(out, (Ug, m.dem_correction_parameter, wls_g), lam.to(cuda_device))
+    Ur, br, wr = U0.clone().requires_grad_(True), bed.clone().requires_grad_(True), wls.clone().requires_grad_(True)
+    out_o, _ = O.forward(Ur, br, rough, cfg, 0.136, 0.136, 0.3, dynamic=(times, wr))
+    gU_o, gb_o, gw_o = torch.autograd.grad(out_o, (Ur, br, wr), lam)
+    e = (relerr(gU, gU_o), relerr(gp, gb_o), relerr(gw, gw_o))
+    print(f"dynamic series: state {relerr(out.detach(), out_o.detach()):.1e}; gradients U0 {e[0]:.1e}, "
+          f"bed {e[1]:.1e}, series {e[2]:.1e}")
+    assert relerr(out.detach(), out_o.detach()) < 1e-12
+    assert max(e) <= 1e-9
