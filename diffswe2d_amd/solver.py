@@ -353,11 +353,24 @@ class SWE2D(nn.Module):
         self._check_state(U)
         level = self._prepare_boundary_level(
             self.cfg.constant_level if boundary_level is None else boundary_level, U)
-        if (torch.is_grad_enabled() and (U.requires_grad or level.requires_grad or self._bed_trainable())
-                and rainfall_rate is None
-                and infiltration is None):
+        sources_grad = any(torch.is_tensor(v) and v.requires_grad for v in (rainfall_rate, infiltration))
+        if torch.is_grad_enabled() and (U.requires_grad or level.requires_grad or self._bed_trainable()
+                                        or sources_grad):
             from .adjoint import operator_adjoint
             rhs = operator_adjoint(self, U, level)
+            if rainfall_rate is not None or infiltration is not None:
+                # the source terms join the depth tendency after the fluxes, rain first
+                # (src/solver.py:520-525), as the kernel adds them
+                def source(v):
+                    t = v.to(dtype=U.dtype, device=U.device) if torch.is_tensor(v) else U.new_tensor(v)
+                    return torch.broadcast_to(t, U[:, 0:1].shape)
+
+                h = rhs[:, 0:1]
+                if rainfall_rate is not None:
+                    h = h + source(rainfall_rate)
+                if infiltration is not None:
+                    h = h - torch.clamp(source(infiltration), min=0)
+                rhs = torch.cat((h, rhs[:, 1:]), dim=1)
             if return_wave_speed:
                 with torch.no_grad():
                     _, ax, ay = self.spatial_operator(U.detach(), return_wave_speed=True,
