@@ -253,3 +253,34 @@ def test_operator_with_sources_is_differentiable(golden, cuda_device):
     print("operator with sources: gradient rel errs (U, bed, rain, infiltration)", [f"{e:.1e}" for e in errs])
     assert relerr(rhs.detach(), ref.detach()) < 1e-13
     assert max(errs) <= 1e-10
+
+
+@pytest.mark.parametrize("shape", [(2, 2), (2, 5), (5, 3), (4, 4), (3, 7)])
+@pytest.mark.parametrize("case", [0, 2, 4])
+def test_tiny_grids_step_and_adjoint_fp64(cuda_device, ieee_oracle, shape, case):
+    """Grids down to 2 x 2 (every cell next to a side, ghosts reaching across the grid): one step and its
+    adjoint with respect to the state and the bed against the oracle."""
+    ny, nx = shape
+    c = STEP_CASES[case]
+    cfg = swe.SWEConfig(boundary_left=c["bc"][0], boundary_right=c["bc"][1], boundary_top=c["bc"][2],
+                        boundary_bottom=c["bc"][3], apply_friction=c["friction"], constant_value=0.05)
+    g = torch.Generator().manual_seed(100 * ny + nx + case)
+    bed = -0.3 + 0.1 * torch.rand(1, 1, ny, nx, dtype=torch.float64, generator=g)
+    rough = torch.full((1, 1, ny, nx), 0.02, dtype=torch.float64)
+    h = (0.2 - bed[0, 0]).clamp_min(0) + 0.05 * torch.rand(ny, nx, dtype=torch.float64, generator=g)
+    U = torch.stack([h, 0.05 * torch.randn(ny, nx, dtype=torch.float64, generator=g) * h,
+                     0.05 * torch.randn(ny, nx, dtype=torch.float64, generator=g) * h])[None]
+    lvl = torch.tensor(c["level"], dtype=torch.float64)
+    lam = torch.randn(U.shape, dtype=torch.float64, generator=g)
+    m = swe.SWE2D(bed, rough, 0.1, 0.1, config=cfg, train_dem=True).to(cuda_device)
+    Ur, br = U.clone().requires_grad_(True), bed.clone().requires_grad_(True)
+    out, dt = O.step(Ur, br, rough, cfg, 0.1, 0.1, 1e30, lvl)
+    gU_ref, gb_ref = torch.autograd.grad(out, (Ur, br), lam)
+    Ug = U.to(cuda_device).requires_grad_(True)
+    outg, dtg = m.step(Ug, 0.0, 1e30, boundary_level=lvl.to(cuda_device))
+    gU, gp = torch.autograd.grad(outg, (Ug, m.dem_correction_parameter), lam.to(cuda_device))
+    e = (relerr(outg.detach(), out.detach()), relerr(gU, gU_ref), relerr(gp, gb_ref))
+    print(f"{ny}x{nx} case {case}: dt {dtg:.6e} vs {float(dt):.6e}; state {e[0]:.1e}, adjoint U {e[1]:.1e}, "
+          f"bed {e[2]:.1e}")
+    assert abs(dtg - float(dt)) <= 1e-12 * float(dt)
+    assert e[0] <= 1e-13 and e[1] <= 1e-10 and e[2] <= 1e-10
