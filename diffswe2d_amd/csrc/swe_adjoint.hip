@@ -371,39 +371,60 @@ void adj_L_kernel(AdjArgs<T> a) {
 // rule depends on it (absorbing), the bed only where the level enters (water
 // level, absorbing); both layers' inputs are read before the edge cells are
 // updated, layer 0 first.
+// The inputs of one position's fold: both layers' ghost adjoints and maps, and
+// what their rules read (edge state for absorbing, bed and level where the level
+// enters) -- none of which the fold itself changes.
+template <class T> struct FoldIn {
+  GhostMap g[2];
+  T gh[2], gu[2], gv[2], gz[2], eh[2], z[2];
+  T level;
+};
+
 template <class T>
-__device__ inline T ghost_fold_at(const AdjArgs<T>& a, int stage, int side, int b, int i) {
+__device__ inline FoldIn<T> ghost_fold_load(const AdjArgs<T>& a, int stage, int side, int b, int i) {
   const Fields<T>& f = a.f;
   const int64_t base = (int64_t)b * 3 * f.plane;
-  GhostMap g[2];
-  T gh[2], gu[2], gv[2], gz[2];
+  FoldIn<T> in;
   for (int layer = 0; layer < 2; ++layer) {
     int r, c;
     if (side == LEFT) { r = i; c = -1 - layer; }
     else if (side == RIGHT) { r = i; c = f.nx + layer; }
     else if (side == TOP) { r = -1 - layer; c = i; }
     else { r = f.ny + layer; c = i; }
-    g[layer] = ghost_map(r, c, f.ny, f.nx, f.bc);
+    in.g[layer] = ghost_map(r, c, f.ny, f.nx, f.bc);
     int64_t gs = ((((int64_t)b * 4 + side) * 2 + layer) * a.lmax + i) * kGhostVals;
-    gh[layer] = a.ghost[gs];
-    gu[layer] = a.ghost[gs + 1];
-    gv[layer] = a.ghost[gs + 2];
-    gz[layer] = a.ghost[gs + 3];
+    in.gh[layer] = a.ghost[gs];
+    in.gu[layer] = a.ghost[gs + 1];
+    in.gv[layer] = a.ghost[gs + 2];
+    in.gz[layer] = a.ghost[gs + 3];
   }
-  const int kind = g[0].kind;  // both layers belong to the same side
+  const int kind = in.g[0].kind;  // both layers belong to the same side
   const bool uses_level = kind == BC_WATER_LEVEL || kind == BC_ABSORBING;
-  const T level = uses_level ? f.level[b * f.level_stride] : T(0);
-  T glev_sum = T(0);
+  in.level = uses_level ? f.level[b * f.level_stride] : T(0);
   for (int layer = 0; layer < 2; ++layer) {
-    const GhostMap& m = g[layer];
+    const GhostMap& m = in.g[layer];
     T eh = T(0), eu, ev;
     if (kind == BC_ABSORBING) {
       if (stage == 2) load_state<T, 2, AdjArgs<T>>(a, base, m.r, m.c, a.half_dt, eh, eu, ev);
       else load_state<T, 1, AdjArgs<T>>(a, base, m.r, m.c, a.half_dt, eh, eu, ev);
     }
-    T z = uses_level ? f.bed[(int64_t)b * f.bed_bstride + (int64_t)m.r * f.nx + m.c] : T(0);
+    in.eh[layer] = eh;
+    in.z[layer] = uses_level ? f.bed[(int64_t)b * f.bed_bstride + (int64_t)m.r * f.nx + m.c] : T(0);
+  }
+  return in;
+}
+
+// Apply one position's fold (layer 0 first); returns its boundary-level adjoint.
+template <class T>
+__device__ inline T ghost_fold_apply(const AdjArgs<T>& a, int side, int b, const FoldIn<T>& in) {
+  const Fields<T>& f = a.f;
+  const int64_t base = (int64_t)b * 3 * f.plane;
+  T glev_sum = T(0);
+  for (int layer = 0; layer < 2; ++layer) {
+    const GhostMap& m = in.g[layer];
     T ge_h, ge_u, ge_v, gmir, glev;
-    ghost_bwd(m, eh, z, level, gh[layer], gu[layer], gv[layer], a.k, ge_h, ge_u, ge_v, gmir, glev);
+    ghost_bwd(m, in.eh[layer], in.z[layer], in.level, in.gh[layer], in.gu[layer], in.gv[layer], a.k, ge_h, ge_u,
+              ge_v, gmir, glev);
     int64_t e = base + (int64_t)m.r * f.nx + m.c;
     a.out[e] += ge_h;
     a.out[e + f.plane] += ge_u;
@@ -416,11 +437,17 @@ __device__ inline T ghost_fold_at(const AdjArgs<T>& a, int stage, int side, int 
     if (a.bedgrad != nullptr) {
       // a ghost cell's bed is its source cell's; the water-level and absorbing
       // rules read it through level - z, so it takes minus the level adjoint too
-      a.bedgrad[(int64_t)b * f.plane + (int64_t)m.r * f.nx + m.c] += gz[layer] - glev;
+      a.bedgrad[(int64_t)b * f.plane + (int64_t)m.r * f.nx + m.c] += in.gz[layer] - glev;
     }
     glev_sum += glev;
   }
   return glev_sum;
+}
+
+// Folds the two ghost layers at position i along `side` of member b.
+template <class T>
+__device__ inline T ghost_fold_at(const AdjArgs<T>& a, int stage, int side, int b, int i) {
+  return ghost_fold_apply(a, side, b, ghost_fold_load(a, stage, side, b, i));
 }
 
 template <class T> __device__ inline T block_sum(T v, T* part) {
@@ -446,15 +473,32 @@ __global__ __launch_bounds__(256) void adj_ghost_side_kernel(AdjArgs<T> a, int s
 
 // Small grids: all four sides in one block per member (sides in turn, separated
 // by barriers), level adjoint summed in the block -- one launch instead of five.
-template <class T>
-__global__ __launch_bounds__(1024) void adj_ghost_all_kernel(AdjArgs<T> a, int stage, T* lam_level) {
-  __shared__ T part[1024];
+// PREFETCH (sides of at most 256 cells): one position per thread on every side,
+// all four sides' inputs loaded at once, then the sides applied in turn; the
+// 256-thread bound leaves the loaded inputs their registers.
+template <class T, bool PREFETCH>
+__global__ __launch_bounds__(PREFETCH ? 256 : 1024) void adj_ghost_all_kernel(AdjArgs<T> a, int stage,
+                                                                              T* lam_level) {
+  __shared__ T part[PREFETCH ? 256 : 1024];
   const int b = blockIdx.x;
   T glev = T(0);
-  for (int side = 0; side < 4; ++side) {
-    const int len = (side == LEFT || side == RIGHT) ? a.f.ny : a.f.nx;
-    for (int i = threadIdx.x; i < len; i += blockDim.x) glev += ghost_fold_at(a, stage, side, b, i);
-    __syncthreads();
+  if (PREFETCH) {
+    FoldIn<T> in[4];
+    for (int side = 0; side < 4; ++side) {
+      const int len = (side == LEFT || side == RIGHT) ? a.f.ny : a.f.nx;
+      if ((int)threadIdx.x < len) in[side] = ghost_fold_load(a, stage, side, b, (int)threadIdx.x);
+    }
+    for (int side = 0; side < 4; ++side) {
+      const int len = (side == LEFT || side == RIGHT) ? a.f.ny : a.f.nx;
+      if ((int)threadIdx.x < len) glev += ghost_fold_apply(a, side, b, in[side]);
+      __syncthreads();
+    }
+  } else {
+    for (int side = 0; side < 4; ++side) {
+      const int len = (side == LEFT || side == RIGHT) ? a.f.ny : a.f.nx;
+      for (int i = threadIdx.x; i < len; i += blockDim.x) glev += ghost_fold_at(a, stage, side, b, i);
+      __syncthreads();
+    }
   }
   T sum = block_sum(glev, part);
   if (threadIdx.x == 0 && lam_level != nullptr) lam_level[b] += sum;
@@ -561,7 +605,10 @@ void ghost_fold(const AdjArgs<T>& a, int stage, int batch, T* lam_level, hipStre
     // the fold's four phases and the sum then synchronise as few wavefronts as possible
     int threads = 64;
     while (threads < a.lmax && threads < 1024) threads <<= 1;
-    hipLaunchKernelGGL((adj_ghost_all_kernel<T>), dim3(batch), dim3(threads), 0, s, a, stage, lam_level);
+    if (a.lmax <= 256)
+      hipLaunchKernelGGL((adj_ghost_all_kernel<T, true>), dim3(batch), dim3(threads), 0, s, a, stage, lam_level);
+    else
+      hipLaunchKernelGGL((adj_ghost_all_kernel<T, false>), dim3(batch), dim3(threads), 0, s, a, stage, lam_level);
     return;
   }
   dim3 grid((unsigned)a.part_blocks, (unsigned)batch);
