@@ -79,6 +79,7 @@ _SIGNATURES = {
     "swe_step_f32": ([_G, _C, _P, _P, _P, _P, _I, _D, _I, _P, _P, _I, _P], _I),
     "swe_step_f64": ([_G, _C, _P, _P, _P, _P, _I, _D, _I, _P, _P, _I, _P], _I),
     "swe_step_stage_f32": ([_G, _C, _I, _P, _P, _P, _P, _I, _D, _I, _P, _P, _I, _P], _I),
+    "swe_step_noalloc": ([_G, _C, _I, _P, _P, _P, _P, _I, _D, _P, _P, _I, _P], _I),
     "swe_step_record_f32": ([_G, _C, _P, _P, _P, _P, _I, _D, _I, _P, _P, _P, _P, _I, _P], _I),
     "swe_step_record_f64": ([_G, _C, _P, _P, _P, _P, _I, _D, _I, _P, _P, _P, _P, _I, _P], _I),
     "swe_run_begin": ([_P, _D, _D, _I, _P], _I),

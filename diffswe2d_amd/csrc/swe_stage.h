@@ -54,6 +54,7 @@ template <class T> struct StageArgs {
   int step_index;        // step index inside a device-driven run
   double max_dt;         // host-driven steps: explicit max_dt
   T* dt_series;          // optional per-step dt record
+  T* dt_out;             // stage 2, optional: the step's dt, written whether or not it is usable
   int record_wave;       // stage 1: accumulate wave speeds
 };
 

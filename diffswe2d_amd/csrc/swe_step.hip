@@ -16,7 +16,10 @@
 #include <algorithm>
 #include <math.h>
 #include <stdint.h>
+#include <stdio.h>
 #include <string.h>
+
+#include <mutex>
 
 #include "swe_common.h"
 #include "swe_host.h"
@@ -77,6 +80,7 @@ __global__ __launch_bounds__(256) void stage_kernel(StageArgs<T> a) {
         }
       }
       if (a.dt_series != nullptr && ok) a.dt_series[a.step_index] = dt;
+      if (a.dt_out != nullptr) *a.dt_out = dt;
     }
     if (!ok) return;
     half_dt = T(0.5) * dt;
@@ -391,7 +395,8 @@ template <class T> int apply_forcing(StageArgs<T>& a, const swe_grid* g, const s
 template <class T>
 int step_impl(const swe_grid* g, const swe_config* c, int stages, const T* U, const T* bed, const T* rough,
               const T* level, int level_stride, double max_dt, int parity, T* U_out, void* ws, int rows_hint,
-              void* stream, T* k1_rec = nullptr, T* k2_rec = nullptr, const swe_forcing* fz = nullptr) {
+              void* stream, T* k1_rec = nullptr, T* k2_rec = nullptr, const swe_forcing* fz = nullptr,
+              T* dt_out = nullptr) {
   int e;
   if ((e = check_grid(g)) || (e = check_cfg(c)) || (e = check_level(level, level_stride, g, c))) return e;
   if (stages < 1 || stages > 3) return fail(SWE_ERR_ARG, "stages must be 1, 2 or 3");
@@ -412,6 +417,7 @@ int step_impl(const swe_grid* g, const swe_config* c, int stages, const T* U, co
   a.max_dt = max_dt;
   a.ctl = w.ctl;
   a.clock = 0;
+  a.dt_out = dt_out;
   if ((stages & 1) && (e = launch_stage<T, 1>(a, s))) return e;
   if (stages & 2) {
     StageArgs<T> b2 = a;
@@ -510,6 +516,76 @@ int advance_chunked(const swe_grid* g, const swe_config* c, const T* bed, const 
   return advance_impl<T>(g, c, bed, rough, level, 0, U_a, U_b, first_step, n_steps, nullptr, dt_series, capacity,
                          nullptr, 0, nullptr, nullptr, 0, ws, rows_hint, stream, tape_k1, tape_k2, tape_first, gauges,
                          n_gauges, obs, chunk_series, true);
+}
+
+}  // namespace
+
+// ------------------------------------------------------------------ step without allocation
+// Scratch of swe_step_noalloc: the workspace layout of one step (control block,
+// wave-speed banks, first-stage tendency) in a fixed array that each device holds
+// from module load to the end of the process.  Host code never uses the array's
+// name as an address: the device address comes from hipGetSymbolAddress, once
+// per device, on the first call with that device current.
+constexpr size_t kNoAllocScratchBytes = (size_t)64 << 20;
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+namespace {
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {};
+
+int scratch_address(hipStream_t stream, void** out) {
+  int dev = -1;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return fail(SWE_ERR_HIP, hipGetErrorString(e));
+  if (dev < 0 || dev >= kMaxDevices) return fail(SWE_ERR_ARG, "device index beyond the scratch table");
+  if (stream != nullptr) {
+    int sdev = -1;
+    e = hipStreamGetDevice(stream, &sdev);
+    if (e != hipSuccess) return fail(SWE_ERR_HIP, hipGetErrorString(e));
+    if (sdev != dev) return fail(SWE_ERR_ARG, "the stream belongs to a device other than the current one");
+  }
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (g_scratch_ptr[dev] == nullptr) {
+    void* p = nullptr;
+    e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || p == nullptr)
+      return fail(SWE_ERR_HIP, e != hipSuccess ? hipGetErrorString(e) : "no address for the step scratch");
+    g_scratch_ptr[dev] = p;
+  }
+  *out = g_scratch_ptr[dev];
+  return 0;
+}
+
+template <class T>
+int step_noalloc_impl(const swe_grid* g, const swe_config* c, const T* U, const T* bed, const T* rough,
+                      const T* level, int level_stride, double max_dt, T* U_out, T* dt_out, int rows_hint,
+                      void* stream) {
+  int e;
+  if ((e = check_grid(g))) return e;
+  if (!dt_out) return fail(SWE_ERR_ARG, "dt_out is required");
+  const size_t need = workspace_bytes<T>(g);
+  if (need > kNoAllocScratchBytes) {
+    char msg[200];
+    snprintf(msg, sizeof(msg),
+             "batch * ny * nx = %lld cells need %zu scratch bytes; the fixed scratch holds %zu (%lld cells)",
+             (long long)g->batch * g->ny * g->nx, need, kNoAllocScratchBytes,
+             (long long)((kNoAllocScratchBytes - kCtrlBytes - kAccRegion) / (3 * sizeof(T))));
+    return fail(SWE_ERR_CAPACITY, msg);
+  }
+  hipStream_t s = (hipStream_t)stream;
+  void* ws = nullptr;
+  if ((e = scratch_address(s, &ws))) return e;
+  // the step accumulates its wave speeds into bank 0, which the previous call left
+  // holding its own: clear it with a kernel (no memset node in a captured graph)
+  WsView<T> w = ws_view<T>(ws);
+  hipLaunchKernelGGL(zero_words, dim3(1), dim3(256), 0, s, (uint32_t*)w.acc,
+                     (int64_t)(2 * kSlots * sizeof(typename Bits<T>::U) / 4));
+  hipError_t he = hipGetLastError();
+  if (he != hipSuccess) return fail(SWE_ERR_HIP, hipGetErrorString(he));
+  return step_impl<T>(g, c, 3, U, bed, rough, level, level_stride, max_dt, 0, U_out, ws, rows_hint, stream, nullptr,
+                      nullptr, nullptr, dt_out);
 }
 
 }  // namespace
@@ -737,6 +813,20 @@ int swe_advance_tape_f64(const swe_grid* g, const swe_config* c, const double* b
   return advance_impl<double>(g, c, bed, rough, level, level_stride, tape_U, nullptr, first_step, n_steps, hmax,
                               dt_series, dt_capacity, nullptr, 0, nullptr, nullptr, 0, ws, rows_hint, stream, tape_k1,
                               tape_k2, tape_first);
+}
+
+int swe_step_noalloc(const swe_grid* g, const swe_config* c, int dtype_bytes, const void* U, const void* bed,
+                     const void* rough, const void* level, int level_stride, double max_dt, void* U_out,
+                     void* dt_out, int rows_hint, void* stream) {
+  if (dtype_bytes == 4)
+    return step_noalloc_impl<float>(g, c, (const float*)U, (const float*)bed, (const float*)rough,
+                                    (const float*)level, level_stride, max_dt, (float*)U_out, (float*)dt_out,
+                                    rows_hint, stream);
+  if (dtype_bytes == 8)
+    return step_noalloc_impl<double>(g, c, (const double*)U, (const double*)bed, (const double*)rough,
+                                     (const double*)level, level_stride, max_dt, (double*)U_out, (double*)dt_out,
+                                     rows_hint, stream);
+  return fail(SWE_ERR_ARG, "dtype_bytes must be 4 or 8");
 }
 
 int swe_water_volume_f32(const swe_grid* g, const float* U, double* volume, void* ws, void* stream) {

@@ -143,6 +143,24 @@ int swe_step_stage_f32(const swe_grid* grid, const swe_config* cfg, int stage, c
                        const float* roughness, const float* level, int level_stride, double max_dt, int parity,
                        float* U_out, void* ws, int rows_hint, void* stream);
 
+/* swe_step_f32 / swe_step_f64 (dtype_bytes 4 or 8) without a workspace argument
+ * and without any allocation.  No counterpart in the reference: it is for
+ * callers that cannot allocate -- code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc.  Its scratch (control block,
+ * wave-speed banks, first-stage tendency) is a fixed 64 MiB device array that
+ * every GPU the library runs on holds until the process ends; the call rejects
+ * a grid whose batch * ny * nx exceeds 5,592,128 cells (FP32) or 2,796,064
+ * cells (FP64) with SWE_ERR_CAPACITY.  dt_out (a device pointer to one value of
+ * the run's dtype) receives the step's dt whether or not it is usable; as in
+ * swe_step_*, U_out is left untouched when dt is non-finite or non-positive, and
+ * the caller raises as solver.py:627-632 does.  Every call on one GPU shares
+ * that GPU's scratch, so calls that overlap on one GPU (several streams or host
+ * threads at once) race; calls in stream order do not.  The current device must
+ * be the stream's. */
+int swe_step_noalloc(const swe_grid* grid, const swe_config* cfg, int dtype_bytes, const void* U, const void* bed,
+                     const void* roughness, const void* level, int level_stride, double max_dt, void* U_out,
+                     void* dt_out, int rows_hint, void* stream);
+
 /* swe_step_*, additionally recording the two stage tendencies k1 and k2
  * (each [batch, 3, ny, nx]) for the adjoint's checkpoint replay. */
 int swe_step_record_f32(const swe_grid* grid, const swe_config* cfg, const float* U, const float* bed,
