@@ -114,6 +114,18 @@ def nearest_pairs(gauges, target, k=8, mask=None):
     return order[:min(k, int(torch.isfinite(mis).sum()))], mis
 
 
+def judge_start(setup, raw_ws, m_prior, accept=0.5):
+    """The plain solver's verdict on a surrogate start.
+
+    Runs the start through the solver (``setup.misfit``) and keeps it only when
+    its gauge misfit is below ``accept`` times the prior's (``m_prior``, the
+    misfit of the prior start); otherwise the prior is the start.  Returns
+    (start raw controls, fallback taken, the surrogate start's misfit)."""
+    m_ws = setup.misfit(raw_ws)
+    fallback = not (m_ws < accept * m_prior)
+    return (setup.prior_raw if fallback else raw_ws), fallback, m_ws
+
+
 class InversionSetup:
     """Coastal run-up domain, truth waveform and gauge observations of the inversion."""
 
@@ -181,10 +193,8 @@ def run(device, n_scenarios=256, n_controls=64, n_chunks=450, grids=scenarios.RU
     starts = torch.cat((setup.prior_raw[None], inversion.raw_from_controls(ctrl_t, -0.02, 0.05)))
     info = {}
     raw_ws = surrogate_start(net, setup.basis, setup.target, starts, dx=setup.dx, info=info)
-    m_ws = setup.misfit(raw_ws)
     m_prior = hist_p[0][2]
-    fallback = not (m_ws < accept * m_prior)
-    start = setup.prior_raw if fallback else raw_ws
+    start, fallback, m_ws = judge_start(setup, raw_ws, m_prior, accept)
     t_start = time.perf_counter() - t2
     raw_w, hist_w = inversion.adam_invert(setup.problem, setup.basis, setup.target, start, epochs=epochs_plain,
                                           lr=0.01 if not fallback else 0.02, target_misfit=bar,

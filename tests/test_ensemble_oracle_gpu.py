@@ -1,0 +1,131 @@
+"""The ensemble workload at its real sizes (272 x 172, 544 x 344, 1088 x 688,
+FP32, per-scenario dt) against the oracle, and the full 256-scenario launch.
+
+The reference batches members on one grid with one shared dt
+(src/solver.py:528-529), so there is no reference ensemble to compare with:
+each scenario is compared with the oracle run of that scenario alone, driven
+like the reference's inversion driver (one duration-driven forward per chunk at
+the chunk's boundary level, then h + z at the gauges,
+Examples/inverse/.../bc_inversion_train.py:332-410).
+
+* One scenario of each grid, 4 chunks of 0.05 s with a west level raised to
+  start a bore: step count equal to the oracle's, final state rel-L2 <= 1e-5 with
+  identical wet/dry masks, gauge samples (h + z) within 1e-5 of the largest
+  depth.
+* 256 scenarios cycling through the three grids, 20 chunks: every scenario
+  finishes, states are finite with h >= 0 and zero momentum on dry cells, each
+  scenario's last gauge sample is h + z of its final state, the finer grids
+  take more steps (their own dt), and three sampled scenarios equal batch-1
+  SWE2D.forward runs bit for bit.
+"""
+import numpy as np
+import pytest
+import torch
+
+import diffswe2d_amd as swe
+from diffswe2d_amd import scenarios
+from diffswe2d_amd.ensemble import Ensemble
+from oracle import swe_oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def cfg():
+    return swe.SWEConfig(boundary_left="water_level", boundary_right="wall", boundary_top="wall",
+                         boundary_bottom="wall")
+
+
+def relerr(a, b):
+    a = torch.as_tensor(a, dtype=torch.float64).cpu()
+    b = torch.as_tensor(b, dtype=torch.float64).cpu()
+    return float(torch.linalg.norm(a - b) / max(float(torch.linalg.norm(b)), 1e-300))
+
+
+def bore(sc, n_chunks):
+    """The seeded waveform, raised by 1.5 cm so a bore enters within the first chunks."""
+    for s in sc:
+        s["levels"] = (np.asarray(s["levels"][:n_chunks]) + 0.015).astype(np.float32).astype(np.float64)
+    return sc
+
+
+def oracle_scenario(s):
+    c = cfg()
+    bed = torch.tensor(s["bed"], dtype=torch.float32)[None, None]
+    rough = torch.tensor(s["rough"], dtype=torch.float32)[None, None]
+    U = torch.tensor(s["U0"], dtype=torch.float32)[None]
+    dts, obs = [], []
+    with torch.no_grad():
+        for k, lvl in enumerate(s["levels"]):
+            U, _ = O.forward(U, bed, rough, c, s["dx"], s["dy"], s["chunk"], start_time=k * s["chunk"],
+                             boundary_level=float(lvl), dt_log=dts)
+            obs.append([float(U[0, 0, iy, ix]) + float(bed[0, 0, iy, ix]) for iy, ix in s["gauges"]])
+    return U[0], np.array(obs), len(dts)
+
+
+def test_ensemble_three_grids_match_oracle(cuda_device):
+    n_chunks = 4
+    sc, _ = scenarios.ensemble_scenarios(n=3, seed=5, n_chunks=450)
+    sc = bore(sc, n_chunks)
+    assert [s["bed"].shape for s in sc] == [(172, 272), (344, 544), (688, 1088)]
+    ens = Ensemble(cfg(), sc, cuda_device, graph=True, steps_per_graph=16)
+    ens.run()
+    res = ens.results()
+    for i, s in enumerate(sc):
+        U_ref, obs_ref, steps_ref = oracle_scenario(s)
+        r = res[i]
+        e = relerr(r["U"], U_ref)
+        g = r["gauges"].cpu().double().numpy()
+        # h + z nearly cancels at the gauges (|h + z| ~ 1 cm over 13 cm of water):
+        # gauge errors are measured against the depth scale of the scenario
+        ge = float(np.max(np.abs(g - obs_ref))) / float(U_ref[0].abs().max())
+        wet, wet_ref = (r["U"][0] > 1e-6).cpu(), U_ref[0] > 1e-6
+        mism = int((wet != wet_ref).sum())
+        moved = float((U_ref[0] - torch.tensor(s["U0"][0], dtype=torch.float32)).abs().max())
+        print(f"scenario {i} {s['bed'].shape}: steps {r['steps']} (oracle {steps_ref}), rel-L2 {e:.3e}, "
+              f"gauge max rel {ge:.3e}, mask mismatches {mism}, max |dh| {moved:.3e}", flush=True)
+        assert r["done"] and r["steps"] == steps_ref
+        assert moved > 1e-3
+        assert e <= 1e-5
+        assert ge <= 1e-5
+        assert mism == 0
+
+
+def test_ensemble_256_scenarios(cuda_device):
+    n_chunks = 20
+    sc, _ = scenarios.ensemble_scenarios(n=256, seed=5, n_chunks=450)
+    sc = bore(sc, n_chunks)
+    ens = Ensemble(cfg(), sc, cuda_device)
+    global_steps = ens.run()
+    res = ens.results()
+    steps = {}
+    for i, (s, r) in enumerate(zip(sc, res)):
+        U = r["U"]
+        assert r["done"], i
+        assert bool(torch.isfinite(U).all()), i
+        h = U[0]
+        assert bool((h >= 0).all()), i
+        dry = h <= 1e-6
+        assert int((U[1:, dry] != 0).sum()) == 0, i
+        bed = torch.tensor(s["bed"], dtype=torch.float32, device=cuda_device)
+        last = torch.stack([h[iy, ix] + bed[iy, ix] for iy, ix in s["gauges"]])
+        assert torch.equal(r["gauges"][-1], last), i
+        assert bool(torch.isfinite(r["gauges"]).all()), i
+        steps.setdefault(s["bed"].shape, []).append(r["steps"])
+    mean = {k: float(np.mean(v)) for k, v in steps.items()}
+    print(f"256 scenarios, {global_steps} global steps; mean steps per grid {mean}", flush=True)
+    s172, s344, s688 = mean[(172, 272)], mean[(344, 544)], mean[(688, 1088)]
+    assert 1.6 < s344 / s172 < 2.4 and 1.6 < s688 / s344 < 2.4
+    # three sampled scenarios, one per grid, against batch-1 forward calls
+    c = cfg()
+    for i in (0, 100, 254):
+        s = sc[i]
+        m = swe.SWE2D(torch.tensor(s["bed"], dtype=torch.float32)[None, None],
+                      torch.tensor(s["rough"], dtype=torch.float32)[None, None], s["dx"], s["dy"], config=c,
+                      train_dem=False).to(cuda_device)
+        U = torch.tensor(s["U0"], dtype=torch.float32, device=cuda_device)[None]
+        n = 0
+        for k, lvl in enumerate(s["levels"]):
+            U, _ = m(U, t_end=s["chunk"], start_time=k * s["chunk"], boundary_level=float(lvl))
+            n += len(m.last_dt_series)
+        assert n == res[i]["steps"], i
+        assert torch.equal(U[0], res[i]["U"]), f"scenario {i}"

@@ -44,3 +44,25 @@ def test_surrogate_warm_start_beats_prior(cuda_device):
     # and the adjoint improves on it further
     raw, hist = inversion.adam_invert(setup.problem, setup.basis, setup.target, raw_ws, epochs=3, lr=0.02)
     assert hist[-1][2] <= m_ws * 1.01
+
+
+def test_untrained_surrogate_falls_back_to_the_prior(cuda_device):
+    """A surrogate with random weights proposes a start the plain solver rejects: the
+    inversion starts from the prior, as if there were no surrogate."""
+    n_ctrl, n_chunks = 16, 60
+    setup = warmstart.InversionSetup(cuda_device, n_controls=n_ctrl, n_chunks=n_chunks, nx=136, ny=86, dx=0.04)
+    net = Surrogate(default_specs(channels=32, dilations=(1, 2, 4), n_cond=1), device=cuda_device, seed=3)
+    m_prior = setup.misfit(setup.prior_raw)
+    raw_ws = warmstart.surrogate_start(net, setup.basis, setup.target, setup.prior_raw, iters=50, dx=setup.dx)
+    start, fallback, m_ws = warmstart.judge_start(setup, raw_ws, m_prior, accept=0.5)
+    print(f"untrained surrogate start: plain-solver misfit {m_ws:.4e} against the prior's {m_prior:.4e}; "
+          f"fallback {fallback}")
+    assert fallback
+    assert torch.equal(start, setup.prior_raw)
+    # the inversion from the fallback start is the inversion from the prior
+    _, hist_a = inversion.adam_invert(setup.problem, setup.basis, setup.target, start, epochs=2, lr=0.02)
+    _, hist_b = inversion.adam_invert(setup.problem, setup.basis, setup.target, setup.prior_raw, epochs=2, lr=0.02)
+    assert [h[2] for h in hist_a] == [h[2] for h in hist_b]
+    # a start is kept only when it beats the prior by the factor: accept = 0 always falls back
+    _, fb0, _ = warmstart.judge_start(setup, setup.prior_raw, m_prior, accept=0.0)
+    assert fb0
