@@ -49,7 +49,9 @@ BC_TYPES = ("transmissive", "periodic", "constant", "water_level", "wall", "abso
 # SQRT_MODE "torch" keeps the reference's own rounding; "ieee" gives the
 # correctly rounded square root that IEEE hardware (and the GPU) computes:
 # float32 via one rounding of the float64 root, float64 via numpy's hardware
-# sqrt, both with the sqrt derivative grad / (2 * result) for autograd.
+# sqrt, both with the sqrt derivative grad / (2 * result) for autograd.  "ieee"
+# also rounds the float32 friction power h^(1/3) once from its FP64 value
+# (_pow_third), as the GPU does.
 SQRT_MODE = "torch"
 
 
@@ -343,10 +345,21 @@ def _speed(h, hu, hv, dry_depth, epsilon):
     return wet, safe, torch.where(moving, _sqrt(s2), zero)
 
 
+def _pow_third(x):
+    """x ** (1/3) as the reference computes it (torch pow), or, with SQRT_MODE "ieee",
+    rounded once from the FP64 power for float32 x (the correctly rounded value the
+    GPU's friction power gives; torch's CPU float32 pow is one ulp off for some
+    arguments).  The exponent is the Python double 1/3, cast to x's dtype by pow."""
+    if SQRT_MODE == "ieee" and x.dtype == torch.float32:
+        third = float(torch.tensor(1.0 / 3.0, dtype=torch.float32))
+        return x.double().pow(third).float()
+    return x.pow(1.0 / 3.0)
+
+
 def manning_coefficient(h, n, epsilon=1e-12):
     """Cf = 9.81 n^2 / h^(1/3) (friction.py:62-66; gravity hard-coded as 9.81)."""
     safe = torch.clamp(h, min=depth_floor(h.dtype, epsilon, 0.0))
-    return 9.81 * torch.clamp(n, min=0.0).square() / safe.pow(1.0 / 3.0)
+    return 9.81 * torch.clamp(n, min=0.0).square() / _pow_third(safe)
 
 
 def roughness_length_coefficient(h, z0, epsilon=1e-12):

@@ -335,6 +335,77 @@ def write_runup(ref, out, nx=136, ny=86, dx=0.04, n_chunks=40):
     np.savez_compressed(out / "runup.npz", **data)
 
 
+def write_runup_full(ref, out, name, n_chunks=450, mask_every=50):
+    """The full coastal run-up: 544 x 344 at dx = 0.01 m, 450 chunks of 0.05 s (22.5 s).
+
+    Driven like the reduced run (one duration-driven forward per chunk at the
+    chunk's boundary level, solver.py:719-779).  Stores every step's dt, the six
+    gauge depths at every chunk end, the wet/dry mask every ``mask_every``
+    chunks and the final state.  One precision per file
+    (``runup_full_f32.npz`` / ``runup_full_f64.npz``) so both can be written at
+    once by two processes.
+    """
+    dt = dict(f32=torch.float32, f64=torch.float64)[name]
+    sc = scenarios.coastal_runup()
+    cfg = ref.config.SWEConfig(boundary_left="water_level", boundary_right="wall",
+                               boundary_top="wall", boundary_bottom="wall")
+    st = RefStep(ref, cfg, torch.tensor(sc["bed"], dtype=dt), torch.tensor(sc["rough"], dtype=dt),
+                 sc["dx"], sc["dy"])
+    s = torch.tensor(sc["U"], dtype=dt)
+    dts, chunk_steps, gauge, masks = [], [], [], []
+    t0 = time.time()
+    for k in range(n_chunks):
+        s = st.advance(s, sc["sample_dt"], torch.tensor(sc["chunk_levels"][k], dtype=dt), dts)
+        chunk_steps.append(len(dts))
+        gauge.append([float(s[0, 0, iy, ix]) for iy, ix in sc["gauges"]])
+        if (k + 1) % mask_every == 0:
+            masks.append((s[0, 0] > cfg.dry_depth).numpy())
+            print(f"run-up full {name}: chunk {k + 1}/{n_chunks}, {len(dts)} steps, "
+                  f"{time.time() - t0:.0f} s", flush=True)
+    data = dict(levels=sc["chunk_levels"][:n_chunks], gauges=np.array(sc["gauges"]),
+                mask_every=np.array(mask_every), dt=np.array(dts), chunk_steps=np.array(chunk_steps),
+                gauge=np.array(gauge), mask=np.packbits(np.stack(masks)), U=s.numpy())
+    np.savez_compressed(out / f"runup_full_{name}.npz", **data)
+
+
+def write_runup_full_ieee(ref, out, n_chunks=450, mask_every=50):
+    """The full run-up in FP32 through the oracle (oracle/swe_oracle.py) with
+    correctly rounded square roots, stored like ``runup_full_f32.npz``.
+
+    torch's CPU float32 sqrt is one ulp off for ~0.7 % of arguments and its
+    float32 pow (Manning's h^(1/3)) for ~1.4 %; the GPU rounds both correctly.
+    Over the run-up's ~11,000 steps that difference moves the reference's FP32
+    trajectory by a few 1e-7 and, at the moving shoreline, can flip a cell
+    between wet and dry.  This run separates the two effects: it is the
+    reference's algorithm (the oracle, pinned to the reference's modules by the
+    fixtures above) with correctly rounded square roots and friction powers."""
+    from oracle import swe_oracle as O
+
+    sc = scenarios.coastal_runup()
+    cfg = ref.config.SWEConfig(boundary_left="water_level", boundary_right="wall",
+                               boundary_top="wall", boundary_bottom="wall")
+    bed = torch.tensor(sc["bed"], dtype=torch.float32)
+    rough = torch.tensor(sc["rough"], dtype=torch.float32)
+    s = torch.tensor(sc["U"], dtype=torch.float32)
+    dts, chunk_steps, gauge, masks = [], [], [], []
+    O.SQRT_MODE = "ieee"
+    t0 = time.time()
+    with torch.no_grad():
+        for k in range(n_chunks):
+            s, _ = O.forward(s, bed, rough, cfg, sc["dx"], sc["dy"], sc["sample_dt"], start_time=k * sc["sample_dt"],
+                             boundary_level=float(sc["chunk_levels"][k]), dt_log=dts)
+            chunk_steps.append(len(dts))
+            gauge.append([float(s[0, 0, iy, ix]) for iy, ix in sc["gauges"]])
+            if (k + 1) % mask_every == 0:
+                masks.append((s[0, 0] > cfg.dry_depth).numpy())
+                print(f"run-up full (oracle, IEEE sqrt): chunk {k + 1}/{n_chunks}, {len(dts)} steps, "
+                      f"{time.time() - t0:.0f} s", flush=True)
+    data = dict(levels=sc["chunk_levels"][:n_chunks], gauges=np.array(sc["gauges"]),
+                mask_every=np.array(mask_every), dt=np.array(dts), chunk_steps=np.array(chunk_steps),
+                gauge=np.array(gauge), mask=np.packbits(np.stack(masks)), U=s.numpy())
+    np.savez_compressed(out / "runup_full_ieee_f32.npz", **data)
+
+
 def bspline_basis(sample_times, n_controls, degree=3):
     """Clamped B-spline basis by the Cox-de Boor recursion.
 
@@ -412,14 +483,20 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--reference", default="/root/reference/src")
     ap.add_argument("--only", default="")
+    ap.add_argument("--threads", type=int, default=8)
     a = ap.parse_args()
-    torch.set_num_threads(8)
+    torch.set_num_threads(a.threads)
     ref = load_reference(Path(a.reference))
     writers = dict(limiter=write_limiter, hll=write_hll, recon=write_recon, ghost=write_ghost,
                    friction=write_friction, steps=write_steps, dam_break=write_dam_break,
-                   runup=write_runup, gradient=write_gradient)
+                   runup=write_runup, gradient=write_gradient,
+                   runup_full_f32=lambda r, o: write_runup_full(r, o, "f32"),
+                   runup_full_f64=lambda r, o: write_runup_full(r, o, "f64"),
+                   runup_full_ieee=write_runup_full_ieee)
+    # the full run-up takes 10-15 minutes per run: only on request
+    on_request = ("runup_full_f32", "runup_full_f64", "runup_full_ieee")
     for name, fn in writers.items():
-        if a.only and name not in a.only.split(","):
+        if (a.only and name not in a.only.split(",")) or (not a.only and name in on_request):
             continue
         t0 = time.time()
         fn(ref, HERE)
