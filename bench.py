@@ -43,6 +43,8 @@ def parse():
     ap.add_argument("--cpu-side", type=int, default=2048, help="crop side of the CPU-baseline sample")
     ap.add_argument("--cpu-steps", type=int, default=50)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-fast", action="store_true",
+                    help="skip the leg with the library's tolerance-bounded FP32 arithmetic")
     ap.add_argument("--adjoint-steps", type=int, default=10,
                     help="steps of the adjoint leg (taped forward, reverse sweep, checkpointed backward); 0 to skip")
     ap.add_argument("--copy-probe", action="store_true",
@@ -147,6 +149,71 @@ def basin_model(n, device):
     return model, U, sc, cfg
 
 
+def timed_run(model, U, warmup, steps, fast):
+    """Device-driven run of warmup + steps steps from U; times the last `steps`.
+
+    Returns (seconds, dt series, final state).  fast: the library's
+    tolerance-bounded FP32 arithmetic instead of the reference's rounding."""
+    import torch
+
+    from diffswe2d_amd import _lib
+
+    lib = _lib.load()
+    bed, rough = model._fields(U)
+    ws = model._workspace(U, bed, rough, fast=fast)
+    lvl, stride = model._level_arg(0.0, U)
+    stream = _lib.stream_ptr(U.device)
+    Ua, Ub = U.clone(), torch.empty_like(U)
+    dts = torch.empty(steps + warmup + 8, dtype=torch.float32, device=U.device)
+
+    def run(first, count):
+        _lib.check(lib.swe_advance_f32(ws["grid"], ws["cfg"], _lib.ptr(bed), _lib.ptr(rough), _lib.ptr(lvl),
+                                       stride, _lib.ptr(Ua), _lib.ptr(Ub), first, count, None, _lib.ptr(dts),
+                                       dts.numel(), None, 0, None, None, 0, _lib.ptr(ws["buf"]),
+                                       int(model.rows_hint), stream), "advance")
+
+    _lib.check(lib.swe_run_begin(_lib.ptr(ws["buf"]), 1e30, -1.0, 4, stream), "run_begin")
+    run(0, warmup)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    run(warmup, steps)
+    torch.cuda.synchronize()
+    sec = time.perf_counter() - t0
+    ctl = _lib.read_control(ws["buf"])
+    from diffswe2d_amd.solver import sync_parity
+
+    sync_parity(ws, ctl)
+    ws["cfg"].arithmetic = 0
+    if ctl.status != 0 or ctl.steps != warmup + steps:
+        raise RuntimeError(f"run failed: status {ctl.status}, steps {ctl.steps}")
+    return sec, dts[:warmup + steps], (Ua if (warmup + steps) % 2 == 0 else Ub)
+
+
+def valu_mix_probe_seconds(dev, blocks=8192, iters=512, reps=3):
+    """Best launch time of the library's VALU mix probe (the stage kernels' x-sweep arithmetic on
+    register values); its VALU instruction count per launch comes from a committed SQ_INSTS_VALU run."""
+    import torch
+
+    from diffswe2d_amd import _lib
+
+    lib = _lib.load()
+    out = torch.empty(blocks * 256, dtype=torch.float32, device=dev)
+    stream = _lib.stream_ptr(dev)
+    _lib.check(lib.swe_valu_mix_probe(_lib.ptr(out), blocks, 8, stream), "valu_mix_probe")
+    best = None
+    for _ in range(reps):
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        _lib.check(lib.swe_valu_mix_probe(_lib.ptr(out), blocks, iters, stream), "valu_mix_probe")
+        e1.record()
+        torch.cuda.synchronize(dev)
+        ms = e0.elapsed_time(e1)
+        best = ms if best is None else min(best, ms)
+    if not bool(torch.isfinite(out).all()):
+        raise RuntimeError("VALU mix probe produced non-finite values")
+    return best * 1e-3
+
+
 def valu_peak(dev, blocks=8192, iters=4096, reps=3):
     """Measured FP32 VALU issue ceiling (lane-instructions / s) from the library's probe kernel."""
     import torch
@@ -170,7 +237,7 @@ def valu_peak(dev, blocks=8192, iters=4096, reps=3):
     return wave_instr * 64 / (best * 1e-3)
 
 
-def time_kernels(model, U, steps, rows):
+def time_kernels(model, U, steps, rows, fast=False):
     """Average duration of each launch, bracketed by events on torch's current stream."""
     import torch
 
@@ -178,7 +245,7 @@ def time_kernels(model, U, steps, rows):
 
     lib = _lib.load()
     bed, rough = model._fields(U)
-    ws = model._workspace(U, bed, rough)
+    ws = model._workspace(U, bed, rough, fast=fast)
     lvl, stride = model._level_arg(0.0, U)
     stream = _lib.stream_ptr(U.device)
     A, Bf = U.clone(), torch.empty_like(U)
@@ -198,6 +265,7 @@ def time_kernels(model, U, steps, rows):
         ws["parity"] ^= 1
         A, Bf = Bf, A
     torch.cuda.synchronize()
+    ws["cfg"].arithmetic = 0
     s1 = sum(e[0].elapsed_time(e[1]) for e in evs) / steps
     s2 = sum(e[1].elapsed_time(e[2]) for e in evs) / steps
     return s1, s2
@@ -343,35 +411,33 @@ def main():
 
     from diffswe2d_amd import _lib
 
-    lib = _lib.load()
-    bed, rough = model._fields(U)
-    ws = model._workspace(U, bed, rough)
-    lvl, stride = model._level_arg(0.0, U)
-    stream = _lib.stream_ptr(dev)
-    Ua, Ub = U.clone(), torch.empty_like(U)
-    dts = torch.empty(a.steps + a.warmup + 8, dtype=torch.float32, device=dev)
-
-    def run(first, count):
-        _lib.check(lib.swe_advance_f32(ws["grid"], ws["cfg"], _lib.ptr(bed), _lib.ptr(rough), _lib.ptr(lvl),
-                                       stride, _lib.ptr(Ua), _lib.ptr(Ub), first, count, None, _lib.ptr(dts),
-                                       dts.numel(), None, 0, None, None, 0, _lib.ptr(ws["buf"]),
-                                       int(model.rows_hint), stream), "advance")
-
-    _lib.check(lib.swe_run_begin(_lib.ptr(ws["buf"]), 1e30, -1.0, 4, stream), "run_begin")
-    run(0, a.warmup)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    run(a.warmup, a.steps)
-    torch.cuda.synchronize()
-    sec = time.perf_counter() - t0
-    ctl = _lib.read_control(ws["buf"])
-    if ctl.status != 0 or ctl.steps != a.warmup + a.steps:
-        raise RuntimeError(f"run failed: status {ctl.status}, steps {ctl.steps}")
+    sec, dts, U_end = timed_run(model, U, a.warmup, a.steps, fast=False)
     ms = sec / a.steps * 1e3
     gcells = cells * a.steps / sec / 1e9
     dt_mean = float(dts[a.warmup:a.warmup + a.steps].double().mean())
     print(f"# {a.steps} steps in {sec:.3f} s: {ms:.3f} ms/step, {gcells:.2f} Gcell/s, mean dt {dt_mean:.4f} s",
           file=sys.stderr, flush=True)
+
+    fast = None
+    if not a.no_fast:
+        # the same run with the tolerance-bounded arithmetic, and its distance from the
+        # reference-rounding run above after the same steps
+        sec_f, dts_f, U_f = timed_run(model, U, a.warmup, a.steps, fast=True)
+        f1, f2 = time_kernels(model, U, a.kernel_steps, int(model.rows_hint), fast=True)
+        d = dts.double()
+        diff = float(torch.linalg.norm((U_f - U_end).double()) / torch.linalg.norm(U_end.double()))
+        fast = {"value": round(cells * a.steps / sec_f / 1e9, 4), "unit": "Gcell-updates/s",
+                "ms_per_step": round(sec_f / a.steps * 1e3, 4),
+                "kernel_ms": {"stage1": round(f1, 4), "stage2": round(f2, 4)},
+                "rel_l2_vs_reference_rounding": diff,
+                "dt_max_rel_vs_reference_rounding": float(((dts_f.double() - d).abs() / d).max()),
+                "wet_dry_mismatch_vs_reference_rounding": int(((U_f[:, 0] > 1e-6) != (U_end[:, 0] > 1e-6)).sum()),
+                "note": "SWE2D.arithmetic = 'fast' (swe_config.arithmetic = SWE_ARITH_FAST): division with one "
+                        "correction, hardware sqrt, h^(1/3) from log2/exp2, no velocity re-division; "
+                        "opt-in, forward runs without gradients only; parity bars in tests/test_fast_arith_gpu.py"}
+        print(f"# fast arithmetic: {fast['ms_per_step']:.3f} ms/step, {fast['value']:.2f} Gcell/s, events stage1 "
+              f"{f1:.3f} stage2 {f2:.3f} ms; rel-L2 vs reference rounding after {a.warmup + a.steps} steps "
+              f"{diff:.3e}", file=sys.stderr, flush=True)
 
     s1, s2 = time_kernels(model, U, a.kernel_steps, int(model.rows_hint))
     # One cell-update moves 32 algorithmic bytes (read h, hu, hv, z, n; write h, hu, hv).
@@ -415,17 +481,25 @@ def main():
         # second roof: VALU issue.  Instructions per cell from the committed SQ_INSTS_VALU run of
         # this workload, the ceiling from the probe kernel measured here.
         per_cell = prof["stage1"]["valu_lane_instr_per_cell"] + prof["stage2"]["valu_lane_instr_per_cell"]
-        peak = valu_peak(dev)
         got = per_cell * cells / (step_kernel_ms * 1e-3)
+        mix = prof.get("valu_mix_probe")
+        if mix is not None:
+            # ceiling: the issue rate of the kernels' own arithmetic without memory (the mix probe)
+            sec_probe = valu_mix_probe_seconds(dev, mix["blocks"], mix["iters"])
+            peak = mix["SQ_INSTS_VALU"] * 64 / sec_probe
+            src = ("profiles/basin8192_fp32_traffic.json (SQ_INSTS_VALU of the stage kernels and of "
+                   "swe_valu_mix_probe); the probe timed here")
+            note = ("peak is the issue rate of swe_valu_mix_probe: the stage kernels' x-sweep arithmetic "
+                    "(divisions, square roots, limiter, HLL, DPP shifts) on register values, no memory")
+        else:
+            peak = valu_peak(dev)
+            src = "profiles/basin8192_fp32_traffic.json (SQ_INSTS_VALU); peak from swe_valu_probe"
+            note = "peak is the measured issue rate of a plain FP32 mul/add stream"
         valu = {"bound": "valu", "achieved": round(got / 1e12, 2), "peak": round(peak / 1e12, 2),
                 "unit": "T lane-instr/s", "frac": round(got / peak, 4),
-                "lane_instr_per_cell_update": round(per_cell, 1),
-                "source": "profiles/basin8192_fp32_traffic.json (SQ_INSTS_VALU); peak from swe_valu_probe",
-                "note": "peak is the measured issue rate of a plain FP32 mul/add stream; the stage kernels "
-                        "issue above it (frac > 1) with reciprocal / square-root instructions overlapping the "
-                        "rest, and their time follows their instruction count"}
-        print(f"# VALU: {got / 1e12:.2f} of {peak / 1e12:.2f} T lane-instr/s measured by the probe "
-              f"({got / peak:.2f})", file=sys.stderr, flush=True)
+                "lane_instr_per_cell_update": round(per_cell, 1), "source": src, "note": note}
+        print(f"# VALU: {got / 1e12:.2f} of {peak / 1e12:.2f} T lane-instr/s ({got / peak:.2f})", file=sys.stderr,
+              flush=True)
         traffic_src = "profiles/basin8192_fp32_traffic.json (FETCH_SIZE x2.0 + WRITE_SIZE x1.0, copy-calibrated)"
 
     inv = None
@@ -455,6 +529,7 @@ def main():
                      "kernel": "step (stage1 + stage2)", "algorithmic_bytes_per_cell_update": 32,
                      "kernel_ms": round(step_kernel_ms, 4)},
         "roofline_valu": valu,
+        "arith_fast": fast,
         "kernel_ms": {"stage1": round(s1, 4), "stage2": round(s2, 4)},
         "adjoint": adj,
         "inversion": inv,

@@ -34,7 +34,7 @@ class SweConfigC(ctypes.Structure):
                 ("constant_value", ctypes.c_double), ("boundary", ctypes.c_int32 * 4),
                 ("water_slope_reset", ctypes.c_int32), ("positivity_limiter", ctypes.c_int32),
                 ("apply_friction", ctypes.c_int32), ("friction_model", ctypes.c_int32),
-                ("grad_bed", ctypes.c_int32), ("pad", ctypes.c_int32)]
+                ("grad_bed", ctypes.c_int32), ("arithmetic", ctypes.c_int32)]
 
 
 class SweScenario(ctypes.Structure):
@@ -117,6 +117,7 @@ _SIGNATURES = {
     "swe_segment_adjoint_chunked_f64": ([_G, _C, _P, _P, _P, _P, _P, _P, _P, _I, _I, _P, _I, _P, _P, _P, _P, _P, _P,
                                          _P, _I, _P], _I),
     "swe_valu_probe": ([_P, _I, _I, _P], _I),
+    "swe_valu_mix_probe": ([_P, _I, _I, _P], _I),
     "swe_division_probe": ([_P, _P, ctypes.c_longlong, _P, _P, _P, _P], _I),
     "swe_sqrt_probe": ([_P, ctypes.c_longlong, _P, _P, _P], _I),
     "swe_pow_probe": ([_P, ctypes.c_longlong, ctypes.c_float, _P, _P, _P], _I),

@@ -11,6 +11,7 @@
 
 #include "../../include/swe2d_hip.h"
 #include "swe_fetch.h"
+#include "swe_host.h"
 #include "swe_math.h"
 
 namespace {
@@ -28,6 +29,39 @@ __global__ __launch_bounds__(256) void valu_probe_kernel(float* out, int iters, 
     a7 = a7 * x + y;
   }
   out[(int64_t)blockIdx.x * blockDim.x + threadIdx.x] = a0 + a1 + a2 + a3 + a4 + a5 + a6 + a7;
+}
+
+// Mix probe: the arithmetic of one x-sweep cell of the stage kernels -- the
+// primitives of a cell from its state (two divisions), the limited increments,
+// both face states, the HLL face flux with its square roots and star divisions,
+// the flux differences over dx -- on register values only, each iteration on
+// new wet, subcritical states (so the same branches as the basin's ocean cells
+// are taken) with the lane shifts of the kernels.  Its issue rate is the VALU
+// ceiling of the stage kernels' own instruction mix (transcendental estimates,
+// FMAs, compares, selects, DPP moves); SQ_INSTS_VALU per launch gives the count.
+__global__ __launch_bounds__(256) void valu_mix_probe_kernel(float* out, int iters, swe::SweConsts<float> k) {
+  using namespace swe;
+  const int lane = threadIdx.x & 63;
+  float acc = 0.0f;
+  for (int i = 0; i < iters; ++i) {
+    const float h_raw = 1.0f + 0.01f * (float)(lane & 7) + 1.0e-3f * (float)(i & 15);
+    const float hu = 0.05f + 1.0e-3f * (float)((lane + i) & 31);
+    const float hv = -0.02f + 5.0e-4f * (float)((lane * 3 + i) & 31);
+    const float z = -1.0f - 0.01f * (float)((lane * 5 + i) & 15);
+    float h, u, v, eta;
+    primitive_cell(h_raw, hu, hv, z, k, h, u, v, eta);
+    Cell<float> cx{h, u, v, eta, z};
+    Cell<float> cm = shup(cx), cp = shdn(cx);
+    Incr<float> xi = increments(cm, cx, cp, k, true);
+    Side<float> right_of_face = side_minus(cx, xi, k);
+    Side<float> left_of_face = side_plus(cx, xi, k);
+    Side<float> nxt = shdn(right_of_face);
+    FaceFlux<float> F = face_flux(left_of_face, cx.h, cx.z, nxt, cp.h, cp.z, k);
+    float pfh = shup(F.fh), pft = shup(F.ft), pfn = shup(F.fn - F.sR);
+    acc += div_by(pfh - F.fh, k.ddx) + div_by(pfn - (F.fn - F.sL), k.ddx) + div_by(pft - F.ft, k.ddx) +
+           F.speed;
+  }
+  out[(int64_t)blockIdx.x * blockDim.x + threadIdx.x] = acc;
 }
 
 __global__ __launch_bounds__(256) void division_probe_kernel(const float* a, const float* b, int64_t n, float* q_rn,
@@ -94,6 +128,19 @@ extern "C" int swe_division_probe(const float* a, const float* b, long long n, f
   if (!a || !b || !q_rn || !q_by || !q_ieee || n <= 0) return SWE_ERR_ARG;
   hipLaunchKernelGGL(division_probe_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, (hipStream_t)stream, a,
                      b, (int64_t)n, q_rn, q_by, q_ieee);
+  return hipGetLastError() == hipSuccess ? 0 : SWE_ERR_HIP;
+}
+
+extern "C" int swe_valu_mix_probe(float* out, int blocks, int iters, void* stream) {
+  if (!out || blocks <= 0 || iters <= 0) return SWE_ERR_ARG;
+  swe_config c{};
+  c.gravity = 9.81;
+  c.dry_depth = 1e-6;
+  c.epsilon = 1e-12;
+  c.cfl = 0.5;
+  c.limiter_theta = 1.3;
+  const swe::SweConsts<float> k = swe::host_consts<float>(&c, 100.0, 100.0);
+  hipLaunchKernelGGL(valu_mix_probe_kernel, dim3(blocks), dim3(256), 0, (hipStream_t)stream, out, iters, k);
   return hipGetLastError() == hipSuccess ? 0 : SWE_ERR_HIP;
 }
 

@@ -21,7 +21,7 @@ namespace swe {
 // division sequence; divisions that recompute forward values use the forward's
 // div_rn, so every branch of the reverse is taken exactly as in the forward.
 // FP64 and host code divide.
-template <class T> SWE_HD inline T gdiv(T a, T b) {
+template <class T> SWE_HD T gdiv(T a, T b) {
 #if defined(__HIP_DEVICE_COMPILE__)
   if constexpr (sizeof(T) == 4) return a * __builtin_amdgcn_rcpf(b);
 #endif

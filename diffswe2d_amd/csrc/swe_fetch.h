@@ -57,7 +57,7 @@ template <class T> struct Prim {
 };
 
 // state of interior cell (r, c) as stage STAGE sees it
-template <class T, int STAGE, class Args>
+template <class T, int STAGE, class Args, class A = RefRound>
 __device__ inline void load_state(const Args& a, int64_t base, int r, int c, T half_dt, T& h, T& hu,
                                   T& hv) {
   int64_t i = base + (int64_t)r * a.f.nx + c;
@@ -69,7 +69,7 @@ __device__ inline void load_state(const Args& a, int64_t base, int r, int c, T h
     T ch = h + half_dt * a.k1[i];
     T cu = hu + half_dt * a.k1[i + P];
     T cv = hv + half_dt * a.k1[i + 2 * P];
-    clean_and_limit(h, hu, hv, ch, cu, cv, a.k, a.positivity != 0);
+    clean_and_limit<T, A>(h, hu, hv, ch, cu, cv, a.k, a.positivity != 0);
     h = ch;
     hu = cu;
     hv = cv;
@@ -105,7 +105,7 @@ __device__ inline CellLoads<T> fetch_loads(const Args& a, int b, int r, int c) {
 
 // the cell from its loads: the predictor clean-up (stage 2), the ghost rule,
 // the primitives -- the arithmetic of load_state and fetch, in their order
-template <class T, int STAGE, class Args>
+template <class T, int STAGE, class Args, class A = RefRound>
 __device__ inline Prim<T> fetch_finish(const Args& a, int b, const CellLoads<T>& w, T half_dt) {
   const Fields<T>& f = a.f;
   const GhostMap& g = w.g;
@@ -114,7 +114,7 @@ __device__ inline Prim<T> fetch_finish(const Args& a, int b, const CellLoads<T>&
     T ch = h + half_dt * w.k1h;
     T cu = hu + half_dt * w.k1u;
     T cv = hv + half_dt * w.k1v;
-    clean_and_limit(h, hu, hv, ch, cu, cv, a.k, a.positivity != 0);
+    clean_and_limit<T, A>(h, hu, hv, ch, cu, cv, a.k, a.positivity != 0);
     h = ch;
     hu = cu;
     hv = cv;
@@ -124,7 +124,7 @@ __device__ inline Prim<T> fetch_finish(const Args& a, int b, const CellLoads<T>&
     T mirror = T(0);
     if (g.kind == BC_WALL) {
       T mh, mu, mv;
-      load_state<T, STAGE, Args>(a, (int64_t)b * 3 * f.plane, g.mr, g.mc, half_dt, mh, mu, mv);
+      load_state<T, STAGE, Args, A>(a, (int64_t)b * 3 * f.plane, g.mr, g.mc, half_dt, mh, mu, mv);
       mirror = (g.side == LEFT || g.side == RIGHT) ? mu : mv;
     }
     T level = (g.kind == BC_WATER_LEVEL || g.kind == BC_ABSORBING) ? f.level[b * f.level_stride] : T(0);
@@ -132,7 +132,7 @@ __device__ inline Prim<T> fetch_finish(const Args& a, int b, const CellLoads<T>&
   }
   Prim<T> p;
   T eta;
-  primitive_cell(h, hu, hv, z, a.k, p.h, p.u, p.v, eta);
+  primitive_cell<T, A>(h, hu, hv, z, a.k, p.h, p.u, p.v, eta);
   p.z = z;
   p.hr = h;
   p.qu = hu;

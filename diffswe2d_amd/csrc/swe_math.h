@@ -120,6 +120,42 @@ template <class T> struct SweConsts {
   T third;        // 1.0 / 3.0                           friction.py:66
 };
 
+// ---------------------------------------------------------------- arithmetic policies
+// RefRound: the reference's rounding -- IEEE division and square root, Manning's
+// h^third correctly rounded -- so results match the reference's eager FP32 ops.
+// FastRound (FP32 forward runs, an explicit opt-in; FP64 ignores it): division
+// from the reciprocal estimate with one remainder correction, the hardware square
+// root, h^third from the log2 / exp2 estimates, and face velocities taken from the
+// reconstruction instead of being re-divided from the face momenta (the
+// reference's (h_cn * u) / h_cn).  Each is within a few ulps of the reference's
+// value; a run's distance from the reference is then a tolerance, not zero.
+struct RefRound {};
+struct FastRound {};
+
+template <class A> inline constexpr bool is_fast = false;
+template <> inline constexpr bool is_fast<FastRound> = true;
+
+template <class A, class T> SWE_HD T adiv(T a, T b) { return div_rn(a, b); }
+template <class A, class T> SWE_HD T adiv_by(T a, const Divisor<T>& d) { return div_by(a, d); }
+template <class A, class T> SWE_HD T asqrt(T x) { return sqrt_rn(x); }
+
+template <> SWE_HD float adiv<FastRound, float>(float a, float b) {
+  const float r = rcp_estimate(b);
+  const float q = a * r;
+  return __builtin_fmaf(__builtin_fmaf(-b, q, a), r, q);
+}
+template <> SWE_HD float adiv_by<FastRound, float>(float a, const Divisor<float>& d) {
+  const float q = a * d.r;
+  return __builtin_fmaf(__builtin_fmaf(-d.b, q, a), d.r, q);
+}
+template <> SWE_HD float asqrt<FastRound, float>(float x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return __builtin_amdgcn_sqrtf(x);
+#else
+  return sqrtf(x);
+#endif
+}
+
 // ---------------------------------------------------------------- limiter (numerics.py:4-27)
 // minmod of (theta*backward, centred, theta*forward): the smallest magnitude when
 // all three share a strict sign, else 0 -- equal to sign(a)*min(|a|,|b|,|c|).
@@ -146,13 +182,13 @@ template <class T> struct Cell {
   T h, un, ut, eta, z;
 };
 
-template <class T>
+template <class T, class A = RefRound>
 SWE_HD void primitive_cell(T h_raw, T hu, T hv, T z, const SweConsts<T>& k, T& h, T& u, T& v, T& eta) {
   h = clamp_lo(h_raw, T(0));
   bool wet = h > k.dry;
   T safe = clamp_lo(h, k.floor);
-  u = wet ? div_rn(hu, safe) : T(0);
-  v = wet ? div_rn(hv, safe) : T(0);
+  u = wet ? adiv<A>(hu, safe) : T(0);
+  v = wet ? adiv<A>(hv, safe) : T(0);
   eta = h + z;
 }
 
@@ -163,7 +199,7 @@ template <class T> struct Incr {
   T dh, deta, dz, dun, dut, q;
 };
 
-template <class T>
+template <class T, class A = RefRound>
 SWE_HD Incr<T> increments(const Cell<T>& m, const Cell<T>& c, const Cell<T>& p, const SweConsts<T>& k,
                           bool water_slope_reset) {
   Incr<T> r;
@@ -180,7 +216,20 @@ SWE_HD Incr<T> increments(const Cell<T>& m, const Cell<T>& c, const Cell<T>& p, 
     bool reset = (r.deta * r.deta > r.dz * r.dz) && ((bed_minus > m.eta) || (bed_plus > p.eta));
     if (reset) r.deta = r.dh + r.dz;
   }
-  r.q = div_rn(T(0.5) * r.dh, clamp_lo(c.h, k.floor));
+  r.q = adiv<A>(T(0.5) * r.dh, clamp_lo(c.h, k.floor));
+  return r;
+}
+
+// increments() of a cell whose stencil (m, c, p) is all dry with depth exactly 0
+// (h = u = v = 0, eta = z): the same values without the arithmetic on zeros --
+// dh, dun, dut and q are +0, deta = dz (eta = z, so the water-slope reset never
+// fires).
+template <class T> SWE_HD Incr<T> increments_dry(const Cell<T>& m, const Cell<T>& c, const Cell<T>& p,
+                                                 const SweConsts<T>& k) {
+  Incr<T> r;
+  r.dz = limited_increment(m.z, c.z, p.z, k.theta);
+  r.deta = r.dz;
+  r.dh = r.dun = r.dut = r.q = T(0);
   return r;
 }
 
@@ -225,7 +274,7 @@ template <class T> struct FaceFlux {
   T fh, fn, ft, sL, sR, speed;
 };
 
-template <class T>
+template <class T, class A = RefRound>
 SWE_HD FaceFlux<T> face_flux(const Side<T>& L, T hL_cell, T zL_cell, const Side<T>& R, T hR_cell, T zR_cell,
                              const SweConsts<T>& k) {
   T z_face = tmin(tmax(L.z, R.z), tmin(L.eta, R.eta));
@@ -242,12 +291,21 @@ SWE_HD FaceFlux<T> face_flux(const Side<T>& L, T hL_cell, T zL_cell, const Side<
   T hL = clamp_lo(hcL, T(0)), hR = clamp_lo(hcR, T(0));
   bool wetL = hL > k.dry, wetR = hR > k.dry;
   T sfL = clamp_lo(hL, k.floor), sfR = clamp_lo(hR, k.floor);
-  T nL = wetL ? div_rn(qnL, sfL) : T(0);
-  T tL = wetL ? div_rn(qtL, sfL) : T(0);
-  T cL = wetL ? sqrt_rn(k.g * sfL) : T(0);
-  T nR = wetR ? div_rn(qnR, sfR) : T(0);
-  T tR = wetR ? div_rn(qtR, sfR) : T(0);
-  T cR = wetR ? sqrt_rn(k.g * sfR) : T(0);
+  T nL, tL, nR, tR;
+  if constexpr (is_fast<A> && sizeof(T) == 4) {
+    // the face velocity is the reconstructed one: (h_cn * u) / h_cn without the re-division
+    nL = wetL ? L.un : T(0);
+    tL = wetL ? L.ut : T(0);
+    nR = wetR ? R.un : T(0);
+    tR = wetR ? R.ut : T(0);
+  } else {
+    nL = wetL ? div_rn(qnL, sfL) : T(0);
+    tL = wetL ? div_rn(qtL, sfL) : T(0);
+    nR = wetR ? div_rn(qnR, sfR) : T(0);
+    tR = wetR ? div_rn(qtR, sfR) : T(0);
+  }
+  T cL = wetL ? asqrt<A>(k.g * sfL) : T(0);
+  T cR = wetR ? asqrt<A>(k.g * sfR) : T(0);
 
   T um = (T(0.5) * (nL + nR) + cL) - cR;
   T cm = T(0.5) * (cL + cR) + T(0.25) * (nL - nR);
@@ -261,8 +319,8 @@ SWE_HD FaceFlux<T> face_flux(const Side<T>& L, T hL_cell, T zL_cell, const Side<
   T pR = hR * (nR * nR + k.half_g * hR);
   T den = clamp_lo(SR - SL, k.hll_floor);
   T slsr = SL * SR;
-  T m_star = div_rn((SR * mL - SL * mR) + slsr * (hR - hL), den);
-  T p_star = div_rn((SR * pL - SL * pR) + slsr * (hR * nR - hL * nL), den);
+  T m_star = adiv<A>((SR * mL - SL * mR) + slsr * (hR - hL), den);
+  T p_star = adiv<A>((SR * pL - SL * pR) + slsr * (hR * nR - hL * nL), den);
 
   T fh, fn;
   if (SL >= T(0)) {
@@ -292,12 +350,12 @@ SWE_HD FaceFlux<T> face_flux(const Side<T>& L, T hL_cell, T zL_cell, const Side<
 
 // ---------------------------------------------------------------- positivity and friction
 // Positivity limiter and dry-cell momentum removal (solver.py:547-570).
-template <class T>
+template <class T, class A = RefRound>
 SWE_HD void clean_and_limit(T oh, T ou, T ov, T& ch, T& cu, T& cv, const SweConsts<T>& k, bool positivity) {
   if (positivity) {
     T dh = ch - oh, du = cu - ou, dv = cv - ov;
     T theta = T(1);
-    if (dh < T(0)) theta = clamp_hi(div_rn(oh, clamp_lo(-dh, k.eps)), T(1));
+    if (dh < T(0)) theta = clamp_hi(adiv<A>(oh, clamp_lo(-dh, k.eps)), T(1));
     ch = oh + theta * dh;
     cu = ou + theta * du;
     cv = ov + theta * dv;
@@ -349,36 +407,47 @@ template <> SWE_HD float pow_third<float>(float x, float third) {
 
 template <> SWE_HD double pow_third<double>(double x, double third) { return pow(x, third); }
 
+// h^third for the friction coefficient under policy A: FastRound takes the
+// exp2(third * log2 h) estimate directly (FP32; within a few ulps).
+template <class A, class T> SWE_HD T apow_third(T x, T third) { return pow_third(x, third); }
+template <> SWE_HD float apow_third<FastRound, float>(float x, float third) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return __builtin_amdgcn_exp2f(__builtin_amdgcn_logf(x) * third);
+#else
+  return exp2f(log2f(x) * third);
+#endif
+}
+
 template <class T> SWE_HD T log_t(T x);
 template <> SWE_HD float log_t<float>(float x) { return logf(x); }
 template <> SWE_HD double log_t<double>(double x) { return log(x); }
 
 // model 0: Manning n; model 1: roughness length z0 (BG_Flood smart friction, friction.py:41-59)
-template <class T>
+template <class T, class A = RefRound>
 SWE_HD void friction(T& h_io, T& hu, T& hv, T coef, T dt, const SweConsts<T>& k, int model) {
   T h = clamp_lo(h_io, T(0));
   bool wet = h > k.dry;
   T safe = clamp_lo(h, k.floor);
-  T u = div_rn(wet ? hu : T(0), safe);
-  T v = div_rn(wet ? hv : T(0), safe);
+  T u = adiv<A>(wet ? hu : T(0), safe);
+  T v = adiv<A>(wet ? hv : T(0), safe);
   T s2 = u * u + v * v;
   bool moving = wet && (s2 > T(0));
-  T speed = moving ? sqrt_rn(clamp_lo(s2, k.fric_floor2)) : T(0);
+  T speed = moving ? asqrt<A>(clamp_lo(s2, k.fric_floor2)) : T(0);
   T cf;
   if (model == 0) {
     T sh = clamp_lo(h, k.fric_floor);
     T n = clamp_lo(coef, T(0));
-    cf = div_rn(k.manning_g * (n * n), pow_third(sh, k.third));
+    cf = adiv<A>(k.manning_g * (n * n), apow_third<A>(sh, k.third));
   } else {
     T fl = k.fric_floor;
-    T ratio = div_rn(clamp_lo(h, fl), clamp_lo(coef, fl));
-    T shallow = div_rn(T(1), clamp_lo(T(0.46) * ratio, fl));
+    T ratio = adiv<A>(clamp_lo(h, fl), clamp_lo(coef, fl));
+    T shallow = adiv<A>(T(1), clamp_lo(T(0.46) * ratio, fl));
     T r = clamp_lo(ratio, fl);
-    T lt = T(2.5) * ((log_t(r) - T(1)) + div_rn(T(1), r) * T(1.359));  // scalar / tensor = reciprocal * scalar
-    T deep = div_rn(T(1), clamp_lo(lt * lt, fl));
+    T lt = T(2.5) * ((log_t(r) - T(1)) + adiv<A>(T(1), r) * T(1.359));  // scalar / tensor = reciprocal * scalar
+    T deep = adiv<A>(T(1), clamp_lo(lt * lt, fl));
     cf = ratio < T(2.718281828459045) ? shallow : deep;
   }
-  T factor = div_rn(T(1), T(1) + div_rn((cf * speed) * dt, safe));
+  T factor = adiv<A>(T(1), T(1) + adiv<A>((cf * speed) * dt, safe));
   h_io = h;
   hu = wet ? hu * factor : T(0);
   hv = wet ? hv * factor : T(0);

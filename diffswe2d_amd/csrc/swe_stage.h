@@ -56,6 +56,7 @@ template <class T> struct StageArgs {
   T* dt_series;          // optional per-step dt record
   T* dt_out;             // stage 2, optional: the step's dt, written whether or not it is usable
   int record_wave;       // stage 1: accumulate wave speeds
+  int fast;              // FP32: FastRound arithmetic (see swe_math.h); forward runs only
 };
 
 // dt of the step from the accumulated maxima (solver.py:618-632):
@@ -121,7 +122,7 @@ template <class T> __device__ inline T wave_max(T v) {
 // One wavefront's strip: columns [strip*60 - 2, strip*60 + 62), rows of segment
 // seg, member b.  Writes the stage output for its 60 x S cells and returns the
 // wave-local maxima of the x- and y-face speeds it evaluated.
-template <class T, int STAGE>
+template <class T, int STAGE, class A = RefRound>
 __device__ inline void stage_strip(const StageArgs<T>& a, int strip, int seg, int b, T dt, T half_dt, T& ax_out,
                                    T& ay_out, bool rain_rec = false, RainAt<T> rec = RainAt<T>{0, 0, 0, T(0)}) {
   const int lane = threadIdx.x & (kWave - 1);
@@ -143,6 +144,12 @@ __device__ inline void stage_strip(const StageArgs<T>& a, int strip, int seg, in
   const bool xface_ok = lane >= 1 && lane <= kWave - 3 && col >= -1 && col <= f.nx - 1;
   const bool ycol_ok = col >= 0 && col < f.nx;
 
+  // Rows whose 64 cells all have depth exactly 0 (after the ghost rules): every
+  // face between two such cells carries zero flux and zero speed, and every such
+  // cell has zero increments of h, u, v (see increments_dry), so those parts of the
+  // sweep are skipped with the values they would produce (+0 fluxes, exactly).
+  // dry1..dry3: rows r-1..r-3 of the iteration; wave-uniform.
+  bool dry1 = false, dry2 = false, dry3 = false;
   for (int it = 0; it < rows + 4; ++it) {
     const int r = y0 - kHalo + it;
     // row r's loads go out first; the x sweep of row o = r - 2 does not need them
@@ -157,37 +164,40 @@ __device__ inline void stage_strip(const StageArgs<T>& a, int strip, int seg, in
       if (a.friction) coef = f.rough[(int64_t)b * f.rough_bstride + cell];
     }
     T rx_h = T(0), rx_u = T(0), rx_v = T(0);
-    if (it >= 4) {
+    if (it >= 4 && !dry2) {
       // x sweep of row o (its state is P2)
       Cell<T> cx = P2.x();
       Cell<T> cm = shup<kDpp>(cx), cp = shdn<kDpp>(cx);
-      Incr<T> xi = increments(cm, cx, cp, k, a.wsr != 0);
+      Incr<T> xi = increments<T, A>(cm, cx, cp, k, a.wsr != 0);
       Side<T> right_of_face = side_minus(cx, xi, k);
       Side<T> left_of_face = side_plus(cx, xi, k);
       Side<T> nxt = shdn<kDpp>(right_of_face);
-      FaceFlux<T> F = face_flux(left_of_face, cx.h, cx.z, nxt, cp.h, cp.z, k);
+      FaceFlux<T> F = face_flux<T, A>(left_of_face, cx.h, cx.z, nxt, cp.h, cp.z, k);
       if (xface_ok) ax = tmax(ax, F.speed);
       T pfh = shup<kDpp>(F.fh), pft = shup<kDpp>(F.ft), pfn = shup<kDpp>(F.fn - F.sR);
-      rx_h = div_by(pfh - F.fh, k.ddx);
-      rx_u = div_by(pfn - (F.fn - F.sL), k.ddx);
-      rx_v = div_by(pft - F.ft, k.ddx);
+      rx_h = adiv_by<A>(pfh - F.fh, k.ddx);
+      rx_u = adiv_by<A>(pfn - (F.fn - F.sL), k.ddx);
+      rx_v = adiv_by<A>(pft - F.ft, k.ddx);
     }
-    Prim<T> Pn = fetch_finish<T, STAGE>(a, b, raw, half_dt);
+    Prim<T> Pn = fetch_finish<T, STAGE, StageArgs<T>, A>(a, b, raw, half_dt);
+    const bool dry0 = __all(Pn.h == T(0));
     if (it >= 2) {
       // y increments of row r-1 and its two face states
       Cell<T> c1 = P1.y();
-      Incr<T> yi = increments(P2.y(), c1, Pn.y(), k, a.wsr != 0);
+      Incr<T> yi = (dry2 && dry1 && dry0) ? increments_dry(P2.y(), c1, Pn.y(), k)
+                                          : increments<T, A>(P2.y(), c1, Pn.y(), k, a.wsr != 0);
       Side<T> below = side_minus(c1, yi, k);
       if (it >= 3) {
         // face (r-2 | r-1)
-        FaceFlux<T> G = face_flux(carryL, P2.h, P2.z, below, P1.h, P1.z, k);
+        FaceFlux<T> G{};
+        if (!(dry3 && dry2 && dry1 && dry0)) G = face_flux<T, A>(carryL, P2.h, P2.z, below, P1.h, P1.z, k);
         if (ycol_ok && r - 2 >= -1 && r - 2 <= f.ny - 1) ay = tmax(ay, G.speed);
         if (it >= 4) {
           // output row o = r - 2
           const int o = r - 2;
-          T ry_h = div_by(gfh - G.fh, k.ddy);
-          T ry_u = div_by(gft - G.ft, k.ddy);
-          T ry_v = div_by(gfn - (G.fn - G.sL), k.ddy);
+          T ry_h = adiv_by<A>(gfh - G.fh, k.ddy);
+          T ry_u = adiv_by<A>(gft - G.ft, k.ddy);
+          T ry_v = adiv_by<A>(gfn - (G.fn - G.sL), k.ddy);
           T th = rx_h + ry_h, tu = rx_u + ry_u, tv = rx_v + ry_v;
           if (lane_out) {
             int64_t i = mbase + (int64_t)o * f.nx + col;
@@ -206,8 +216,10 @@ __device__ inline void stage_strip(const StageArgs<T>& a, int strip, int seg, in
                 a.k2_out[i + 2 * f.plane] = tv;
               }
               T nh = oh + dt * th, nu = ou + dt * tu, nv = ov + dt * tv;
-              clean_and_limit(oh, ou, ov, nh, nu, nv, k, a.positivity != 0);
-              if (a.friction) swe::friction(nh, nu, nv, coef, dt, k, a.friction_model);
+              clean_and_limit<T, A>(oh, ou, ov, nh, nu, nv, k, a.positivity != 0);
+              // friction leaves a dry cell as the clean-up left it (h >= 0, zero
+              // momentum): skipped when no cell of the row is wet
+              if (a.friction && __any(nh > k.dry)) swe::friction<T, A>(nh, nu, nv, coef, dt, k, a.friction_model);
               a.out[i] = nh;
               a.out[i + f.plane] = nu;
               a.out[i + 2 * f.plane] = nv;
@@ -226,6 +238,9 @@ __device__ inline void stage_strip(const StageArgs<T>& a, int strip, int seg, in
     }
     P2 = P1;
     P1 = Pn;
+    dry3 = dry2;
+    dry2 = dry1;
+    dry1 = dry0;
   }
 
   ax_out = ax;

@@ -29,7 +29,7 @@
 
 namespace swe {
 
-template <class T, int STAGE>
+template <class T, int STAGE, class A>
 __global__ __launch_bounds__(256) void stage_kernel(StageArgs<T> a) {
   const int lane = threadIdx.x & (kWave - 1);
   const int wid = blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6);
@@ -95,7 +95,7 @@ __global__ __launch_bounds__(256) void stage_kernel(StageArgs<T> a) {
     double ts = (double)(T)t;
     rec = rain_at<T>(a.rain_times, a.rain_nt, STAGE == 1 ? ts : ts + 0.5 * (double)dt);
   }
-  stage_strip<T, STAGE>(a, strip, seg, b, dt, half_dt, ax, ay, a.rain_rates != nullptr, rec);
+  stage_strip<T, STAGE, A>(a, strip, seg, b, dt, half_dt, ax, ay, a.rain_rates != nullptr, rec);
   if (STAGE == 1 && a.record_wave) {
     ax = wave_max(ax);
     ay = wave_max(ay);
@@ -285,6 +285,8 @@ int check_cfg(const swe_config* c) {
   if (!(c->gravity > 0.0) || !(c->epsilon > 0.0) || !(c->dry_depth >= 0.0))
     return fail(SWE_ERR_ARG, "Require gravity > 0, epsilon > 0, dry_depth >= 0.");
   if (c->friction_model < 0 || c->friction_model > 1) return fail(SWE_ERR_ARG, "unknown friction model");
+  if (c->arithmetic != SWE_ARITH_REFERENCE && c->arithmetic != SWE_ARITH_FAST)
+    return fail(SWE_ERR_ARG, "unknown arithmetic mode");
   return 0;
 }
 
@@ -330,6 +332,7 @@ StageArgs<T> make_args(const swe_grid* g, const swe_config* c, const T* bed, con
   a.positivity = c->positivity_limiter;
   a.friction = c->apply_friction;
   a.friction_model = c->friction_model;
+  a.fast = c->arithmetic == SWE_ARITH_FAST;
   return a;
 }
 
@@ -339,7 +342,14 @@ template <class T> dim3 stage_grid(const StageArgs<T>& a) {
 }
 
 template <class T, int STAGE> int launch_stage(const StageArgs<T>& a, hipStream_t s) {
-  hipLaunchKernelGGL((stage_kernel<T, STAGE>), stage_grid(a), dim3(kWave * kWavesPerBlock), 0, s, a);
+  if constexpr (sizeof(T) == 4) {
+    if (a.fast)
+      hipLaunchKernelGGL((stage_kernel<T, STAGE, FastRound>), stage_grid(a), dim3(kWave * kWavesPerBlock), 0, s, a);
+    else
+      hipLaunchKernelGGL((stage_kernel<T, STAGE, RefRound>), stage_grid(a), dim3(kWave * kWavesPerBlock), 0, s, a);
+  } else {
+    hipLaunchKernelGGL((stage_kernel<T, STAGE, RefRound>), stage_grid(a), dim3(kWave * kWavesPerBlock), 0, s, a);
+  }
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return fail(SWE_ERR_HIP, hipGetErrorString(e));
   return 0;
@@ -409,6 +419,7 @@ int step_impl(const swe_grid* g, const swe_config* c, int stages, const T* U, co
   StageArgs<T> a = make_args<T>(g, c, bed, rough, level, level_stride, rows_hint);
   if ((e = apply_forcing<T>(a, g, fz))) return e;
   T* k1buf = k1_rec != nullptr ? k1_rec : w.k1;
+  if (k1_rec != nullptr) a.fast = 0;  // a recorded step feeds the adjoint, which replays the reference rounding
   a.U = U;
   a.out = k1buf;
   a.acc = w.acc + p * 2 * kSlots;
@@ -456,6 +467,7 @@ int advance_impl(const swe_grid* g, const swe_config* c, const T* bed, const T* 
   a.clock = 1;
   a.hmax = hmax;
   a.dt_series = dt_series;
+  if (tape_k1 != nullptr || chunked) a.fast = 0;  // taped / chunked runs feed the adjoint
   const size_t state = (size_t)g->batch * 3 * (size_t)g->ny * g->nx;
   for (int k = first_step; k < first_step + n_steps; ++k) {
     int p = k & 1;

@@ -96,6 +96,10 @@ class SWE2D(nn.Module):
         self._ws = {}
         self._field_cache = {}
         self.rows_hint = 0  # wavefront rows per strip; 0 = library heuristic / autotuned value
+        # "reference": the reference's rounding (default).  "fast": FP32 forward runs
+        # without gradients use the library's tolerance-bounded arithmetic
+        # (swe_config.arithmetic); differentiable paths always use the reference's.
+        self.arithmetic = "reference"
         self.adjoint_checkpoint_every = 64  # K: forward checkpoints for the adjoint replay
 
     # ------------------------------------------------------------------ properties
@@ -281,7 +285,12 @@ class SWE2D(nn.Module):
 
         return autotune.lookup(U.device, U.shape[-2], U.shape[-1], U.shape[0], U.dtype)
 
-    def _workspace(self, U, bed, rough):
+    def _fast(self):
+        if self.arithmetic not in ("reference", "fast"):
+            raise ValueError(f"arithmetic must be 'reference' or 'fast', got {self.arithmetic!r}")
+        return self.arithmetic == "fast"
+
+    def _workspace(self, U, bed, rough, fast=False):
         B, _, ny, nx = U.shape
         key = (U.device, U.dtype, B, ny, nx, bed.shape[0], rough.shape[0])
         ws = self._ws.get(key)
@@ -298,6 +307,7 @@ class SWE2D(nn.Module):
             self._ws[key] = ws
         ws["cfg"] = _lib.make_config(self.cfg)
         ws["cfg"].grad_bed = int(getattr(self, "_grad_bed", False))  # set by the adjoint during a backward
+        ws["cfg"].arithmetic = 1 if fast else 0
         return ws
 
     def _level_arg(self, level, U):
@@ -379,7 +389,7 @@ class SWE2D(nn.Module):
             return rhs
         U = U.detach().contiguous()
         bed, rough = self._fields(U)
-        ws = self._workspace(U, bed, rough)
+        ws = self._workspace(U, bed, rough, fast=self._fast())
         lvl, stride = self._level_arg(level, U)
         lib = _lib.load()
         rhs = torch.empty_like(U)
@@ -463,7 +473,7 @@ class SWE2D(nn.Module):
 
     def _step_raw(self, U, level, max_dt_value, forcing=None):
         bed, rough = self._fields(U)
-        ws = self._workspace(U, bed, rough)
+        ws = self._workspace(U, bed, rough, fast=self._fast())
         lvl, stride = self._level_arg(level, U)
         lib = _lib.load()
         out = torch.empty_like(U)
@@ -557,7 +567,7 @@ class SWE2D(nn.Module):
         """Device-driven loop: dt, elapsed time and stopping live on the GPU."""
         lib = _lib.load()
         bed, rough = self._fields(U0)
-        ws = self._workspace(U0, bed, rough)
+        ws = self._workspace(U0, bed, rough, fast=self._fast())
         level = self._resolve_boundary_level(absolute_start, U0, boundary_level)
         lvl, stride = self._level_arg(level, U0)
         stream = _lib.stream_ptr(U0.device)

@@ -73,8 +73,18 @@ typedef struct swe_config {
   int32_t friction_model;
   int32_t grad_bed;               /* adjoint calls: also accumulate the bed adjoint (see
                                      swe_adjoint_bed_grad_offset); 0 elsewhere */
-  int32_t pad;
+  int32_t arithmetic;             /* SWE_ARITH_REFERENCE (default): the reference's rounding.
+                                     SWE_ARITH_FAST: FP32 forward steps with division from
+                                     the reciprocal estimate and one correction, hardware
+                                     square root, h^(1/3) from log2 / exp2, face velocities
+                                     without re-division -- each within a few ulps, so
+                                     results agree with the reference to a tolerance, not
+                                     bit for bit.  Ignored by FP64, by steps that record
+                                     k1 / k2 or a tape, by chunked runs and by the adjoint. */
 } swe_config;
+
+/* swe_config.arithmetic */
+enum { SWE_ARITH_REFERENCE = 0, SWE_ARITH_FAST = 1 };
 
 /* Device control block at the head of the workspace (read back by the host). */
 /* Source terms of a step (solver.py:520-525, 609-641): a rainfall record
@@ -353,6 +363,14 @@ int swe_segment_adjoint_chunked_f64(const swe_grid* grid, const swe_config* cfg,
  * (blocks * 256 floats).  Timing it gives the FP32 VALU ceiling against which
  * the stage kernels' instruction counts are priced. */
 int swe_valu_probe(float* out, int blocks, int iters, void* stream);
+
+/* VALU mix probe (no counterpart in the reference): `blocks` blocks of 256
+ * lanes each run `iters` iterations of the stage kernels' x-sweep arithmetic for
+ * one cell (primitives, limited increments, face states, HLL face flux, flux
+ * differences over dx) on register values, with the kernels' lane shifts, then
+ * write one float per lane to out (blocks * 256 floats).  Its issue rate is the
+ * VALU ceiling of the kernels' own instruction mix. */
+int swe_valu_mix_probe(float* out, int blocks, int iters, void* stream);
 
 /* Division probe: for n operand pairs (device arrays), q_rn = the kernels'
  * FP32 division (reciprocal estimate, one Newton step, two remainder
