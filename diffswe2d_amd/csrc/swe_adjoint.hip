@@ -256,6 +256,12 @@ void adj_L_kernel(AdjArgs<T> a) {
   SideGrad<T> gm_pend{};             // row n-1 as the lower state of face (n-2 | n-1)
   SideGrad<T> gm_prev{};             // row n-2 as the lower state of face (n-3 | n-2)
   PrimGrad<T> A3{}, A2{}, A1{};      // y-sweep adjoints of primitives, rows n-3..n-1
+  // Rows whose 64 cells all have depth exactly 0 and a zero adjoint of both
+  // momentum tendencies: a face between two such rows' cells is dry on both
+  // sides (its fluxes are the constant 0) and its topographic source terms meet a
+  // zero adjoint, so it passes no gradient; the x sweep of such a row and such
+  // y-faces are skipped (their adjoints are zeros).  dry1..dry3: rows n-1..n-3.
+  bool dry1 = false, dry2 = false, dry3 = false;
 
   for (int it = 0; it < rows + 6; ++it) {
     const int n = y0 - kHaloA + it;
@@ -265,7 +271,7 @@ void adj_L_kernel(AdjArgs<T> a) {
     PredLoads<T> pl{};
     if (fused_pred) pl = predictor_loads(a, b, n, col);
     PrimGrad<T> gx{};
-    if (it >= 6 && n - 3 >= 0 && n - 3 < f.ny) gx = x_sweep_bwd(a, P3, L3);  // x-faces exist on interior rows only
+    if (it >= 6 && n - 3 >= 0 && n - 3 < f.ny && !dry3) gx = x_sweep_bwd(a, P3, L3);  // x-faces: interior rows
     Prim<T> Pn = fetch_finish<T, STAGE, AdjArgs<T>>(a, b, raw, half);
     Lam3<T> Ln{T(0), T(0), T(0)};
     if (fused_pred) {
@@ -278,6 +284,7 @@ void adj_L_kernel(AdjArgs<T> a) {
     } else {
       Ln = load_lam(a, b, n, col);
     }
+    const bool dry0 = __all(Pn.h == T(0) && Ln.u == T(0) && Ln.v == T(0));
     Incr<T> yi_cur{};
     bool reset_cur = false;
     PrimGrad<T> An{};
@@ -291,8 +298,9 @@ void adj_L_kernel(AdjArgs<T> a) {
       T gft = gdiv(L1.u - L2.u, k.dy);
       T gsL = gdiv(L2.v, k.dy);
       T gsR = gdiv(-L1.v, k.dy);
-      SideGrad<T> gL, gR;
-      face_flux_bwd(up, P2.h, P2.z, dn, P1.h, P1.z, k, gfh, gfn, gft, gsL, gsR, gL, gR);
+      SideGrad<T> gL{}, gR{};
+      if (!(dry3 && dry2 && dry1 && dry0))
+        face_flux_bwd(up, P2.h, P2.z, dn, P1.h, P1.z, k, gfh, gfn, gft, gsL, gsR, gL, gR);
       // row n-2 has both face adjoints now
       CellGrad<T> cg = sides_bwd(P2.y(), yi_prev, reset_prev, gL, gm_prev, k);
       A2.h += cg.h;
@@ -357,6 +365,9 @@ void adj_L_kernel(AdjArgs<T> a) {
     yi_prev = yi_cur;
     reset_prev = reset_cur;
     gm_prev = gm_pend;
+    dry3 = dry2;
+    dry2 = dry1;
+    dry1 = dry0;
   }
 }
 
