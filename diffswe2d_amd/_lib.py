@@ -14,7 +14,7 @@ import torch
 HERE = Path(__file__).resolve().parent
 LIB_PATH = HERE / "libswe2d_hip.so"
 
-SWE_ABI_VERSION = 2
+SWE_ABI_VERSION = 3
 BC_CODES = {"transmissive": 0, "periodic": 1, "constant": 2, "water_level": 3, "wall": 4, "absorbing": 5}
 FRICTION_CODES = {"manning": 0}
 
@@ -51,6 +51,12 @@ class SweForcing(ctypes.Structure):
                 ("time", ctypes.c_double)]
 
 
+class SweDynamicLevel(ctypes.Structure):
+    _fields_ = [("times", ctypes.c_void_p), ("levels", ctypes.c_void_p), ("n", ctypes.c_int32),
+                ("pad", ctypes.c_int32), ("start_time", ctypes.c_double), ("step_level", ctypes.c_void_p),
+                ("step_index", ctypes.c_void_p), ("step_weight", ctypes.c_void_p)]
+
+
 class SweControl(ctypes.Structure):
     _fields_ = [("elapsed", ctypes.c_double * 2), ("duration", ctypes.c_double), ("tol", ctypes.c_double),
                 ("cap", ctypes.c_double), ("bad_dt", ctypes.c_double), ("done", ctypes.c_int32),
@@ -66,6 +72,7 @@ _D = ctypes.c_double
 _G = ctypes.POINTER(SweGrid)
 _C = ctypes.POINTER(SweConfigC)
 _F = ctypes.POINTER(SweForcing)
+_DL = ctypes.POINTER(SweDynamicLevel)
 
 _SIGNATURES = {
     "swe_abi_version": ([], _I),
@@ -91,6 +98,10 @@ _SIGNATURES = {
                                 _P], _I),
     "swe_advance_forced_f64": ([_G, _C, _F, _P, _P, _P, _I, _P, _P, _I, _I, _P, _P, _I, _P, _I, _P, _P, _I, _P, _I,
                                 _P], _I),
+    "swe_advance_dynamic_f32": ([_G, _C, _DL, _F, _P, _P, _P, _P, _P, _P, _I, _I, _I, _P, _P, _I, _P, _I, _P, _P,
+                                 _P, _I, _P], _I),
+    "swe_advance_dynamic_f64": ([_G, _C, _DL, _F, _P, _P, _P, _P, _P, _P, _I, _I, _I, _P, _P, _I, _P, _I, _P, _P,
+                                 _P, _I, _P], _I),
     "swe_run_begin_chunked": ([_P, _D, _I, _P, _I, _P], _I),
     "swe_advance_chunked_f32": ([_G, _C, _P, _P, _P, _P, _P, _P, _I, _I, _I, _P, _P, _I, _P, _I, _P, _P, _I, _P], _I),
     "swe_advance_chunked_f64": ([_G, _C, _P, _P, _P, _P, _P, _P, _I, _I, _I, _P, _P, _I, _P, _I, _P, _P, _I, _P], _I),

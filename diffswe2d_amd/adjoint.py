@@ -291,17 +291,22 @@ class _ForwardFn(torch.autograd.Function):
             raise RuntimeError(
                 f"max_steps={max_steps} reached after {elapsed:.8f} s; requested duration={duration:.8f} s.")
         ctx.model, ctx.dts, ctx.segments = model, dts, segments
+        ctx.U_meta = (U0_.shape, U0_.dtype, U0_.device)
         ctx.save_for_backward(lvl)
-        ctx.mark_non_differentiable(hmax)
         model.last_dt_series = dts
         model.last_adjoint_segments = [(kind, first, n) for kind, first, n, _ in segments]
         return U, hmax
 
     @staticmethod
-    def backward(ctx, g_U, _g_hmax):
+    def backward(ctx, g_U, g_hmax):
+        if g_hmax is not None and bool((g_hmax != 0).any()):
+            raise NotImplementedError(
+                "gradients through hmax of a device-driven run are not supported; set "
+                "model.differentiable_hmax = True to run the per-step loop, whose hmax is differentiable")
         lvl, = ctx.saved_tensors
         model, dts, segments = ctx.model, ctx.dts, ctx.segments
-        lam = g_U.contiguous().clone()
+        shape, dtype, device = ctx.U_meta
+        lam = g_U.contiguous().clone() if g_U is not None else torch.zeros(shape, dtype=dtype, device=device)
         lam_level = torch.zeros(lam.shape[0], dtype=lam.dtype, device=lam.device)
         want_bed = ctx.needs_input_grad[2]
         region = None

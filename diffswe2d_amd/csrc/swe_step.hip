@@ -19,6 +19,7 @@
 #include <stdio.h>
 #include <string.h>
 
+#include <limits>
 #include <mutex>
 
 #include "swe_common.h"
@@ -209,6 +210,37 @@ __global__ void chunk_clock_kernel(Control* c, const T* U_out, const T* bed, int
       c->level_f32 = (float)lv;
     }
   }
+}
+
+// Dynamic boundary series (solver.py:391-415), evaluated before each step of a
+// device-driven run: the level at the step's start time start + elapsed, in the
+// run's dtype -- time rounded like U.new_tensor(current_time), the interval from
+// searchsorted(times, time) clamped to [1, n - 1] (n = 1: both ends are sample 0,
+// as times[-1] is), weight (time - t0) / clamp(t1 - t0, min = eps), level
+// w0 + weight * (w1 - w0) -- written where the stage kernels read the level.
+// Optionally records each step's level, interval index and weight.
+template <class T>
+__global__ void dynamic_level_kernel(Control* c, const T* times, const T* levels, int n, double start, T eps,
+                                     int parity, int step_index, T* step_level, int32_t* step_idx, T* step_weight) {
+  if (threadIdx.x != 0 || c->done) return;
+  const T t = (T)(start + c->elapsed[parity]);
+  int lo = 0, hi = n;
+  while (lo < hi) {  // first index with times[i] >= t
+    const int mid = (lo + hi) >> 1;
+    if (times[mid] < t) lo = mid + 1;
+    else hi = mid;
+  }
+  const int i = n > 1 ? (lo < 1 ? 1 : (lo > n - 1 ? n - 1 : lo)) : 0;
+  const int i0 = i > 0 ? i - 1 : n - 1;
+  const T t0 = times[i0], t1 = times[i];
+  const T w0 = levels[i0], w1 = levels[i];
+  const T weight = (t - t0) / tmax(t1 - t0, eps);
+  const T level = w0 + weight * (w1 - w0);
+  if (sizeof(T) == 8) c->level_f64 = (double)level;
+  else c->level_f32 = (float)level;
+  if (step_level != nullptr) step_level[step_index] = level;
+  if (step_idx != nullptr) step_idx[step_index] = i;
+  if (step_weight != nullptr) step_weight[step_index] = weight;
 }
 
 template <class T>
@@ -448,7 +480,7 @@ int advance_impl(const swe_grid* g, const swe_config* c, const T* bed, const T* 
                  int gauge_capacity, void* ws, int rows_hint, void* stream, T* tape_k1 = nullptr,
                  T* tape_k2 = nullptr, int tape_first = 0, const int32_t* chunk_gauges = nullptr,
                  int chunk_n_gauges = 0, T* chunk_obs = nullptr, int32_t* chunk_series = nullptr,
-                 bool chunked = false, const swe_forcing* fz = nullptr) {
+                 bool chunked = false, const swe_forcing* fz = nullptr, const swe_dynamic_level* dyn = nullptr) {
   int e;
   if ((e = check_grid(g)) || (e = check_cfg(c)) || (e = check_level(level, level_stride, g, c))) return e;
   if (!U_a || (!U_b && !tape_k1) || !bed || !ws || (c->apply_friction && !rough))
@@ -468,6 +500,9 @@ int advance_impl(const swe_grid* g, const swe_config* c, const T* bed, const T* 
   a.hmax = hmax;
   a.dt_series = dt_series;
   if (tape_k1 != nullptr || chunked) a.fast = 0;  // taped / chunked runs feed the adjoint
+  if (dyn != nullptr && (dyn->step_level || dyn->step_index || dyn->step_weight) &&
+      first_step + n_steps > dt_capacity)
+    return fail(SWE_ERR_CAPACITY, "dynamic level record capacity exceeded");
   const size_t state = (size_t)g->batch * 3 * (size_t)g->ny * g->nx;
   for (int k = first_step; k < first_step + n_steps; ++k) {
     int p = k & 1;
@@ -481,6 +516,13 @@ int advance_impl(const swe_grid* g, const swe_config* c, const T* bed, const T* 
       dst = U_a + (j + 1) * state;
       k1 = tape_k1 + j * state;
       k2 = tape_k2 + j * state;
+    }
+    if (dyn != nullptr) {
+      hipLaunchKernelGGL((dynamic_level_kernel<T>), dim3(1), dim3(64), 0, s, w.ctl, (const T*)dyn->times,
+                         (const T*)dyn->levels, dyn->n, dyn->start_time, std::numeric_limits<T>::epsilon(), p, k,
+                         (T*)dyn->step_level, dyn->step_index, (T*)dyn->step_weight);
+      hipError_t he = hipGetLastError();
+      if (he != hipSuccess) return fail(SWE_ERR_HIP, hipGetErrorString(he));
     }
     StageArgs<T> s1 = a;
     s1.U = src;
@@ -598,6 +640,22 @@ int step_noalloc_impl(const swe_grid* g, const swe_config* c, const T* U, const 
   if (he != hipSuccess) return fail(SWE_ERR_HIP, hipGetErrorString(he));
   return step_impl<T>(g, c, 3, U, bed, rough, level, level_stride, max_dt, 0, U_out, ws, rows_hint, stream, nullptr,
                       nullptr, nullptr, dt_out);
+}
+
+template <class T>
+int advance_dynamic(const swe_grid* g, const swe_config* c, const swe_dynamic_level* dyn, const swe_forcing* fz,
+                    const T* bed, const T* rough, T* U_a, T* U_b, T* tape_k1, T* tape_k2, int tape_first,
+                    int first_step, int n_steps, T* hmax, T* dt_series, int capacity, const int32_t* gauges,
+                    int n_gauges, T* gauge_depth, T* gauge_level, void* ws, int rows_hint, void* stream) {
+  if (!dyn || !dyn->times || !dyn->levels || dyn->n < 1) return fail(SWE_ERR_ARG, "dynamic series needs times, levels, n >= 1");
+  if (!ws) return fail(SWE_ERR_ARG, "null workspace");
+  if ((tape_k1 == nullptr) != (tape_k2 == nullptr)) return fail(SWE_ERR_ARG, "record both tapes or neither");
+  // the stage kernels read the step's level from the control block
+  Control* ctl = (Control*)ws;
+  const T* level = sizeof(T) == 8 ? (const T*)&ctl->level_f64 : (const T*)&ctl->level_f32;
+  return advance_impl<T>(g, c, bed, rough, level, 0, U_a, U_b, first_step, n_steps, hmax, dt_series, capacity,
+                         gauges, n_gauges, gauge_depth, gauge_level, capacity, ws, rows_hint, stream, tape_k1, tape_k2,
+                         tape_first, nullptr, 0, nullptr, nullptr, false, fz, dyn);
 }
 
 }  // namespace
@@ -839,6 +897,26 @@ int swe_step_noalloc(const swe_grid* g, const swe_config* c, int dtype_bytes, co
                                      (const double*)level, level_stride, max_dt, (double*)U_out, (double*)dt_out,
                                      rows_hint, stream);
   return fail(SWE_ERR_ARG, "dtype_bytes must be 4 or 8");
+}
+
+int swe_advance_dynamic_f32(const swe_grid* g, const swe_config* c, const swe_dynamic_level* dyn,
+                            const swe_forcing* fz, const float* bed, const float* rough, float* U_a, float* U_b,
+                            float* tape_k1, float* tape_k2, int tape_first, int first_step, int n_steps, float* hmax,
+                            float* dt_series, int capacity, const int32_t* gauges, int n_gauges, float* gauge_depth,
+                            float* gauge_level, void* ws, int rows_hint, void* stream) {
+  return advance_dynamic<float>(g, c, dyn, fz, bed, rough, U_a, U_b, tape_k1, tape_k2, tape_first, first_step, n_steps,
+                                hmax, dt_series, capacity, gauges, n_gauges, gauge_depth, gauge_level, ws, rows_hint,
+                                stream);
+}
+
+int swe_advance_dynamic_f64(const swe_grid* g, const swe_config* c, const swe_dynamic_level* dyn,
+                            const swe_forcing* fz, const double* bed, const double* rough, double* U_a, double* U_b,
+                            double* tape_k1, double* tape_k2, int tape_first, int first_step, int n_steps,
+                            double* hmax, double* dt_series, int capacity, const int32_t* gauges, int n_gauges,
+                            double* gauge_depth, double* gauge_level, void* ws, int rows_hint, void* stream) {
+  return advance_dynamic<double>(g, c, dyn, fz, bed, rough, U_a, U_b, tape_k1, tape_k2, tape_first, first_step,
+                                 n_steps, hmax, dt_series, capacity, gauges, n_gauges, gauge_depth, gauge_level, ws,
+                                 rows_hint, stream);
 }
 
 int swe_water_volume_f32(const swe_grid* g, const float* U, double* volume, void* ws, void* stream) {

@@ -100,6 +100,10 @@ class SWE2D(nn.Module):
         # without gradients use the library's tolerance-bounded arithmetic
         # (swe_config.arithmetic); differentiable paths always use the reference's.
         self.arithmetic = "reference"
+        # With gradients, forward() runs device-driven and returns an hmax that carries
+        # no gradient (a backward through it raises); True takes the per-step loop,
+        # whose hmax = maximum(hmax, h) is differentiable as in src/solver.py:765.
+        self.differentiable_hmax = False
         self.adjoint_checkpoint_every = 64  # K: forward checkpoints for the adjoint replay
 
     # ------------------------------------------------------------------ properties
@@ -524,15 +528,25 @@ class SWE2D(nn.Module):
         self._check_state(U0, "U0")
         cap = None if max_dt_cap is None else self._scalar_float(max_dt_cap, "max_dt_cap")
         level_req = boundary_level is not None and torch.is_tensor(boundary_level) and boundary_level.requires_grad
-        needs_grad = torch.is_grad_enabled() and (U0.requires_grad or level_req or self._bed_trainable())
+        dynamic = boundary_level is None and self.use_dynamic_bc
+        series_req = dynamic and torch.is_tensor(self.bc_wls) and self.bc_wls.requires_grad
+        needs_grad = torch.is_grad_enabled() and (U0.requires_grad or level_req or series_req
+                                                  or self._bed_trainable())
         forced = self.rainfall is not None or infiltration is not None
-        # per-step work of the loop (src/solver.py:765-801), or a level interpolated per step (:391-415)
-        per_step = callback is not None or self.track_gauges or (boundary_level is None and self.use_dynamic_bc)
-        if needs_grad and not forced and not per_step:
+        if needs_grad:
+            # the per-step work of the loop (src/solver.py:765-801: gauges, callback, a
+            # differentiable hmax) and forcing take the host loop of differentiable steps,
+            # like the reference's; the rest runs device-driven with the hand-written adjoint
+            if (callback is not None or self.track_gauges or forced or self.differentiable_hmax
+                    or (dynamic and U0.shape[0] != 1)):
+                return self._forward_host(U0, duration, absolute_start, max_steps, callback, boundary_level, cap,
+                                          infiltration)
+            if dynamic:
+                from .chunked import dynamic_forward
+                return dynamic_forward(self, U0, duration, absolute_start, max_steps, cap)
             from .adjoint import differentiable_forward
             return differentiable_forward(self, U0, duration, absolute_start, max_steps, boundary_level, cap)
-        if needs_grad or callback is not None or (boundary_level is None and self.use_dynamic_bc):
-            # host loop of (differentiable) steps, like the reference's loop
+        if callback is not None:
             return self._forward_host(U0, duration, absolute_start, max_steps, callback, boundary_level, cap,
                                       infiltration)
         return self._forward_device(U0.detach().contiguous(), duration, absolute_start, max_steps,
@@ -568,8 +582,17 @@ class SWE2D(nn.Module):
         lib = _lib.load()
         bed, rough = self._fields(U0)
         ws = self._workspace(U0, bed, rough, fast=self._fast())
-        level = self._resolve_boundary_level(absolute_start, U0, boundary_level)
-        lvl, stride = self._level_arg(level, U0)
+        dynamic = boundary_level is None and self.use_dynamic_bc
+        if dynamic:
+            # the series is interpolated on the device at every step's start time (src/solver.py:391-415)
+            from .chunked import dynamic_arg, dynamic_series
+
+            series = dynamic_series(self, U0)
+            dyn = dynamic_arg(*series, absolute_start)
+            lvl, stride = None, 0
+        else:
+            level = self._resolve_boundary_level(absolute_start, U0, boundary_level)
+            lvl, stride = self._level_arg(level, U0)
         stream = _lib.stream_ptr(U0.device)
         Ua = U0.clone()
         Ub = torch.empty_like(U0)
@@ -591,7 +614,14 @@ class SWE2D(nn.Module):
         _lib.check(lib.swe_run_begin(_lib.ptr(ws["buf"]), float(duration), -1.0 if cap is None else float(cap),
                                      U0.element_size(), stream), "run_begin")
         fz, keep = self._forcing(U0, infiltration, absolute_start)
-        if fz is None:
+        if dynamic:
+            adv = lib.swe_advance_dynamic_f32 if U0.dtype == torch.float32 else lib.swe_advance_dynamic_f64
+
+            def fn(grid, cfg, bed_, rough_, _lvl, _stride, Ua_, Ub_, first, n, hmax_, dts_, cap_, gauges_, n_g, gd_,
+                   gl_, _gcap, ws_, rows, stream_):
+                return adv(grid, cfg, ctypes.byref(dyn), None if fz is None else ctypes.byref(fz), bed_, rough_, Ua_,
+                           Ub_, None, None, 0, first, n, hmax_, dts_, cap_, gauges_, n_g, gd_, gl_, ws_, rows, stream_)
+        elif fz is None:
             fn = lib.swe_advance_f32 if U0.dtype == torch.float32 else lib.swe_advance_f64
         else:
             forced = lib.swe_advance_forced_f32 if U0.dtype == torch.float32 else lib.swe_advance_forced_f64

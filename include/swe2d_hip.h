@@ -28,7 +28,7 @@
 extern "C" {
 #endif
 
-#define SWE_ABI_VERSION 2
+#define SWE_ABI_VERSION 3
 
 enum {
   SWE_OK = 0,
@@ -219,6 +219,40 @@ int swe_advance_forced_f64(const swe_grid* grid, const swe_config* cfg, const sw
                            double* U_a, double* U_b, int first_step, int n_steps, double* hmax, double* dt_series,
                            int dt_capacity, const int32_t* gauges, int n_gauges, double* gauge_depth,
                            double* gauge_level, int gauge_capacity, void* ws, int rows_hint, void* stream);
+
+/* Dynamic boundary series of a device-driven run -- replaces the per-step
+ * interpolation of SWE2D._interpolate_dynamic_boundary (solver.py:391-415) in
+ * the forward loop (:751-757) with a one-thread launch before each step.  The
+ * level of step k is the series at start_time + the run's elapsed time, in the
+ * run's dtype (times and levels arrays of that dtype); it holds for both stages
+ * of the step.  step_level / step_index / step_weight (optional, capacity
+ * entries) record each step's level, interval index (searchsorted clamped to
+ * [1, n - 1]; 0 when n = 1) and interpolation weight, for the adjoint. */
+typedef struct swe_dynamic_level {
+  const void* times;
+  const void* levels;
+  int32_t n, pad;
+  double start_time;
+  void* step_level;
+  int32_t* step_index;
+  void* step_weight;
+} swe_dynamic_level;
+
+/* swe_advance_* / swe_advance_tape_* (tape_k1 and tape_k2 both given: U_a is
+ * the state tape as in swe_advance_tape_*) with the boundary level taken from a
+ * dynamic series instead of a level argument; forcing is optional (NULL).
+ * capacity bounds dt_series, the gauge records and the level records. */
+int swe_advance_dynamic_f32(const swe_grid* grid, const swe_config* cfg, const swe_dynamic_level* dyn,
+                            const swe_forcing* forcing, const float* bed, const float* roughness, float* U_a,
+                            float* U_b, float* tape_k1, float* tape_k2, int tape_first, int first_step, int n_steps,
+                            float* hmax, float* dt_series, int capacity, const int32_t* gauges, int n_gauges,
+                            float* gauge_depth, float* gauge_level, void* ws, int rows_hint, void* stream);
+int swe_advance_dynamic_f64(const swe_grid* grid, const swe_config* cfg, const swe_dynamic_level* dyn,
+                            const swe_forcing* forcing, const double* bed, const double* roughness, double* U_a,
+                            double* U_b, double* tape_k1, double* tape_k2, int tape_first, int first_step,
+                            int n_steps, double* hmax, double* dt_series, int capacity, const int32_t* gauges,
+                            int n_gauges, double* gauge_depth, double* gauge_level, void* ws, int rows_hint,
+                            void* stream);
 
 /* Chunked run -- replaces the inversion driver's chunk loop
  * (bc_inversion_train.py:332-349, 401-410: forward(U, t_end=chunk,
