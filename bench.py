@@ -12,9 +12,10 @@ steps (inputs resident in HBM, device-driven loop, no host sync inside).
 figure prices the whole step at 32 B per cell-update (read h, hu, hv, z, n and
 write h, hu, hv once).  ``inversion`` runs the reference's Monai Valley
 boundary inversion to its early stop and sets its time to the best misfit
-beside the published run's.  ``cpu_baseline`` times the CPU oracle (an eager
-PyTorch restatement of the reference step) on a 2048 x 2048 crop of the same
-basin on the host cores.
+beside the published run's.  ``warmstart`` runs the coastal run-up inversion
+plainly and surrogate-started and compares their times to the plain run's best
+misfit.  ``cpu_baseline`` times the CPU oracle (an eager PyTorch restatement of
+the reference step) on the full basin on the host cores.
 """
 from __future__ import annotations
 
@@ -41,8 +42,11 @@ def parse():
     ap.add_argument("--n", type=int, default=8192, help="grid side (default: the 8192^2 basin)")
     ap.add_argument("--rows", type=int, default=0, help="rows per wavefront strip (0 = autotuned / heuristic)")
     ap.add_argument("--kernel-steps", type=int, default=50, help="steps timed launch by launch with events")
-    ap.add_argument("--cpu-side", type=int, default=2048, help="crop side of the CPU-baseline sample")
-    ap.add_argument("--cpu-steps", type=int, default=50)
+    ap.add_argument("--cpu-steps", type=int, default=2, help="timed oracle steps of the full basin (after 1 warm-up)")
+    ap.add_argument("--warmstart-scenarios", type=int, default=256,
+                    help="ensemble size of the warm-start leg (the run-up inversion, plain vs surrogate-started); "
+                         "0 to skip")
+    ap.add_argument("--warmstart-epochs", type=int, default=40, help="adjoint epochs of the plain inversion")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fast", action="store_true",
                     help="skip the leg with the library's tolerance-bounded FP32 arithmetic")
@@ -353,33 +357,74 @@ def time_adjoint(model, U, steps):
     return t_fwd * 1e3, out["reverse"], out["replay_reverse"]
 
 
-def cpu_baseline(sc, cfg, side, steps):
-    """Oracle (eager PyTorch CPU, FP32) on a side x side crop around the hump."""
-    import numpy as np
+def cpu_model_name():
+    """The host CPU's model name (/proc/cpuinfo), or None."""
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
+def cpu_baseline(sc, cfg, steps, run_steps):
+    """Oracle (eager PyTorch CPU, FP32, the reference's op sequence) on the full basin.
+
+    One warm-up step, then `steps` timed steps of the whole grid; the per-step
+    time is also extrapolated to the run's `run_steps` steps (labelled as such)."""
     import torch
 
     from oracle import swe_oracle as O
 
-    n = sc["U"].shape[-1]
-    h = sc["U"][0, 0]
-    cy, cx = np.unravel_index(np.argmax(h - h.mean()), h.shape)
-    y0 = int(min(max(cy - side // 2, 0), n - side))
-    x0 = int(min(max(cx - side // 2, 0), n - side))
-    sl = (slice(None), slice(None), slice(y0, y0 + side), slice(x0, x0 + side))
-    U = torch.from_numpy(np.ascontiguousarray(sc["U"][sl]))
-    bed = torch.from_numpy(np.ascontiguousarray(sc["bed"][sl]))
-    rough = torch.from_numpy(np.ascontiguousarray(sc["rough"][sl]))
+    U = torch.from_numpy(sc["U"])
+    bed = torch.from_numpy(sc["bed"])
+    rough = torch.from_numpy(sc["rough"])
     threads = torch.get_num_threads()
+    n = U.shape[-1]
     with torch.no_grad():
+        t0 = time.perf_counter()
         U, _ = O.step(U, bed, rough, cfg, sc["dx"], sc["dy"], 1e30)  # warm-up
+        warm = time.perf_counter() - t0
+        print(f"# cpu baseline: warm-up step {warm:.1f} s", file=sys.stderr, flush=True)
         t0 = time.perf_counter()
         for _ in range(steps):
             U, _ = O.step(U, bed, rough, cfg, sc["dx"], sc["dy"], 1e30)
         sec = time.perf_counter() - t0
-    rate = side * side * steps / sec / 1e9
-    return dict(value=rate, unit="Gcell-updates/s", cores=threads, kind="port",
-                sample=f"oracle FP32, {side}x{side} crop of the basin around the hump, 1 warm-up + {steps} "
-                       f"timed steps, {sec / steps:.3f} s/step on {threads} threads")
+    per_step = sec / steps
+    return dict(value=n * n / per_step / 1e9, unit="Gcell-updates/s", cores=threads, kind="port",
+                sample=f"oracle FP32 (eager PyTorch, the reference's op sequence) on the full {n}x{n} basin: "
+                       f"1 warm-up + {steps} timed steps, {per_step:.2f} s/step on {threads} threads",
+                seconds_per_step=round(per_step, 3),
+                extrapolated_run_seconds={"steps": run_steps, "seconds": round(per_step * run_steps, 1),
+                                          "note": "extrapolated from the timed steps, not measured"},
+                os_cpu_count=os.cpu_count(), cpu_model=cpu_model_name(),
+                threads_note="torch threads = min(os.cpu_count(), OMP_NUM_THREADS): the host's CPU share "
+                             "for one GPU")
+
+
+def warmstart_leg(dev, n_scenarios, epochs):
+    """The learned warm start on the coastal run-up inversion (544 x 344, 64 spline controls,
+    450 chunks): plain adjoint inversion from the prior vs surrogate-started inversion, time to
+    the plain run's best gauge misfit, with and without the one-off ensemble and training."""
+    from diffswe2d_amd import warmstart
+
+    rep = warmstart.run(dev, n_scenarios=n_scenarios, epochs_plain=epochs, log=None)
+    plain, warm = rep["plain"], rep["warm"]
+    one_off = rep["ensemble"]["seconds"] + rep["surrogate"]["seconds"]
+    t_warm = warm["seconds_to_bar"]
+    return {"workload": f"coastal run-up boundary inversion 544x344 FP32, 64 spline controls, 450 chunks; "
+                        f"{n_scenarios}-scenario ensemble (272x172 / 544x344 / 1088x688), 1D-conv surrogate",
+            "plain": {"epochs": plain["epochs"], "seconds": round(plain["seconds"], 2),
+                      "best_misfit": plain["best_misfit"]},
+            "warm": {"reached_plain_misfit": warm["reached"], "seconds_to_plain_misfit": t_warm,
+                     "epochs": warm["epochs"], "start_misfit": warm["start_misfit"],
+                     "prior_misfit": warm["prior_misfit"], "fallback_to_prior": warm["fallback"]},
+            "one_off": {"ensemble_seconds": round(rep["ensemble"]["seconds"], 2),
+                        "ensemble_gcell_per_s": round(rep["ensemble"]["gcell_per_s"], 2),
+                        "training_seconds": round(rep["surrogate"]["seconds"], 2)},
+            "ratio_warm_over_plain": None if t_warm is None else round(t_warm / plain["seconds"], 4),
+            "ratio_with_one_off": None if t_warm is None else round((t_warm + one_off) / plain["seconds"], 4)}
 
 
 def main():
@@ -510,11 +555,17 @@ def main():
               f"(published run: {inv['published']['seconds_to_best']} s; {inv['speedup_vs_published']}x)",
               file=sys.stderr, flush=True)
 
+    warm = None
+    if a.warmstart_scenarios > 0:
+        warm = warmstart_leg(dev, a.warmstart_scenarios, a.warmstart_epochs)
+        print(f"# warm start: plain {warm['plain']['seconds']} s to misfit {warm['plain']['best_misfit']:.3e}; warm "
+              f"{warm['warm']['seconds_to_plain_misfit']} s (ratio {warm['ratio_warm_over_plain']}, with the one-off "
+              f"ensemble and training {warm['ratio_with_one_off']})", file=sys.stderr, flush=True)
+
     cpu = None
     if not a.no_cpu:
         torch.set_num_threads(max(1, min(os.cpu_count() or 1, int(os.environ.get("OMP_NUM_THREADS", "16")))))
-        side = min(a.cpu_side, n)
-        cpu = cpu_baseline(sc, cfg, side, a.cpu_steps)
+        cpu = cpu_baseline(sc, cfg, a.cpu_steps, sc["steps"])
         print(f"# cpu baseline {cpu['value']:.4f} Gcell/s ({cpu['sample']})", file=sys.stderr, flush=True)
 
     out = {
@@ -534,6 +585,7 @@ def main():
         "kernel_ms": {"stage1": round(s1, 4), "stage2": round(s2, 4)},
         "adjoint": adj,
         "inversion": inv,
+        "warmstart": warm,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

@@ -6,15 +6,22 @@ through whole ``forward`` calls driven by the inversion script
 (Examples/inverse/.../bc_inversion_train.py:332-412).  Here the same gradients
 come from hand-written adjoint kernels (csrc/swe_adjoint.hip):
 
-* ``differentiable_step``     one step; inputs U and the boundary level.
+* ``differentiable_step``     one step; inputs U, the boundary level and the bed.
 * ``differentiable_forward``  a duration-driven forward call.  The forward
-  keeps a checkpoint of the state every K steps (``model.adjoint_checkpoint_every``,
-  default 64) plus the dt series; the backward replays each K-step segment
-  (recording k1, k2 per step) and sweeps it in reverse.
+  records the tape of each K-step segment (states, k1, k2) while it fits in
+  memory and otherwise keeps a checkpoint of the segment's first state
+  (``model.adjoint_checkpoint_every``, default 64); the backward sweeps recorded
+  segments in reverse and replays checkpointed ones first.
 * ``operator_adjoint``        L^T for ``spatial_operator``.
 
-Gradients reach U and the water-level boundary value.  The bed (DEM
-correction) and roughness are treated as constants.
+Gradients reach U, the water-level boundary value (a scalar, one per member, a
+dynamic series through ``chunked.dynamic_forward`` or the per-step loop) and
+the bed (``SWE2D.bed``, and through it a trainable DEM correction).  Roughness
+is a buffer in the reference and is not differentiated; the source terms of
+``step`` / ``forward`` (rain, infiltration) are not differentiated either
+(``spatial_operator`` returns their gradients).  The hmax of a device-driven
+forward carries no gradient: a backward through it raises, and
+``model.differentiable_hmax = True`` runs the per-step loop instead.
 """
 from __future__ import annotations
 

@@ -203,7 +203,8 @@ def gauge_run(model, U0, series, chunk, gauges, max_steps=10_000_000):
     if model.use_dynamic_bc:
         raise NotImplementedError("gauge_run uses its explicit per-chunk levels, not a dynamic boundary series")
     if model.rainfall is not None:
-        raise NotImplementedError("rainfall forcing is not built into the HIP step")
+        raise NotImplementedError("the chunked device run does not carry the rainfall record; "
+                                  "run the chunks through SWE2D.forward")
     model._check_state(U0, "U0")
     ny, nx = U0.shape[-2:]
     g = torch.as_tensor(gauges, dtype=torch.int32).reshape(-1, 2)

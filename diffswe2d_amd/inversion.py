@@ -120,7 +120,8 @@ class GaugeProblem:
         if series.numel() < self.n_chunks:
             raise ValueError("series shorter than the number of chunks")
         bed_g = self._gauge_bed()
-        if self.fused and self.U0.shape[0] == 1 and not self.model.use_dynamic_bc:
+        if (self.fused and self.U0.shape[0] == 1 and not self.model.use_dynamic_bc
+                and self.model.rainfall is None):
             from .chunked import gauge_run
 
             obs, _ = gauge_run(self.model, self.U0, series[:self.n_chunks], self.chunk, self.gauges)

@@ -7,11 +7,14 @@ on a ROCm device, and a missing library raises at the first call.
 
 Differences a caller can see:
   * ``forward`` integrates on the device without a host synchronisation per
-    step (the reference syncs 2-3 times per step); a callback or a dynamic
-    boundary series falls back to host-driven steps with identical numerics.
+    step (the reference syncs 2-3 times per step), a dynamic boundary series
+    included; a callback takes host-driven steps with identical numerics.
   * gradients flow through ``step``/``forward`` via the hand-written adjoint
-    (``diffswe2d_amd.adjoint``) to U and the boundary level; bed / DEM
-    corrections are not differentiated.
+    (``diffswe2d_amd.adjoint``) to U, the boundary level (or dynamic series) and
+    the bed / trainable DEM correction.  Roughness and the source terms of
+    ``step``/``forward`` are not differentiated (a tensor infiltration that
+    requires grad raises); the hmax of a device-driven differentiable forward
+    raises on backward unless ``differentiable_hmax`` is set.
 """
 from __future__ import annotations
 
@@ -289,6 +292,14 @@ class SWE2D(nn.Module):
 
         return autotune.lookup(U.device, U.shape[-2], U.shape[-1], U.shape[0], U.dtype)
 
+    @staticmethod
+    def _check_source_grad(infiltration):
+        """step / forward do not differentiate their source terms: refuse to drop a requested gradient."""
+        if torch.is_grad_enabled() and torch.is_tensor(infiltration) and infiltration.requires_grad:
+            raise NotImplementedError(
+                "step() and forward() do not return gradients with respect to infiltration; "
+                "spatial_operator() does (or pass a tensor that does not require grad)")
+
     def _fast(self):
         if self.arithmetic not in ("reference", "fast"):
             raise ValueError(f"arithmetic must be 'reference' or 'fast', got {self.arithmetic!r}")
@@ -399,20 +410,29 @@ class SWE2D(nn.Module):
         rhs = torch.empty_like(U)
         B = U.shape[0]
 
-        def field(v):
+        def full(v):
             if v is None:
                 return None
             t = v.to(dtype=U.dtype, device=U.device) if torch.is_tensor(v) else U.new_tensor(v)
-            t = torch.broadcast_to(t, U[:, 0:1].shape)
-            if B > 1 and not bool((t == t[:1]).all()):
-                raise NotImplementedError("per-member rain / infiltration fields are not supported")
-            return t[:1].contiguous()
+            return torch.broadcast_to(t, U[:, 0:1].shape)
 
-        rain, infil = field(rainfall_rate), field(infiltration)
+        rain_f, infil_f = full(rainfall_rate), full(infiltration)
+        per_member = B > 1 and any(t is not None and not bool((t == t[:1]).all()) for t in (rain_f, infil_f))
+        # shared fields go to the kernel; per-member ones are added after it, in the kernel's order
+        # (flux tendency + rain - max(infiltration, 0), src/solver.py:520-525)
+        rain = None if rain_f is None or per_member else rain_f[:1].contiguous()
+        infil = None if infil_f is None or per_member else infil_f[:1].contiguous()
         fn = lib.swe_spatial_operator_f32 if U.dtype == torch.float32 else lib.swe_spatial_operator_f64
         _lib.check(fn(ws["grid"], ws["cfg"], _lib.ptr(U), _lib.ptr(bed), _lib.ptr(lvl), stride, _lib.ptr(rain),
                       _lib.ptr(infil), _lib.ptr(rhs), _lib.ptr(ws["buf"]), self._rows(U),
                       _lib.stream_ptr(U.device)), "spatial_operator")
+        if per_member:
+            h = rhs[:, 0:1]
+            if rain_f is not None:
+                h = h + rain_f
+            if infil_f is not None:
+                h = h - torch.clamp(infil_f, min=0)
+            rhs = torch.cat((h, rhs[:, 1:]), dim=1)
         if return_wave_speed:
             ax, ay = self._bank_speeds(ws, U)
             return rhs, ax, ay
@@ -463,6 +483,7 @@ class SWE2D(nn.Module):
         level = self._prepare_boundary_level(
             self.cfg.constant_level if boundary_level is None else boundary_level, U)
         max_dt_value = self._scalar_float(max_dt, "max_dt") if torch.is_tensor(max_dt) else float(max_dt)
+        self._check_source_grad(infiltration)
         forcing = None
         if self.rainfall is not None or infiltration is not None:
             # time_float = float(model_time) of a tensor in U's dtype (:608-611)
@@ -527,6 +548,7 @@ class SWE2D(nn.Module):
             return U0, U0[:, 0].clone()
         self._check_state(U0, "U0")
         cap = None if max_dt_cap is None else self._scalar_float(max_dt_cap, "max_dt_cap")
+        self._check_source_grad(infiltration)
         level_req = boundary_level is not None and torch.is_tensor(boundary_level) and boundary_level.requires_grad
         dynamic = boundary_level is None and self.use_dynamic_bc
         series_req = dynamic and torch.is_tensor(self.bc_wls) and self.bc_wls.requires_grad

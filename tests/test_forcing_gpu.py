@@ -93,3 +93,58 @@ def test_step_adjoint_with_rain_fp64(cuda_device):
     print(f"adjoint with rain: rel err {e:.2e}; level {float(gl):.6e} vs {float(gl_ref):.6e}")
     assert e <= 1e-10
     assert abs(float(gl) - float(gl_ref)) <= 1e-9 * max(1.0, abs(float(gl_ref)))
+
+
+@pytest.mark.parametrize("dtype", [torch.float32, torch.float64])
+def test_operator_with_per_member_sources(cuda_device, dtype):
+    """spatial_operator with rain and infiltration fields that differ between members: the call
+    without autograd gives the values of the call with autograd and of the oracle
+    (src/solver.py:520-525 adds them to the depth tendency per member)."""
+    m, cfg, bed, rough, U, rf, infil = setup(dtype, cuda_device)
+    rain = 1e-3 * torch.rand(U.shape[0], 1, *U.shape[-2:], dtype=dtype, generator=torch.Generator().manual_seed(4))
+    level = torch.tensor(0.12, dtype=dtype)
+    with torch.no_grad():
+        rhs = m.spatial_operator(U.to(cuda_device), rainfall_rate=rain.to(cuda_device),
+                                 infiltration=infil.to(cuda_device), boundary_level=level.to(cuda_device))
+        old, O.SQRT_MODE = O.SQRT_MODE, "ieee"  # correctly rounded square roots, as the GPU's
+        try:
+            ref = O.spatial_operator(U, bed, cfg, 0.1, 0.1, level, rain=rain, infiltration=infil)
+        finally:
+            O.SQRT_MODE = old
+    Ug = U.to(cuda_device).requires_grad_(True)
+    rhs_g = m.spatial_operator(Ug, rainfall_rate=rain.to(cuda_device), infiltration=infil.to(cuda_device),
+                               boundary_level=level.to(cuda_device))
+    assert torch.equal(rhs, rhs_g.detach())
+    e = relerr(rhs, ref)
+    print(f"{dtype}: per-member sources, rel err vs oracle {e:.2e}")
+    assert e <= (1e-6 if dtype == torch.float32 else 1e-13)
+
+
+def test_step_refuses_infiltration_gradient(cuda_device):
+    """step / forward do not differentiate the infiltration field: asking for it raises."""
+    m, cfg, bed, rough, U, rf, infil = setup(torch.float64, cuda_device)
+    i = infil.to(cuda_device).requires_grad_(True)
+    with pytest.raises(NotImplementedError, match="infiltration"):
+        m.step(U.to(cuda_device), 0.0, 1e30, infiltration=i, boundary_level=0.12)
+    with pytest.raises(NotImplementedError, match="infiltration"):
+        m(U.to(cuda_device), t_end=0.1, infiltration=i, boundary_level=0.12)
+    with torch.no_grad():
+        m.step(U.to(cuda_device), 0.0, 1e30, infiltration=i, boundary_level=0.12)
+
+
+def test_gauge_problem_with_rainfall_runs_chunk_by_chunk(cuda_device):
+    """An inversion problem on a model with a rainfall record takes the per-chunk loop (the fused
+    chunked run does not carry the record) and observes the same values as explicit forward calls."""
+    from diffswe2d_amd import inversion
+
+    m, cfg, bed, rough, U, rf, infil = setup(torch.float64, cuda_device, batch=1)
+    U0 = U.to(cuda_device)
+    prob = inversion.GaugeProblem(m, U0, [(5, 7), (20, 30)], chunk=0.1, n_chunks=3)
+    series = torch.tensor([0.12, 0.13, 0.11], dtype=torch.float64, device=cuda_device)
+    with torch.no_grad():
+        obs = prob.simulate(series)
+        s, ref = U0, []
+        for k in range(3):
+            s, _ = m(s, t_end=0.1, start_time=0.1 * k, boundary_level=series[k])
+            ref.append(torch.stack([s[0, 0, 5, 7] + m.bed[0, 0, 5, 7], s[0, 0, 20, 30] + m.bed[0, 0, 20, 30]]))
+    assert torch.equal(obs, torch.stack(ref))
