@@ -2,12 +2,14 @@
 //
 // Every convolution (forward, input gradient, weight gradient) is a GEMM in
 // "NT" form, C[m][n] = sum_k A[m][k] * B[n][k], with both operands bf16 and
-// K contiguous, accumulated in FP32 by v_mfma_f32_16x16x16_bf16.  A 256-thread
-// block computes a 64 x 64 tile of C: four wavefronts of 32 x 32, each a 2 x 2
-// grid of 16 x 16 MFMA tiles whose operands are loaded straight from global
-// memory as 4-element (8-byte) vectors.  Long-K products (weight gradients over
-// batch x time) are split over blockIdx.z into FP32 partials that a second
-// kernel sums in a fixed order, so results are deterministic.
+// K contiguous, accumulated in FP32 by gfx950's v_mfma_f32_16x16x32_bf16.  A
+// 256-thread block computes a 64 x 64 tile of C: per 32-wide K step it stages
+// the 64 x 32 operand slices in LDS, and each of its four wavefronts reads its
+// 32 x 32 share as a 2 x 2 grid of 16 x 16 MFMA tiles (each A and B slice read
+// from HBM once per block instead of once per wavefront).  Long-K products
+// (weight gradients over batch x time) are split over blockIdx.z into FP32
+// partials that a second kernel sums in a fixed order, so results are
+// deterministic.
 //
 // Data layouts: activations are channel-major [C][B*L] (bf16); im2col writes
 // the "column" operand [B*L][Cin*Kw] (or its transpose) with 'same' zero
@@ -39,36 +41,64 @@ __device__ inline bf16x4 load4(const uint16_t* p, bool ok) {
   return *reinterpret_cast<const bf16x4*>(p);
 }
 
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+
+constexpr int kTile = 64;              // C tile of a block, in M and in N
+constexpr int kKStep = 32;             // K per v_mfma_f32_16x16x32_bf16
+constexpr int kLdsRow = kKStep + 8;    // bf16 per LDS row: 80 bytes, so the 16 rows one
+                                       // MFMA operand read touches start in distinct banks
+
+// C = A * B^T on gfx950's 16x16x32 BF16 MFMA.  Per K step the block stages the
+// 64 x 32 slices of A and B in LDS (one coalesced 16-byte load per thread and
+// operand) and each of its four wavefronts reads its 32 x 32 share as 2 x 2
+// MFMA tiles: lane l holds row (l % 16) of a tile and K elements 8 (l / 16) ..
+// 8 (l / 16) + 7 of the step, the same K mapping for A and B.
 __global__ __launch_bounds__(256) void gemm_nt_bf16_kernel(int M, int N, int K, const uint16_t* __restrict__ A,
                                                            long lda, const uint16_t* __restrict__ B, long ldb,
                                                            float* __restrict__ C, long ldc, long split_stride,
                                                            int k_per_split) {
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int m0 = blockIdx.y * 64 + (wave >> 1) * 32;
-  const int n0 = blockIdx.x * 64 + (wave & 1) * 32;
+  __shared__ __attribute__((aligned(16))) uint16_t As[kTile * kLdsRow];
+  __shared__ __attribute__((aligned(16))) uint16_t Bs[kTile * kLdsRow];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int bm = blockIdx.y * kTile, bn = blockIdx.x * kTile;
+  const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
   const int kb = blockIdx.z * k_per_split;
   const int ke = min(K, kb + k_per_split);
+  // staging: thread t moves row t / 4, K elements 8 (t % 4) .. 8 (t % 4) + 7 of each slice
+  const int sr = tid >> 2, sk = (tid & 3) * 8;
+  const int am = bm + sr, bnr = bn + sr;
   const int r = lane & 15, g = lane >> 4;
   f32x4 acc[2][2];
   for (int i = 0; i < 2; ++i)
     for (int j = 0; j < 2; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
-  const int am0 = m0 + r, am1 = m0 + 16 + r, bn0 = n0 + r, bn1 = n0 + 16 + r;
-  for (int k = kb; k < ke; k += 16) {
-    const int kk = k + 4 * g;
-    bf16x4 a0 = load4(A + (long)am0 * lda + kk, am0 < M);
-    bf16x4 a1 = load4(A + (long)am1 * lda + kk, am1 < M);
-    bf16x4 b0 = load4(B + (long)bn0 * ldb + kk, bn0 < N);
-    bf16x4 b1 = load4(B + (long)bn1 * ldb + kk, bn1 < N);
-    acc[0][0] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a0, b0, acc[0][0], 0, 0, 0);
-    acc[0][1] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a0, b1, acc[0][1], 0, 0, 0);
-    acc[1][0] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a1, b0, acc[1][0], 0, 0, 0);
-    acc[1][1] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a1, b1, acc[1][1], 0, 0, 0);
+  for (int k = kb; k < ke; k += kKStep) {
+    // K and every split boundary are multiples of 16, so an 8-element chunk is
+    // either wholly before ke or wholly past it
+    const int kk = k + sk;
+    const bool ina = kk < ke && am < M, inb = kk < ke && bnr < N;
+    const bf16x4 a_lo = load4(A + (long)am * lda + kk, ina), a_hi = load4(A + (long)am * lda + kk + 4, ina);
+    const bf16x4 b_lo = load4(B + (long)bnr * ldb + kk, inb), b_hi = load4(B + (long)bnr * ldb + kk + 4, inb);
+    __syncthreads();  // the previous step's operand reads are done
+    *reinterpret_cast<bf16x4*>(&As[sr * kLdsRow + sk]) = a_lo;
+    *reinterpret_cast<bf16x4*>(&As[sr * kLdsRow + sk + 4]) = a_hi;
+    *reinterpret_cast<bf16x4*>(&Bs[sr * kLdsRow + sk]) = b_lo;
+    *reinterpret_cast<bf16x4*>(&Bs[sr * kLdsRow + sk + 4]) = b_hi;
+    __syncthreads();
+    const bf16x8 a0 = *reinterpret_cast<const bf16x8*>(&As[(wm + r) * kLdsRow + 8 * g]);
+    const bf16x8 a1 = *reinterpret_cast<const bf16x8*>(&As[(wm + 16 + r) * kLdsRow + 8 * g]);
+    const bf16x8 b0 = *reinterpret_cast<const bf16x8*>(&Bs[(wn + r) * kLdsRow + 8 * g]);
+    const bf16x8 b1 = *reinterpret_cast<const bf16x8*>(&Bs[(wn + 16 + r) * kLdsRow + 8 * g]);
+    acc[0][0] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a0, b0, acc[0][0], 0, 0, 0);
+    acc[0][1] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a0, b1, acc[0][1], 0, 0, 0);
+    acc[1][0] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1, b0, acc[1][0], 0, 0, 0);
+    acc[1][1] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1, b1, acc[1][1], 0, 0, 0);
   }
+  // D layout of a 16 x 16 MFMA tile: lane l holds rows 4 (l / 16) .. +3 of column l % 16
   float* Cs = C + (long)blockIdx.z * split_stride;
   for (int i = 0; i < 2; ++i)
     for (int j = 0; j < 2; ++j)
       for (int q = 0; q < 4; ++q) {
-        int m = m0 + 16 * i + 4 * g + q, n = n0 + 16 * j + r;
+        int m = bm + wm + 16 * i + 4 * g + q, n = bn + wn + 16 * j + r;
         if (m < M && n < N) Cs[(long)m * ldc + n] = acc[i][j][q];
       }
 }
@@ -88,10 +118,13 @@ __global__ void reduce_splits_kernel(const float* P, long split_stride, int spli
 __global__ void im2col_kernel(const uint16_t* X, int Cin, int Bn, int L, int Kw, int dil, int Kpad, int transpose,
                               uint16_t* col) {
   long idx = blockIdx.x * (long)blockDim.x + threadIdx.x;
-  long total = (long)Bn * L * Kpad;
+  const long BL = (long)Bn * L;
+  long total = BL * Kpad;
   if (idx >= total) return;
-  int kc = (int)(idx % Kpad);
-  long n = idx / Kpad;
+  // consecutive threads take consecutive output elements: along K for [B*L][Kpad],
+  // along B*L for the transpose (whose reads of X are then consecutive too)
+  int kc = transpose ? (int)(idx / BL) : (int)(idx % Kpad);
+  long n = transpose ? idx % BL : idx / Kpad;
   int b = (int)(n / L), t = (int)(n % L);
   uint16_t v = 0;
   if (kc < Cin * Kw) {

@@ -257,3 +257,31 @@ def train(net, series, targets, epochs=60, batch=64, lr=2e-3, log=None, time_bud
         if time_budget is not None and time.perf_counter() - t0 > time_budget:
             break
     return hist
+
+
+if __name__ == "__main__":
+    import argparse
+    import json
+
+    ap = argparse.ArgumentParser(description="Time surrogate training on synthetic pairs of the warm start's shapes "
+                                             "(boundary series [L] + a condition channel -> 6 gauge series [L])")
+    ap.add_argument("--pairs", type=int, default=256)
+    ap.add_argument("--length", type=int, default=450)
+    ap.add_argument("--epochs", type=int, default=3)
+    ap.add_argument("--batch", type=int, default=32)
+    a = ap.parse_args()
+    dev = torch.device("cuda:0")
+    g = torch.Generator().manual_seed(0)
+    series = (0.01 * torch.randn(a.pairs, a.length, generator=g)).to(dev)
+    targets = (0.01 * torch.randn(a.pairs, 6, a.length, generator=g)).to(dev)
+    cond = torch.zeros(a.pairs, 1, device=dev)
+    net = Surrogate(default_specs(n_cond=1), device=dev)
+    train(net, series, targets, epochs=1, batch=a.batch, cond=cond)  # first-use kernel loads
+    torch.cuda.synchronize()
+    hist = train(net, series, targets, epochs=a.epochs, batch=a.batch, cond=cond)
+    steps = a.epochs * max(1, a.pairs // a.batch)
+    sec = hist[-1][1]
+    # forward GEMM FLOPs of one sample, x3 for a training step (forward, input and weight gradients)
+    flops = sum(2.0 * s.cout * s.cin * s.kw * a.length for s in net.specs) * a.batch * 3
+    print(json.dumps({"params": net.n_params(), "batch": a.batch, "steps": steps, "ms_per_step": 1e3 * sec / steps,
+                      "gemm_tflop_per_step": flops / 1e12, "tflops_effective": flops * steps / sec / 1e12}))
