@@ -204,11 +204,11 @@ __device__ inline PrimGrad<T> x_sweep_bwd(const AdjArgs<T>& a, const Prim<T>& P,
   Side<T> minus_next = shdn(side_minus(cx, xi, k));
   T lh1 = shdn(L.h), lu1 = shdn(L.u), lv1 = shdn(L.v);
   // face (c | c+1): normal momentum hu, tangential hv
-  T gfh = gdiv(lh1 - L.h, k.dx);
-  T gfn = gdiv(lu1 - L.u, k.dx);
-  T gft = gdiv(lv1 - L.v, k.dx);
-  T gsL = gdiv(L.u, k.dx);
-  T gsR = gdiv(-lu1, k.dx);
+  T gfh = gdiv_by(lh1 - L.h, k.ddx);
+  T gfn = gdiv_by(lu1 - L.u, k.ddx);
+  T gft = gdiv_by(lv1 - L.v, k.ddx);
+  T gsL = gdiv_by(L.u, k.ddx);
+  T gsR = gdiv_by(-lu1, k.ddx);
   SideGrad<T> gL, gR;
   face_flux_bwd(plus, cx.h, cx.z, minus_next, cp.h, cp.z, k, gfh, gfn, gft, gsL, gsR, gL, gR);
   SideGrad<T> gm = shupg(gR);  // this cell as the right state of face (c-1 | c)
@@ -293,11 +293,11 @@ void adj_L_kernel(AdjArgs<T> a) {
       // face (n-2 | n-1): normal momentum hv, tangential hu
       Side<T> up = side_plus(P2.y(), yi_prev, k);
       Side<T> dn = side_minus(P1.y(), yi_cur, k);
-      T gfh = gdiv(L1.h - L2.h, k.dy);
-      T gfn = gdiv(L1.v - L2.v, k.dy);
-      T gft = gdiv(L1.u - L2.u, k.dy);
-      T gsL = gdiv(L2.v, k.dy);
-      T gsR = gdiv(-L1.v, k.dy);
+      T gfh = gdiv_by(L1.h - L2.h, k.ddy);
+      T gfn = gdiv_by(L1.v - L2.v, k.ddy);
+      T gft = gdiv_by(L1.u - L2.u, k.ddy);
+      T gsL = gdiv_by(L2.v, k.ddy);
+      T gsR = gdiv_by(-L1.v, k.ddy);
       SideGrad<T> gL{}, gR{};
       if (!(dry3 && dry2 && dry1 && dry0))
         face_flux_bwd(up, P2.h, P2.z, dn, P1.h, P1.z, k, gfh, gfn, gft, gsL, gsR, gL, gR);

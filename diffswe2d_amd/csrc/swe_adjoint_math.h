@@ -28,7 +28,25 @@ template <class T> SWE_HD T gdiv(T a, T b) {
   return a / b;
 }
 
-// weight of a in minimum(a, b) / maximum(a, b)
+// 1 / b for a denominator that several gradient terms share: the hardware
+// reciprocal in FP32 device code (about 1 ulp), a division otherwise.
+template <class T> SWE_HD T grcp(T b) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  if constexpr (sizeof(T) == 4) return __builtin_amdgcn_rcpf(b);
+#endif
+  return T(1) / b;
+}
+
+// a / d for a divisor fixed for the whole run (dx, dy): FP32 multiplies by its
+// rounded reciprocal, FP64 divides.
+template <class T> SWE_HD T gdiv_by(T a, const Divisor<T>& d) {
+  if constexpr (sizeof(T) == 4) return a * d.r;
+  return a / d.b;
+}
+
+// weight of a in minimum(a, b) / maximum(a, b).  The two weights of one
+// comparison are 1 and 0 or 0.5 and 0.5, so the second is 1 minus the first
+// and g * w2 = g - g * w1 exactly; the reverse computes one and subtracts.
 template <class T> SWE_HD T wmin(T a, T b) { return a < b ? T(1) : (a == b ? T(0.5) : T(0)); }
 template <class T> SWE_HD T wmax(T a, T b) { return a > b ? T(1) : (a == b ? T(0.5) : T(0)); }
 
@@ -46,9 +64,9 @@ SWE_HD void limited_increment_bwd(T qm, T q, T qp, T theta, T g, T& gqm, T& gq, 
   T mbc = tmin(ab, ac);
   // sign(x)^2 = 1 on the same-sign branch, so the gradient is the min weight
   T wa = wmin(aa, mbc);
-  T wbc = wmin(mbc, aa);
+  T wbc = T(1) - wa;          // wmin(mbc, aa)
   T wb = wbc * wmin(ab, ac);
-  T wc = wbc * wmin(ac, ab);
+  T wc = wbc - wb;            // wbc * wmin(ac, ab)
   T ga = g * wa, gb = g * wb, gc = g * wc;
   gq += theta * ga - theta * gc;
   gqm += -theta * ga - T(0.5) * gb;
@@ -97,8 +115,9 @@ SWE_HD CellGrad<T> sides_bwd(const Cell<T>& c, const Incr<T>& d, bool reset, con
   if (wet && facR_in >= T(-10) && facR_in <= T(10)) gq += gfacR;
   // q = (0.5 dh) / max(h, floor)
   T s = clamp_lo(c.h, k.floor);
-  gdh += gdiv(T(0.5) * gq, s);
-  T gs = gdiv(-gq * (T(0.5) * d.dh), s * s);
+  T gq_s = gq * grcp(s);
+  gdh += T(0.5) * gq_s;
+  T gs = -gq_s * d.q;  // -gq * (0.5 dh) / s^2
   if (c.h >= k.floor) r.h += gs;
   // water-slope reset: deta' = reset ? dh + dz : deta
   if (reset) {
@@ -161,10 +180,11 @@ SWE_HD void primitive_bwd(T h_raw, T hu, T hv, const SweConsts<T>& k, T gh, T gu
   bool wet = h > k.dry;
   T safe = clamp_lo(h, k.floor);
   gh += geta;  // eta = h + z
-  g_hu = wet ? gdiv(gu, safe) : T(0);
-  g_hv = wet ? gdiv(gv, safe) : T(0);
+  T r = grcp(safe);
+  g_hu = wet ? gu * r : T(0);
+  g_hv = wet ? gv * r : T(0);
   if (wet) {
-    T gs = gdiv(-(gu * hu + gv * hv), safe * safe);
+    T gs = -(g_hu * hu + g_hv * hv) * r;  // -(gu hu + gv hv) / safe^2
     if (h >= k.floor) gh += gs;
   }
   g_hraw = h_raw >= T(0) ? gh : T(0);
@@ -236,8 +256,9 @@ SWE_HD void face_flux_bwd(const Side<T>& L, T hL_cell, T zL_cell, const Side<T>&
     }
     T gSL = 0, gSR = 0;
     if (gms != T(0) || gps != T(0)) {
-      T gnum_m = gdiv(gms, den), gnum_p = gdiv(gps, den);
-      T gden = gdiv(-gms * num_m, den * den) - gdiv(gps * num_p, den * den);
+      T rd = grcp(den);
+      T gnum_m = gms * rd, gnum_p = gps * rd;
+      T gden = -(gnum_m * num_m + gnum_p * num_p) * rd;  // -(gms num_m + gps num_p) / den^2
       if (dS >= k.hll_floor) {
         gSR += gden;
         gSL -= gden;
@@ -274,21 +295,21 @@ SWE_HD void face_flux_bwd(const Side<T>& L, T hL_cell, T zL_cell, const Side<T>&
       gnR += gSL;
       gcR -= T(2) * gSL;
     } else {
-      T w1 = wmin(a1, a2), w2 = wmin(a2, a1);
-      gnL += gSL * w1;
-      gcL -= gSL * w1;
-      gum += gSL * w2;
-      gcm -= gSL * w2;
+      T t1 = gSL * wmin(a1, a2), t2 = gSL - t1;
+      gnL += t1;
+      gcL -= t1;
+      gum += t2;
+      gcm -= t2;
     }
     if (dryR) {
       gnL += gSR;
       gcL += T(2) * gSR;
     } else {
-      T w1 = wmax(b1, b2), w2 = wmax(b2, b1);
-      gnR += gSR * w1;
-      gcR += gSR * w1;
-      gum += gSR * w2;
-      gcm += gSR * w2;
+      T t1 = gSR * wmax(b1, b2), t2 = gSR - t1;
+      gnR += t1;
+      gcR += t1;
+      gum += t2;
+      gcm += t2;
     }
     gnL += T(0.5) * gum + T(0.25) * gcm;
     gnR += T(0.5) * gum - T(0.25) * gcm;
@@ -298,16 +319,18 @@ SWE_HD void face_flux_bwd(const Side<T>& L, T hL_cell, T zL_cell, const Side<T>&
     T gsfL = 0, gsfR = 0;
     T gqnL = 0, gqtL = 0, gqnR = 0, gqtR = 0;
     if (wetL) {
+      T r = grcp(sfL);
       gsfL += gdiv(gcL * k.g, T(2) * cL);
-      gqnL = gdiv(gnL, sfL);
-      gqtL = gdiv(gtL, sfL);
-      gsfL -= gdiv(gnL * qnL, sfL * sfL) + gdiv(gtL * qtL, sfL * sfL);
+      gqnL = gnL * r;
+      gqtL = gtL * r;
+      gsfL -= (gqnL * qnL + gqtL * qtL) * r;  // (gn qn + gt qt) / sf^2
     }
     if (wetR) {
+      T r = grcp(sfR);
       gsfR += gdiv(gcR * k.g, T(2) * cR);
-      gqnR = gdiv(gnR, sfR);
-      gqtR = gdiv(gtR, sfR);
-      gsfR -= gdiv(gnR * qnR, sfR * sfR) + gdiv(gtR * qtR, sfR * sfR);
+      gqnR = gnR * r;
+      gqtR = gtR * r;
+      gsfR -= (gqnR * qnR + gqtR * qtR) * r;
     }
     if (hL >= k.floor) ghL += gsfL;
     if (hR >= k.floor) ghR += gsfR;
@@ -331,16 +354,17 @@ SWE_HD void face_flux_bwd(const Side<T>& L, T hL_cell, T zL_cell, const Side<T>&
   // hydrostatic depths
   T gmLh = mLh >= T(0) ? g_hcL : T(0);
   T gmRh = mRh >= T(0) ? g_hcR : T(0);
-  T gaL = gmLh * wmin(aL, L.hlin), gLh = gmLh * wmin(L.hlin, aL);
-  T gaR = gmRh * wmin(aR, R.hlin), gRh = gmRh * wmin(R.hlin, aR);
+  T gaL = gmLh * wmin(aL, L.hlin), gLh = gmLh - gaL;
+  T gaR = gmRh * wmin(aR, R.hlin), gRh = gmRh - gaR;
   T gLeta = gaL, gReta = gaR;
   g_zf -= gaL + gaR;
   T gemin = g_zf * wmin(emin, zmax);
-  gLeta += gemin * wmin(L.eta, R.eta);
-  gReta += gemin * wmin(R.eta, L.eta);
-  T gzmax = g_zf * wmin(zmax, emin);
+  T gLe = gemin * wmin(L.eta, R.eta);
+  gLeta += gLe;
+  gReta += gemin - gLe;
+  T gzmax = g_zf - gemin;
   gL.z = gzmax * wmax(L.z, R.z);
-  gR.z = gzmax * wmax(R.z, L.z);
+  gR.z = gzmax - gL.z;
   gL.zcell = gBL;  // s_L = half_g (h_L + h_cL) (z_L - z_face)
   gR.zcell = gBR;
   gL.hlin = gLh;
@@ -392,8 +416,9 @@ SWE_HD void clean_and_limit_bwd(T oh, T ou, T ov, T ch, T cu, T cv, const SweCon
   if (dh < T(0)) {
     T gtheta = g2h * dh + g2u * du + g2v * dv;
     T gq = q <= T(1) ? gtheta : T(0);
-    g_oh += gdiv(gq, s);
-    T gs = gdiv(-gq * oh, s * s);
+    T gq_s = gq * grcp(s);
+    g_oh += gq_s;
+    T gs = -gq_s * q;  // -gq oh / s^2
     if (-dh >= k.eps) gdh -= gs;  // s = max(-dh, eps)
   }
   // delta = cand - old
@@ -461,8 +486,9 @@ SWE_HD void friction_bwd(T h_in, T hu, T hv, T coef, T dt, const SweConsts<T>& k
   T ghv = wet ? gv_out * f : T(0);
   T gh = gh_out;
   T gden = -gf * f * f;
-  T gb = gdiv(gden, safe);
-  T gsafe = gdiv(-gden * b, safe * safe);
+  T rs = grcp(safe);
+  T gb = gden * rs;
+  T gsafe = -gb * c;  // -gden b / safe^2
   T ga = gb * dt;
   T gcf = ga * speed;
   T gspeed = ga * cf;
@@ -471,11 +497,12 @@ SWE_HD void friction_bwd(T h_in, T hu, T hv, T coef, T dt, const SweConsts<T>& k
     T gss2 = gdiv(gspeed, T(2) * speed);
     T gs2 = s2 >= k.fric_floor2 ? gss2 : T(0);
     T gu = T(2) * u * gs2, gv = T(2) * v * gs2;
+    T ghu_s = gu * rs, ghv_s = gv * rs;
     if (wet) {
-      ghu += gdiv(gu, safe);
-      ghv += gdiv(gv, safe);
+      ghu += ghu_s;
+      ghv += ghv_s;
     }
-    gsafe -= gdiv(gu * um + gv * vm, safe * safe);
+    gsafe -= (ghu_s * um + ghv_s * vm) * rs;  // (gu um + gv vm) / safe^2
   }
   if (h >= k.floor) gh += gsafe;
   g_h = h_in >= T(0) ? gh : T(0);
