@@ -6,12 +6,13 @@
 //   adj_L<2>       L^T at the predictor P = clean(U, U + dt/2 k1) -> lam_P (+ ghosts)
 //   adj_ghost      ghost-cell adjoints folded back onto the edge cells (four
 //                  per-side launches) and the boundary-level reduction
-//   adj_predictor  predictor clean-up reversed               -> lam_k1, lam_acc
-//   adj_L<1>       L^T at U                                  -> lam_U = lam_acc + ...
+//   adj_L<1>       the predictor clean-up reversed on each row as the sweep
+//                  fetches it (-> lam_k1 and its direct adjoint of U), then
+//                  L^T at U                                  -> lam_U = lam_acc + ...
 //   adj_ghost      as above
-// The elementwise reverses stay separate launches: folded into the L^T sweeps
-// they would be recomputed on every halo cell at the sweeps' low occupancy,
-// which measured slower than one streaming pass each.
+// The corrector reverse stays a separate launch: folded into the L^T sweep it
+// would be recomputed on every halo cell at the sweep's low occupancy, which
+// measured slower than one streaming pass.
 // L^T mirrors the forward stage's decomposition: one wavefront streams down a
 // strip of 64 columns, now with a 3-cell halo (58 outputs): a cell's adjoint
 // depends on the four faces within two cells of it and those faces on the
@@ -146,14 +147,6 @@ __device__ inline void predictor_core(const AdjArgs<T>& a, T oh, T ou, T ov, T k
   lam = Lam3<T>{half * gc_h, half * gc_u, half * gc_v};
 }
 
-// the same at interior cell i
-template <class T>
-__device__ inline void predictor_bwd(const AdjArgs<T>& a, int64_t i, Lam3<T>& lam, Lam3<T>& acc) {
-  const int64_t P = a.f.plane;
-  predictor_core(a, a.U[i], a.U[i + P], a.U[i + 2 * P], a.k1[i], a.k1[i + P], a.k1[i + 2 * P], a.lam_P[i],
-                 a.lam_P[i + P], a.lam_P[i + 2 * P], lam, acc);
-}
-
 // loads of the predictor reverse at padded cell (r, c), issued ahead of their use
 template <class T> struct PredLoads {
   T kh, ku, kv, gh, gu, gv;
@@ -248,6 +241,11 @@ void adj_L_kernel(AdjArgs<T> a) {
   const int rows = min(a.S, f.ny + 2 - y0);
   const bool lane_out = lane >= kHaloA && lane < kWave - kHaloA && col < f.nx + 2;
   const T half = a.half_dt;
+  // Stage 1 with the predictor reverse fused: the clean-up's direct adjoint of U
+  // of the row fetched now waits in LDS (per lane, a ring four rows deep) until
+  // the row is emitted three iterations later.
+  __shared__ T pacc_ring[STAGE == 1 ? kWavesPerBlock : 1][4][3][kWave];
+  T(*ring)[3][kWave] = pacc_ring[STAGE == 1 ? (threadIdx.x >> 6) : 0];
 
   Prim<T> P3{}, P2{}, P1{};          // rows n-3, n-2, n-1
   Lam3<T> L3{}, L2{}, L1{};          // adjoint of the tendency, same rows
@@ -277,9 +275,13 @@ void adj_L_kernel(AdjArgs<T> a) {
     if (fused_pred) {
       // lam of row n: the predictor clean-up reversed (on interior cells the stage-1
       // fetch loaded the cell's own state)
-      if (pl.in) {
-        Lam3<T> pacc;
-        predictor_core(a, raw.h, raw.hu, raw.hv, pl.kh, pl.ku, pl.kv, pl.gh, pl.gu, pl.gv, Ln, pacc);
+      Lam3<T> pacc{T(0), T(0), T(0)};
+      if (pl.in) predictor_core(a, raw.h, raw.hu, raw.hv, pl.kh, pl.ku, pl.kv, pl.gh, pl.gu, pl.gv, Ln, pacc);
+      if constexpr (STAGE == 1) {
+        const int slot = it & 3;
+        ring[slot][0][lane] = pacc.h;
+        ring[slot][1][lane] = pacc.u;
+        ring[slot][2][lane] = pacc.v;
       }
     } else {
       Ln = load_lam(a, b, n, col);
@@ -325,13 +327,15 @@ void adj_L_kernel(AdjArgs<T> a) {
           int64_t i = (int64_t)b * 3 * f.plane + (int64_t)r * f.nx + col;
           if (a.base != nullptr) {
             T bh = a.base[i], bu = a.base[i + f.plane], bv = a.base[i + 2 * f.plane];
-            if (STAGE == 1 && a.lam_P != nullptr) {
-              // the predictor clean-up's direct adjoint of U, after the corrector's
-              Lam3<T> pl, pacc;
-              predictor_bwd(a, i, pl, pacc);
-              bh += pacc.h;
-              bu += pacc.u;
-              bv += pacc.v;
+            if constexpr (STAGE == 1) {
+              if (a.lam_P != nullptr) {
+                // the predictor clean-up's direct adjoint of U (formed when the row
+                // was fetched), after the corrector's
+                const int slot = (it + 1) & 3;
+                bh += ring[slot][0][lane];
+                bu += ring[slot][1][lane];
+                bv += ring[slot][2][lane];
+              }
             }
             lh += bh;
             lu += bu;
