@@ -6,13 +6,17 @@ fills the GPU on small grids but pays the 4-row halo more often; large S
 amortises the halo.  ``tune_rows`` times both stage launches with HIP events
 for each candidate on the caller's own state and keeps the fastest; results
 are cached per (device, grid, batch, dtype) and saved to ``tuned_rows.json``
-next to this file, which ``SWE2D`` and the ensemble consult when their
-``rows_hint`` is 0.
+next to this file, which ``SWE2D`` consults when its ``rows_hint`` is 0.
+
+``tune_ensemble_rows`` does the same for an ensemble (csrc/swe_ensemble.hip,
+``Ensemble``): one S per grid size, chosen by timing the whole ensemble over a
+few global steps; ``Ensemble`` reads the result for every scenario that names
+no ``rows`` of its own, and leaves the choice to the library's heuristic when
+that mix of sizes was never tuned.
 """
 from __future__ import annotations
 
 import json
-import os
 from pathlib import Path
 
 import torch
@@ -79,3 +83,88 @@ def tune_rows(model, U, candidates=CANDIDATES, reps=8, save=True):
         except OSError:
             pass
     return best, report
+
+
+ENSEMBLE_CANDIDATES = (8, 16, 32, 64)
+
+
+def ensemble_mix(scenarios):
+    """{(ny, nx): number of scenarios of that size}."""
+    mix = {}
+    for sc in scenarios:
+        shape = tuple(int(v) for v in sc["bed"].shape)
+        mix[shape] = mix.get(shape, 0) + 1
+    return mix
+
+
+def _ens_key(device, mix):
+    name = torch.cuda.get_device_name(device) if torch.cuda.is_available() else "cpu"
+    sizes = ",".join(f"{ny}x{nx}:{c}" for (ny, nx), c in sorted(mix.items()))
+    return f"{name}|ensemble|{sizes}|float32"
+
+
+def lookup_ensemble(device, mix) -> dict:
+    """Tuned rows {(ny, nx): S} for an ensemble of this mix of grid sizes, or {} when not tuned."""
+    entry = _load().get(_ens_key(device, mix), {})
+    return {tuple(int(v) for v in k.split("x")): int(S) for k, S in entry.get("rows", {}).items()}
+
+
+def _time_ensemble(cfg, scenarios, rows, device, steps):
+    from .ensemble import Ensemble
+
+    ens = Ensemble(cfg, [dict(sc, rows=rows[tuple(int(v) for v in sc["bed"].shape)]) for sc in scenarios],
+                   device, graph=False)
+    stream = _lib.stream_ptr(device)
+    ens._advance(0, 2, stream)
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+    ev[0].record()
+    ens._advance(2, steps, stream)
+    ev[1].record()
+    torch.cuda.synchronize(device)
+    return ev[0].elapsed_time(ev[1]) / steps
+
+
+def tune_ensemble_rows(cfg, scenarios, device, candidates=ENSEMBLE_CANDIDATES, steps=32, save=True):
+    """Rows per strip for each grid size of an ensemble; returns ({(ny, nx): S}, report).
+
+    The whole ensemble is timed as it will run (every scenario, one launch per
+    stage), `steps` global steps per trial after two untimed ones, in a scratch
+    ensemble built from copies of the inputs.  One pass of coordinate descent,
+    largest share of the cells first: each size's S is varied over the
+    candidates with the others held, and the fastest kept.  With ``save`` the
+    result goes to the table under (device, mix of sizes)."""
+    if steps % 2:
+        raise ValueError("steps must be even (the ensemble's buffers alternate every step)")
+    device = torch.device(device)
+    mix = ensemble_mix(scenarios)
+    shapes = sorted(mix, key=lambda sh: -sh[0] * sh[1] * mix[sh])
+    current = {sh: max([c for c in candidates if c <= max(sh[0] // 8, min(candidates))] or [min(candidates)])
+               for sh in shapes}
+    timed = {}
+
+    def trial(rows):
+        key = tuple(rows[sh] for sh in shapes)
+        if key not in timed:
+            timed[key] = _time_ensemble(cfg, scenarios, rows, device, steps)
+        return timed[key]
+
+    best_ms = trial(current)
+    for sh in shapes:
+        for S in candidates:
+            if S > sh[0] or S == current[sh]:
+                continue
+            rows = dict(current)
+            rows[sh] = S
+            ms = trial(rows)
+            if ms < best_ms:
+                best_ms, current = ms, rows
+    report = {"rows": {f"{ny}x{nx}": S for (ny, nx), S in current.items()}, "ms_per_global_step": round(best_ms, 4),
+              "trials": {",".join(f"{ny}x{nx}:{S}" for (ny, nx), S in zip(shapes, key)): round(ms, 4)
+                         for key, ms in timed.items()}}
+    if save:
+        _load()[_ens_key(device, mix)] = report
+        try:
+            TABLE.write_text(json.dumps(_load(), indent=1, sort_keys=True))
+        except OSError:
+            pass
+    return dict(current), report

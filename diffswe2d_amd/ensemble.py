@@ -12,17 +12,20 @@ from __future__ import annotations
 
 import ctypes
 import struct
+import time
 
 import numpy as np
 import torch
 
-from . import _lib
+from . import _lib, autotune
 
 
 class Ensemble:
     def __init__(self, cfg, scenarios, device, graph=True, steps_per_graph=64):
         """scenarios: dicts with bed [ny,nx], rough [ny,nx], dx, dy, U0 [3,ny,nx], levels [n_chunks],
-        chunk (s), gauges [(iy, ix)], optional rows (rows per wavefront strip)."""
+        chunk (s), gauges [(iy, ix)], optional rows (rows per wavefront strip; without it the
+        rows tuned for that grid size and count (``autotune.tune_ensemble_rows``), else the
+        library's heuristic)."""
         if steps_per_graph % 2:
             raise ValueError("steps_per_graph must be even (buffers alternate every step)")
         self.lib = _lib.load()
@@ -32,8 +35,10 @@ class Ensemble:
         self.graph_enabled = graph
         self.spg = steps_per_graph
         self._keep = []
+        self.rows = []  # rows per strip asked for each scenario (0: the library's heuristic)
         table = (_lib.SweScenario * self.n)()
         self.views = []
+        tuned = autotune.lookup_ensemble(self.device, autotune.ensemble_mix(scenarios))
         f32 = dict(dtype=torch.float32, device=self.device)
         for i, sc in enumerate(scenarios):
             ny, nx = np.asarray(sc["bed"]).shape
@@ -55,7 +60,8 @@ class Ensemble:
             t.bed, t.roughness, t.levels = bed.data_ptr(), rough.data_ptr(), levels.data_ptr()
             t.n_chunks, t.chunk_seconds = len(levels), float(sc["chunk"])
             t.gauges, t.n_gauges, t.gauge_out = gauges.data_ptr(), gauges.shape[0], gout.data_ptr()
-            t.rows_hint = int(sc.get("rows", 0))
+            t.rows_hint = int(sc.get("rows", 0)) or tuned.get((ny, nx), 0)
+            self.rows.append(int(t.rows_hint))
             self._keep += [Ua, Ub, k1, bed, rough, levels, gauges, gout]
             self.views.append(dict(Ua=Ua, Ub=Ub, gauge_out=gout))
         nbytes = self.lib.swe_ensemble_workspace_bytes(self.n)
@@ -78,10 +84,13 @@ class Ensemble:
     def n_done(self) -> int:
         return int(self.ws[4:8].view(torch.int32).item())
 
-    def run(self, max_steps=10_000_000, log=None, log_every=2048):
+    def run(self, max_steps=10_000_000, log=None, log_every=2048, timeline=None):
         """Advance until every scenario has finished its chunks; returns global steps taken.
 
-        ``log`` (optional) receives a progress line every ``log_every`` global steps."""
+        ``log`` (optional) receives a progress line every ``log_every`` global steps.
+        ``timeline`` (a list, optional) receives (global steps, time.perf_counter(),
+        scenarios done) after every replay; the done count is read back from the
+        device, so each entry follows the replay's completion."""
         stream = _lib.stream_ptr(self.device)
         if self.graph_enabled and self._graph is None and self.steps % 2 == 0:
             g = torch.cuda.CUDAGraph()
@@ -96,6 +105,8 @@ class Ensemble:
                 self._advance(self.steps, self.spg, stream)
                 self.steps += self.spg
             done = self.n_done()
+            if timeline is not None:
+                timeline.append((self.steps, time.perf_counter(), done))
             if done >= self.n:
                 break
             if log and self.steps % log_every < self.spg:
@@ -123,3 +134,72 @@ class Ensemble:
             U = v["Ua"] if c["final_parity"] == 0 else v["Ub"]
             res.append(dict(gauges=v["gauge_out"], U=U, steps=c["steps"], done=bool(c["done"])))
         return res
+
+
+def phase_rates(timeline, cells, steps):
+    """Cell-update rates over a run's timeline, split at the global steps where a grid size finishes.
+
+    cells[i], steps[i]: cells and own step count of scenario i.  Returns one dict per
+    phase (global step range, scenarios still running, seconds, Gcell-updates/s)."""
+    cells = np.asarray(cells, dtype=np.float64)
+    steps = np.asarray(steps, dtype=np.int64)
+    ends = sorted(set(int(s) for s in steps))
+    t_prev, s_prev = timeline[0][1], timeline[0][0]
+    out, bound = [], 0
+    acc = dict(cells=0.0, seconds=0.0, start=s_prev)
+    for s, t, _ in timeline[1:]:
+        done_cells = float((cells * (np.minimum(steps, s) - np.minimum(steps, s_prev))).sum())
+        acc["cells"] += done_cells
+        acc["seconds"] += t - t_prev
+        t_prev, s_prev = t, s
+        while bound < len(ends) and s >= ends[bound]:
+            bound += 1
+            if acc["seconds"] > 0:
+                out.append(dict(global_steps=(acc["start"], s), running=int((steps > acc["start"]).sum()),
+                                seconds=round(acc["seconds"], 4),
+                                gcell_per_s=round(acc["cells"] / acc["seconds"] / 1e9, 2)))
+            acc = dict(cells=0.0, seconds=0.0, start=s)
+    return out
+
+
+if __name__ == "__main__":
+    # Timer of the run-up ensemble (seeded scenarios over the three run-up grids):
+    # construction, the graph-replayed run, and the throughput of each phase of
+    # the run (all grid sizes running, then the finer ones alone).
+    import argparse
+    import json
+
+    from . import scenarios
+    from .warmstart import runup_config
+
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--scenarios", type=int, default=256)
+    ap.add_argument("--chunks", type=int, default=450)
+    ap.add_argument("--tune", action="store_true", help="tune rows per grid size first (cached in the table)")
+    a = ap.parse_args()
+    dev = torch.device("cuda")
+    sc, _ = scenarios.ensemble_scenarios(n=a.scenarios, n_chunks=a.chunks)
+    tune_s, tuned = 0.0, None
+    if a.tune:
+        t0 = time.perf_counter()
+        _, tuned = autotune.tune_ensemble_rows(runup_config(), sc, dev)
+        tune_s = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    ens = Ensemble(runup_config(), sc, dev)
+    torch.cuda.synchronize(dev)
+    t1 = time.perf_counter()
+    tl = [(0, t1, 0)]
+    steps = ens.run(timeline=tl)
+    t2 = time.perf_counter()
+    res = ens.results()
+    cells = [s["bed"].size for s in sc]
+    own = [r["steps"] for r in res]
+    total = float(np.dot(cells, own))
+    print(json.dumps(dict(
+        scenarios=a.scenarios, chunks=a.chunks, global_steps=steps, cell_updates=total,
+        rows={f"{s['bed'].shape[0]}x{s['bed'].shape[1]}": r for s, r in zip(sc, ens.rows)},
+        tune_seconds=round(tune_s, 2), tuning=tuned,
+        construct_seconds=round(t1 - t0, 3), run_seconds=round(t2 - t1, 3),
+        run_gcell_per_s=round(total / (t2 - t1) / 1e9, 2),
+        with_construction_gcell_per_s=round(total / (t2 - t0) / 1e9, 2),
+        phases=phase_rates(tl, cells, own))))

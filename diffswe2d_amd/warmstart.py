@@ -38,12 +38,21 @@ def resolution_condition(dx):
 
 
 def generate_pairs(n, device, seed=5, n_controls=64, n_chunks=450, sample_dt=0.05, grids=scenarios.RUNUP_GRIDS,
-                   log=None):
+                   log=None, tune=True):
     """Run the ensemble; returns (series [n, n_chunks], gauges [n, 6, n_chunks], controls, stats).
 
-    stats["cond"] holds each pair's resolution condition [n, 1]."""
+    With ``tune`` the rows per strip of each grid size are tuned first unless
+    the table already holds them (``autotune.tune_ensemble_rows``; its time is
+    stats["tune_seconds"], outside stats["seconds"]).  stats["cond"] holds each
+    pair's resolution condition [n, 1]."""
     sc, ctrl = scenarios.ensemble_scenarios(n=n, seed=seed, n_controls=n_controls, n_chunks=n_chunks,
                                             sample_dt=sample_dt, grids=grids)
+    tune_s = 0.0
+    if tune:
+        if not autotune.lookup_ensemble(device, autotune.ensemble_mix(sc)):
+            t = time.perf_counter()
+            autotune.tune_ensemble_rows(runup_config(), sc, device)
+            tune_s = time.perf_counter() - t
     t0 = time.perf_counter()
     ens = Ensemble(runup_config(), sc, device)
     steps = ens.run(log=log)
@@ -53,7 +62,7 @@ def generate_pairs(n, device, seed=5, n_controls=64, n_chunks=450, sample_dt=0.0
     series = torch.tensor(np.stack([s["levels"] for s in sc]), dtype=torch.float32, device=device)
     cells = sum(s["bed"].size * r["steps"] for s, r in zip(sc, res))
     cond = torch.tensor([[resolution_condition(s["dx"])] for s in sc], dtype=torch.float32, device=device)
-    stats = dict(scenarios=n, seconds=sec, global_steps=steps, cell_updates=cells,
+    stats = dict(scenarios=n, seconds=sec, tune_seconds=tune_s, global_steps=steps, cell_updates=cells,
                  gcell_per_s=cells / sec / 1e9, per_scenario_steps=[r["steps"] for r in res], cond=cond)
     if log:
         log(f"ensemble: {n} scenarios, {cells / 1e9:.2f} G cell-updates in {sec:.2f} s "
