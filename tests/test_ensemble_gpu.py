@@ -47,3 +47,31 @@ def test_ensemble_matches_single_runs(cuda_device, graph):
         assert n_steps == res[i]["steps"], (i, n_steps, res[i]["steps"])
         assert torch.equal(U[0], res[i]["U"]), f"scenario {i}: final state differs"
         np.testing.assert_array_equal(np.float32(obs), res[i]["gauges"].cpu().numpy())
+
+
+def test_rows_per_strip_change_nothing_but_time(cuda_device, monkeypatch, tmp_path):
+    """The tuner's choice (read back by Ensemble from its table) and two fixed strip
+    heights give bit-identical results: tiling only changes which wavefront computes a cell."""
+    from diffswe2d_amd import autotune
+
+    monkeypatch.setattr(autotune, "TABLE", tmp_path / "tuned_rows.json")
+    monkeypatch.setattr(autotune, "_cache", {})
+    cfg = swe.SWEConfig(boundary_left="water_level", boundary_right="wall", boundary_top="wall",
+                        boundary_bottom="wall")
+    sc = make()
+    best, rep = autotune.tune_ensemble_rows(cfg, sc, cuda_device, candidates=(4, 8, 16), steps=4)
+    print("tuned rows", best, rep["trials"])
+    assert set(best) == set(autotune.ensemble_mix(sc)) and all(v in (4, 8, 16) for v in best.values())
+    outs = []
+    for rows in (None, 4, 16):  # None: whatever the table holds for this mix
+        ens = Ensemble(cfg, sc if rows is None else [dict(s, rows=rows) for s in sc], cuda_device, graph=True,
+                       steps_per_graph=16)
+        if rows is None:
+            assert ens.rows == [best[tuple(s["bed"].shape)] for s in sc]
+        ens.run()
+        outs.append(ens.results())
+    for other in outs[1:]:
+        for a, b in zip(outs[0], other):
+            assert a["steps"] == b["steps"]
+            assert torch.equal(a["U"], b["U"])
+            assert torch.equal(a["gauges"], b["gauges"])
