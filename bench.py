@@ -450,7 +450,7 @@ def main():
     else:
         from diffswe2d_amd import autotune
 
-        best, rep = autotune.tune_rows(model, U, candidates=(16, 32, 64, 128), reps=6, save=False)
+        best, rep = autotune.tune_rows(model, U, candidates=(16, 32, 40, 48, 56, 64, 128), reps=10, save=False)
         tuned = {str(k): round(v, 4) for k, v in rep.items()}
         print(f"# autotuned rows per strip: {best} ({tuned})", file=sys.stderr, flush=True)
     n = a.n

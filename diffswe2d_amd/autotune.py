@@ -24,7 +24,7 @@ import torch
 from . import _lib
 
 TABLE = Path(__file__).resolve().parent / "tuned_rows.json"
-CANDIDATES = (4, 8, 16, 32, 64, 128)
+CANDIDATES = (4, 8, 16, 32, 48, 64, 128)
 _cache = None
 
 

@@ -335,7 +335,10 @@ int pick_rows(const swe_grid* g, int rows_hint) {
   // the chip): the chain is then shortest (5 rows instead of 6) and nothing waits
   // for a SIMD; two-row strips otherwise.
   if (S < 2) S = (long long)g->ny * per_col <= 1024 ? 1 : 2;
-  if (S > 64) S = 64;
+  // large grids: 48 rows (measured on 8192^2: 48 rows 1.763 ms per step, 32 rows
+  // 1.778, 64 rows 1.793, 128 rows 1.928; taller strips pay less halo but leave
+  // fewer, longer wavefronts to balance the SIMDs)
+  if (S > 48) S = 48;
   if (S > g->ny) S = g->ny;
   return S;
 }
