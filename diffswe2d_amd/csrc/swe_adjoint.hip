@@ -228,7 +228,7 @@ template <class T, int STAGE, bool BED>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(adj_waves_per_eu<T>())))
 void adj_L_kernel(AdjArgs<T> a) {
   const int lane = threadIdx.x & (kWave - 1);
-  const int wid = blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6);
+  const int wid = blockIdx.x * kWavesPerBlock + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // wave-uniform
   const int b = blockIdx.y;
   const int strip = wid % a.strips;
   const int seg = wid / a.strips;

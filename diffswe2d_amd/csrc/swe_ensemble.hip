@@ -71,7 +71,7 @@ template <int STAGE>
 __global__ __launch_bounds__(256) void ens_stage_kernel(const EnsEntry* tab, EnsClock* clk, const EnsHeader* hdr,
                                                         int parity) {
   const int lane = threadIdx.x & (kWave - 1);
-  const long long wid = (long long)blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6);
+  const long long wid = (long long)blockIdx.x * kWavesPerBlock + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // wave-uniform
   if (wid >= hdr->total_waves) return;
   int s = find_scenario(tab, hdr->n, wid);
   s = __builtin_amdgcn_readfirstlane(s);

@@ -84,10 +84,76 @@ template <class T> struct CellLoads {
   T h, hu, hv, k1h, k1u, k1v, z;
 };
 
+// Raw buffer access for interior rows.  A buffer resource (base, size) sits in
+// scalar registers; a load takes the lane's byte offset in a vector register
+// (fixed for the whole strip) and the row's byte offset in a scalar register,
+// so moving to the next row is one scalar add instead of a 64-bit vector
+// address per load and lane.  Word 3 of the resource is the gfx9 raw-dword
+// format.  Offsets past the size read 0 and drop stores (no fault).
+__device__ inline __amdgpu_buffer_rsrc_t buffer_rsrc(const void* base, unsigned long long bytes) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), (short)0, (int)(unsigned)bytes, 0x00020000);
+}
+template <class T> __device__ inline T buf_load(__amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff) {
+  if constexpr (sizeof(T) == 4) {
+    return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b32(r, (int)voff, (int)soff, 0));
+  } else {
+    return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b64(r, (int)voff, (int)soff, 0));
+  }
+}
+template <class T> __device__ inline void buf_store(T v, __amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff) {
+  if constexpr (sizeof(T) == 4) {
+    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), r, (int)voff, (int)soff, 0);
+  } else {
+    using U2 = unsigned __attribute__((ext_vector_type(2)));
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(U2, v), r, (int)voff, (int)soff, 0);
+  }
+}
+
+// One member's arrays as buffers, for wavefronts whose strip has no ghost column.
+// usable: every array of the member fits the 32-bit buffer range.
+template <class T> struct RowBufs {
+  __amdgpu_buffer_rsrc_t U, k1, bed;
+  unsigned plane_bytes, row_bytes;
+  bool usable;
+};
+template <class T, class Args> __device__ inline RowBufs<T> row_bufs(const Args& a, int b) {
+  const Fields<T>& f = a.f;
+  RowBufs<T> q;
+  const unsigned long long pb = (unsigned long long)f.plane * sizeof(T);
+  q.usable = 3 * pb <= 0xffffffffull;
+  q.plane_bytes = (unsigned)pb;
+  q.row_bytes = (unsigned)f.nx * (unsigned)sizeof(T);
+  q.U = buffer_rsrc(a.U + (int64_t)b * 3 * f.plane, 3 * pb);
+  q.k1 = buffer_rsrc(a.k1 + (int64_t)b * 3 * f.plane, 3 * pb);
+  q.bed = buffer_rsrc(f.bed + (int64_t)b * f.bed_bstride, pb);
+  return q;
+}
+
+// strip_in: every lane's column of this wavefront is interior and the member's
+// arrays fit buffers (wave-uniform); then an interior row needs no ghost rule
+// and loads through the buffers q at lane byte offset voff.
 template <class T, int STAGE, class Args>
-__device__ inline CellLoads<T> fetch_loads(const Args& a, int b, int r, int c) {
+__device__ inline CellLoads<T> fetch_loads(const Args& a, int b, int r, int c, bool strip_in = false,
+                                           const RowBufs<T>* q = nullptr, unsigned voff = 0) {
   const Fields<T>& f = a.f;
   CellLoads<T> w;
+  if (strip_in && r >= 0 && r < f.ny) {
+    w.g.r = w.g.mr = r;
+    w.g.c = w.g.mc = c;
+    w.g.kind = -1;
+    w.g.side = 0;
+    const unsigned so = (unsigned)r * q->row_bytes;
+    w.h = buf_load<T>(q->U, voff, so);
+    w.hu = buf_load<T>(q->U, voff, so + q->plane_bytes);
+    w.hv = buf_load<T>(q->U, voff, so + 2 * q->plane_bytes);
+    if (STAGE == 2) {
+      w.k1h = buf_load<T>(q->k1, voff, so);
+      w.k1u = buf_load<T>(q->k1, voff, so + q->plane_bytes);
+      w.k1v = buf_load<T>(q->k1, voff, so + 2 * q->plane_bytes);
+    }
+    w.z = buf_load<T>(q->bed, voff, so);
+    return w;
+  }
   w.g = ghost_map(r, c, f.ny, f.nx, f.bc);
   const int64_t i = (int64_t)b * 3 * f.plane + (int64_t)w.g.r * f.nx + w.g.c;
   const int64_t P = f.plane;

@@ -143,6 +143,15 @@ __device__ inline void stage_strip(const StageArgs<T>& a, int strip, int seg, in
   const bool lane_out = lane >= kHalo && lane < kWave - kHalo && col < f.nx;
   const bool xface_ok = lane >= 1 && lane <= kWave - 3 && col >= -1 && col <= f.nx - 1;
   const bool ycol_ok = col >= 0 && col < f.nx;
+  const RowBufs<T> q = row_bufs<T>(a, b);
+  // no ghost column and buffers usable (wave-uniform): interior rows load through q
+  const bool strip_in = q.usable && x0 - kHalo >= 0 && x0 - kHalo + kWave <= f.nx;
+  const unsigned voff = (unsigned)col * (unsigned)sizeof(T);  // meaningful where col >= 0
+  // the member's output (and stage-2 roughness / k2 record) as buffers, same sizes
+  const unsigned long long pb = (unsigned long long)f.plane * sizeof(T);
+  const __amdgpu_buffer_rsrc_t out_buf = buffer_rsrc(a.out + mbase, 3 * pb);
+  const __amdgpu_buffer_rsrc_t k2_buf = buffer_rsrc(a.k2_out + mbase, 3 * pb);
+  const __amdgpu_buffer_rsrc_t rough_buf = buffer_rsrc(f.rough + (int64_t)b * f.rough_bstride, pb);
 
   // Rows whose 64 cells all have depth exactly 0 (after the ghost rules): every
   // face between two such cells carries zero flux and zero speed, and every such
@@ -153,15 +162,23 @@ __device__ inline void stage_strip(const StageArgs<T>& a, int strip, int seg, in
   for (int it = 0; it < rows + 4; ++it) {
     const int r = y0 - kHalo + it;
     // row r's loads go out first; the x sweep of row o = r - 2 does not need them
-    CellLoads<T> raw = fetch_loads<T, STAGE>(a, b, r, col);
+    CellLoads<T> raw = fetch_loads<T, STAGE>(a, b, r, col, strip_in, &q, voff);
     // stage 2: the output row's step-start state and roughness, needed at its update
     T oh = T(0), ou = T(0), ov = T(0), coef = T(0);
     if (STAGE == 2 && it >= 4 && lane_out) {
-      const int64_t cell = (int64_t)(r - 2) * f.nx + col;
-      oh = a.U[mbase + cell];
-      ou = a.U[mbase + cell + f.plane];
-      ov = a.U[mbase + cell + 2 * f.plane];
-      if (a.friction) coef = f.rough[(int64_t)b * f.rough_bstride + cell];
+      if (strip_in) {
+        const unsigned so = (unsigned)(r - 2) * q.row_bytes;
+        oh = buf_load<T>(q.U, voff, so);
+        ou = buf_load<T>(q.U, voff, so + q.plane_bytes);
+        ov = buf_load<T>(q.U, voff, so + 2 * q.plane_bytes);
+        if (a.friction) coef = buf_load<T>(rough_buf, voff, so);
+      } else {
+        const int64_t cell = (int64_t)(r - 2) * f.nx + col;
+        oh = a.U[mbase + cell];
+        ou = a.U[mbase + cell + f.plane];
+        ov = a.U[mbase + cell + 2 * f.plane];
+        if (a.friction) coef = f.rough[(int64_t)b * f.rough_bstride + cell];
+      }
     }
     T rx_h = T(0), rx_u = T(0), rx_v = T(0);
     if (it >= 4 && !dry2) {
@@ -205,24 +222,43 @@ __device__ inline void stage_strip(const StageArgs<T>& a, int strip, int seg, in
             if (a.rain != nullptr) th = th + a.rain[(int64_t)b * a.rain_bstride + cell];
             else if (rain_rec) th = th + rain_value(a.rain_rates, f.plane, rec, cell);
             if (a.infil != nullptr) th = th - clamp_lo(a.infil[(int64_t)b * a.infil_bstride + cell], T(0));
+            const unsigned so = (unsigned)o * q.row_bytes;
             if (STAGE == 1) {
-              a.out[i] = th;
-              a.out[i + f.plane] = tu;
-              a.out[i + 2 * f.plane] = tv;
+              if (strip_in) {
+                buf_store(th, out_buf, voff, so);
+                buf_store(tu, out_buf, voff, so + q.plane_bytes);
+                buf_store(tv, out_buf, voff, so + 2 * q.plane_bytes);
+              } else {
+                a.out[i] = th;
+                a.out[i + f.plane] = tu;
+                a.out[i + 2 * f.plane] = tv;
+              }
             } else {
               if (a.k2_out != nullptr) {
-                a.k2_out[i] = th;
-                a.k2_out[i + f.plane] = tu;
-                a.k2_out[i + 2 * f.plane] = tv;
+                if (strip_in) {
+                  buf_store(th, k2_buf, voff, so);
+                  buf_store(tu, k2_buf, voff, so + q.plane_bytes);
+                  buf_store(tv, k2_buf, voff, so + 2 * q.plane_bytes);
+                } else {
+                  a.k2_out[i] = th;
+                  a.k2_out[i + f.plane] = tu;
+                  a.k2_out[i + 2 * f.plane] = tv;
+                }
               }
               T nh = oh + dt * th, nu = ou + dt * tu, nv = ov + dt * tv;
               clean_and_limit<T, A>(oh, ou, ov, nh, nu, nv, k, a.positivity != 0);
               // friction leaves a dry cell as the clean-up left it (h >= 0, zero
               // momentum): skipped when no cell of the row is wet
               if (a.friction && __any(nh > k.dry)) swe::friction<T, A>(nh, nu, nv, coef, dt, k, a.friction_model);
-              a.out[i] = nh;
-              a.out[i + f.plane] = nu;
-              a.out[i + 2 * f.plane] = nv;
+              if (strip_in) {
+                buf_store(nh, out_buf, voff, so);
+                buf_store(nu, out_buf, voff, so + q.plane_bytes);
+                buf_store(nv, out_buf, voff, so + 2 * q.plane_bytes);
+              } else {
+                a.out[i] = nh;
+                a.out[i + f.plane] = nu;
+                a.out[i + 2 * f.plane] = nv;
+              }
               if (a.hmax != nullptr) {
                 int64_t hi = (int64_t)b * f.plane + cell;
                 a.hmax[hi] = tmax(a.hmax[hi], nh);

@@ -33,7 +33,9 @@ namespace swe {
 template <class T, int STAGE, class A>
 __global__ __launch_bounds__(256) void stage_kernel(StageArgs<T> a) {
   const int lane = threadIdx.x & (kWave - 1);
-  const int wid = blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6);
+  // the wavefront's index, made wave-uniform for the compiler (it cannot see that
+  // threadIdx.x >> 6 is): the strip's rows and columns then live in scalar registers
+  const int wid = blockIdx.x * kWavesPerBlock + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int b = blockIdx.y;
   const int strip = wid % a.strips;
   const int seg = wid / a.strips;
