@@ -147,11 +147,11 @@ __device__ inline void stage_strip(const StageArgs<T>& a, int strip, int seg, in
   // no ghost column and buffers usable (wave-uniform): interior rows load through q
   const bool strip_in = q.usable && x0 - kHalo >= 0 && x0 - kHalo + kWave <= f.nx;
   const unsigned voff = (unsigned)col * (unsigned)sizeof(T);  // meaningful where col >= 0
-  // the member's output (and stage-2 roughness / k2 record) as buffers, same sizes
-  const unsigned long long pb = (unsigned long long)f.plane * sizeof(T);
-  const __amdgpu_buffer_rsrc_t out_buf = buffer_rsrc(a.out + mbase, 3 * pb);
-  const __amdgpu_buffer_rsrc_t k2_buf = buffer_rsrc(a.k2_out + mbase, 3 * pb);
-  const __amdgpu_buffer_rsrc_t rough_buf = buffer_rsrc(f.rough + (int64_t)b * f.rough_bstride, pb);
+  // the member's output as a buffer of the same size (the rarer stage-2 loads and
+  // stores -- roughness, the k2 record -- keep global addresses: each resource
+  // held across the loop costs four scalar registers, and the stage-2 loop is
+  // short of them)
+  const __amdgpu_buffer_rsrc_t out_buf = buffer_rsrc(a.out + mbase, 3ull * q.plane_bytes);
 
   // Rows whose 64 cells all have depth exactly 0 (after the ghost rules): every
   // face between two such cells carries zero flux and zero speed, and every such
@@ -171,7 +171,7 @@ __device__ inline void stage_strip(const StageArgs<T>& a, int strip, int seg, in
         oh = buf_load<T>(q.U, voff, so);
         ou = buf_load<T>(q.U, voff, so + q.plane_bytes);
         ov = buf_load<T>(q.U, voff, so + 2 * q.plane_bytes);
-        if (a.friction) coef = buf_load<T>(rough_buf, voff, so);
+        if (a.friction) coef = f.rough[(int64_t)b * f.rough_bstride + (int64_t)(r - 2) * f.nx + col];
       } else {
         const int64_t cell = (int64_t)(r - 2) * f.nx + col;
         oh = a.U[mbase + cell];
@@ -235,15 +235,9 @@ __device__ inline void stage_strip(const StageArgs<T>& a, int strip, int seg, in
               }
             } else {
               if (a.k2_out != nullptr) {
-                if (strip_in) {
-                  buf_store(th, k2_buf, voff, so);
-                  buf_store(tu, k2_buf, voff, so + q.plane_bytes);
-                  buf_store(tv, k2_buf, voff, so + 2 * q.plane_bytes);
-                } else {
-                  a.k2_out[i] = th;
-                  a.k2_out[i + f.plane] = tu;
-                  a.k2_out[i + 2 * f.plane] = tv;
-                }
+                a.k2_out[i] = th;
+                a.k2_out[i + f.plane] = tu;
+                a.k2_out[i + 2 * f.plane] = tv;
               }
               T nh = oh + dt * th, nu = ou + dt * tu, nv = ov + dt * tv;
               clean_and_limit<T, A>(oh, ou, ov, nh, nu, nv, k, a.positivity != 0);
