@@ -76,6 +76,11 @@ template <class T> __device__ inline T step_dt(const typename Bits<T>::U* acc, c
 
 template <class T> __device__ inline bool dt_ok(T dt) { return isfinite(dt) && dt > T(0); }
 
+// whether a stage launch needs the source-term / k2-record paths of stage_strip
+template <class T> inline bool stage_has_sources(const StageArgs<T>& a, int stage) {
+  return a.rain != nullptr || a.infil != nullptr || a.rain_rates != nullptr || (stage == 2 && a.k2_out != nullptr);
+}
+
 // Where the rainfall record is sampled at one model time (forcing.py:24-38):
 // right = searchsorted(times, t) clamped to [1, nt-1]; weight from the bracketing
 // pair; times before the first / after the last record hold the end value.
@@ -121,8 +126,11 @@ template <class T> __device__ inline T wave_max(T v) {
 
 // One wavefront's strip: columns [strip*60 - 2, strip*60 + 62), rows of segment
 // seg, member b.  Writes the stage output for its 60 x S cells and returns the
-// wave-local maxima of the x- and y-face speeds it evaluated.
-template <class T, int STAGE, class A = RefRound>
+// wave-local maxima of the x- and y-face speeds it evaluated.  SRC: the launch
+// carries a source term (rain field, rainfall record, infiltration) or records
+// k2; without it those paths are compiled out, which frees the scalar and
+// vector registers they hold across the row loop (3 % of a plain step).
+template <class T, int STAGE, class A = RefRound, bool SRC = true>
 __device__ inline void stage_strip(const StageArgs<T>& a, int strip, int seg, int b, T dt, T half_dt, T& ax_out,
                                    T& ay_out, bool rain_rec = false, RainAt<T> rec = RainAt<T>{0, 0, 0, T(0)}) {
   const int lane = threadIdx.x & (kWave - 1);
@@ -219,9 +227,11 @@ __device__ inline void stage_strip(const StageArgs<T>& a, int strip, int seg, in
           if (lane_out) {
             int64_t i = mbase + (int64_t)o * f.nx + col;
             int64_t cell = (int64_t)o * f.nx + col;
-            if (a.rain != nullptr) th = th + a.rain[(int64_t)b * a.rain_bstride + cell];
-            else if (rain_rec) th = th + rain_value(a.rain_rates, f.plane, rec, cell);
-            if (a.infil != nullptr) th = th - clamp_lo(a.infil[(int64_t)b * a.infil_bstride + cell], T(0));
+            if constexpr (SRC) {
+              if (a.rain != nullptr) th = th + a.rain[(int64_t)b * a.rain_bstride + cell];
+              else if (rain_rec) th = th + rain_value(a.rain_rates, f.plane, rec, cell);
+              if (a.infil != nullptr) th = th - clamp_lo(a.infil[(int64_t)b * a.infil_bstride + cell], T(0));
+            }
             const unsigned so = (unsigned)o * q.row_bytes;
             if (STAGE == 1) {
               if (strip_in) {
@@ -234,10 +244,12 @@ __device__ inline void stage_strip(const StageArgs<T>& a, int strip, int seg, in
                 a.out[i + 2 * f.plane] = tv;
               }
             } else {
-              if (a.k2_out != nullptr) {
-                a.k2_out[i] = th;
-                a.k2_out[i + f.plane] = tu;
-                a.k2_out[i + 2 * f.plane] = tv;
+              if constexpr (SRC) {
+                if (a.k2_out != nullptr) {
+                  a.k2_out[i] = th;
+                  a.k2_out[i + f.plane] = tu;
+                  a.k2_out[i + 2 * f.plane] = tv;
+                }
               }
               T nh = oh + dt * th, nu = ou + dt * tu, nv = ov + dt * tv;
               clean_and_limit<T, A>(oh, ou, ov, nh, nu, nv, k, a.positivity != 0);

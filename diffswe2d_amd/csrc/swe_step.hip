@@ -30,7 +30,7 @@
 
 namespace swe {
 
-template <class T, int STAGE, class A>
+template <class T, int STAGE, class A, bool SRC>
 __global__ __launch_bounds__(256) void stage_kernel(StageArgs<T> a) {
   const int lane = threadIdx.x & (kWave - 1);
   // the wavefront's index, made wave-uniform for the compiler (it cannot see that
@@ -91,14 +91,16 @@ __global__ __launch_bounds__(256) void stage_kernel(StageArgs<T> a) {
 
   T ax, ay;
   RainAt<T> rec{0, 0, 0, T(0)};
-  if (a.rain_rates != nullptr) {
-    // model_time = U.new_tensor(start + elapsed) (solver.py:751-757); stage 2 samples
-    // the record at time + 0.5 dt in double (:634-637)
-    double t = a.clock ? a.time0 + a.ctl->elapsed[a.parity] : a.time0;
-    double ts = (double)(T)t;
-    rec = rain_at<T>(a.rain_times, a.rain_nt, STAGE == 1 ? ts : ts + 0.5 * (double)dt);
+  if constexpr (SRC) {
+    if (a.rain_rates != nullptr) {
+      // model_time = U.new_tensor(start + elapsed) (solver.py:751-757); stage 2 samples
+      // the record at time + 0.5 dt in double (:634-637)
+      double t = a.clock ? a.time0 + a.ctl->elapsed[a.parity] : a.time0;
+      double ts = (double)(T)t;
+      rec = rain_at<T>(a.rain_times, a.rain_nt, STAGE == 1 ? ts : ts + 0.5 * (double)dt);
+    }
   }
-  stage_strip<T, STAGE, A>(a, strip, seg, b, dt, half_dt, ax, ay, a.rain_rates != nullptr, rec);
+  stage_strip<T, STAGE, A, SRC>(a, strip, seg, b, dt, half_dt, ax, ay, a.rain_rates != nullptr, rec);
   if (STAGE == 1 && a.record_wave) {
     ax = wave_max(ax);
     ay = wave_max(ay);
@@ -378,14 +380,19 @@ template <class T> dim3 stage_grid(const StageArgs<T>& a) {
   return dim3((unsigned)((waves + kWavesPerBlock - 1) / kWavesPerBlock), (unsigned)a.f.B, 1);
 }
 
+template <class T, int STAGE, class A> void launch_stage_as(const StageArgs<T>& a, hipStream_t s) {
+  if (stage_has_sources(a, STAGE))
+    hipLaunchKernelGGL((stage_kernel<T, STAGE, A, true>), stage_grid(a), dim3(kWave * kWavesPerBlock), 0, s, a);
+  else
+    hipLaunchKernelGGL((stage_kernel<T, STAGE, A, false>), stage_grid(a), dim3(kWave * kWavesPerBlock), 0, s, a);
+}
+
 template <class T, int STAGE> int launch_stage(const StageArgs<T>& a, hipStream_t s) {
   if constexpr (sizeof(T) == 4) {
-    if (a.fast)
-      hipLaunchKernelGGL((stage_kernel<T, STAGE, FastRound>), stage_grid(a), dim3(kWave * kWavesPerBlock), 0, s, a);
-    else
-      hipLaunchKernelGGL((stage_kernel<T, STAGE, RefRound>), stage_grid(a), dim3(kWave * kWavesPerBlock), 0, s, a);
+    if (a.fast) launch_stage_as<T, STAGE, FastRound>(a, s);
+    else launch_stage_as<T, STAGE, RefRound>(a, s);
   } else {
-    hipLaunchKernelGGL((stage_kernel<T, STAGE, RefRound>), stage_grid(a), dim3(kWave * kWavesPerBlock), 0, s, a);
+    launch_stage_as<T, STAGE, RefRound>(a, s);
   }
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return fail(SWE_ERR_HIP, hipGetErrorString(e));
