@@ -89,7 +89,7 @@ template <class T> struct CellLoads {
 // (fixed for the whole strip) and the row's byte offset in a scalar register,
 // so moving to the next row is one scalar add instead of a 64-bit vector
 // address per load and lane.  Word 3 of the resource is the gfx9 raw-dword
-// format.  Offsets past the size read 0 and drop stores (no fault).
+// format.  Callers use these only for rows and columns inside the arrays.
 __device__ inline __amdgpu_buffer_rsrc_t buffer_rsrc(const void* base, unsigned long long bytes) {
   return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), (short)0, (int)(unsigned)bytes, 0x00020000);
 }
