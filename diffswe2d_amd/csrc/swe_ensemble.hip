@@ -96,7 +96,7 @@ __global__ __launch_bounds__(256) void ens_stage_kernel(const EnsEntry* tab, Ens
     a.out = parity ? e.Ua : e.Ub;
   }
   float ax, ay;
-  stage_strip<float, STAGE, RefRound, false>(a, strip, seg, 0, dt, half, ax, ay);  // no sources, no k2 record
+  stage_strip<float, STAGE, RefRound, false, false>(a, strip, seg, 0, dt, half, ax, ay);  // no sources, no k2 record
   if (STAGE == 1) {
     ax = wave_max(ax);
     ay = wave_max(ay);

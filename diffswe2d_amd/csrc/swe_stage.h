@@ -76,9 +76,9 @@ template <class T> __device__ inline T step_dt(const typename Bits<T>::U* acc, c
 
 template <class T> __device__ inline bool dt_ok(T dt) { return isfinite(dt) && dt > T(0); }
 
-// whether a stage launch needs the source-term / k2-record paths of stage_strip
-template <class T> inline bool stage_has_sources(const StageArgs<T>& a, int stage) {
-  return a.rain != nullptr || a.infil != nullptr || a.rain_rates != nullptr || (stage == 2 && a.k2_out != nullptr);
+// whether a stage launch needs the source-term paths of stage_strip
+template <class T> inline bool stage_has_sources(const StageArgs<T>& a) {
+  return a.rain != nullptr || a.infil != nullptr || a.rain_rates != nullptr;
 }
 
 // Where the rainfall record is sampled at one model time (forcing.py:24-38):
@@ -127,10 +127,10 @@ template <class T> __device__ inline T wave_max(T v) {
 // One wavefront's strip: columns [strip*60 - 2, strip*60 + 62), rows of segment
 // seg, member b.  Writes the stage output for its 60 x S cells and returns the
 // wave-local maxima of the x- and y-face speeds it evaluated.  SRC: the launch
-// carries a source term (rain field, rainfall record, infiltration) or records
-// k2; without it those paths are compiled out, which frees the scalar and
-// vector registers they hold across the row loop (3 % of a plain step).
-template <class T, int STAGE, class A = RefRound, bool SRC = true>
+// carries a source term (rain field, rainfall record, infiltration); K2: stage 2
+// records k2.  Without them those paths are compiled out, which frees the scalar
+// and vector registers they would hold across the row loop (3 % of a plain step).
+template <class T, int STAGE, class A = RefRound, bool SRC = true, bool K2 = true>
 __device__ inline void stage_strip(const StageArgs<T>& a, int strip, int seg, int b, T dt, T half_dt, T& ax_out,
                                    T& ay_out, bool rain_rec = false, RainAt<T> rec = RainAt<T>{0, 0, 0, T(0)}) {
   const int lane = threadIdx.x & (kWave - 1);
@@ -244,7 +244,7 @@ __device__ inline void stage_strip(const StageArgs<T>& a, int strip, int seg, in
                 a.out[i + 2 * f.plane] = tv;
               }
             } else {
-              if constexpr (SRC) {
+              if constexpr (K2) {
                 if (a.k2_out != nullptr) {
                   a.k2_out[i] = th;
                   a.k2_out[i + f.plane] = tu;

@@ -30,7 +30,7 @@
 
 namespace swe {
 
-template <class T, int STAGE, class A, bool SRC>
+template <class T, int STAGE, class A, bool SRC, bool K2>
 __global__ __launch_bounds__(256) void stage_kernel(StageArgs<T> a) {
   const int lane = threadIdx.x & (kWave - 1);
   // the wavefront's index, made wave-uniform for the compiler (it cannot see that
@@ -100,7 +100,7 @@ __global__ __launch_bounds__(256) void stage_kernel(StageArgs<T> a) {
       rec = rain_at<T>(a.rain_times, a.rain_nt, STAGE == 1 ? ts : ts + 0.5 * (double)dt);
     }
   }
-  stage_strip<T, STAGE, A, SRC>(a, strip, seg, b, dt, half_dt, ax, ay, a.rain_rates != nullptr, rec);
+  stage_strip<T, STAGE, A, SRC, K2>(a, strip, seg, b, dt, half_dt, ax, ay, a.rain_rates != nullptr, rec);
   if (STAGE == 1 && a.record_wave) {
     ax = wave_max(ax);
     ay = wave_max(ay);
@@ -380,11 +380,17 @@ template <class T> dim3 stage_grid(const StageArgs<T>& a) {
   return dim3((unsigned)((waves + kWavesPerBlock - 1) / kWavesPerBlock), (unsigned)a.f.B, 1);
 }
 
-template <class T, int STAGE, class A> void launch_stage_as(const StageArgs<T>& a, hipStream_t s) {
-  if (stage_has_sources(a, STAGE))
-    hipLaunchKernelGGL((stage_kernel<T, STAGE, A, true>), stage_grid(a), dim3(kWave * kWavesPerBlock), 0, s, a);
+template <class T, int STAGE, class A, bool SRC> void launch_stage_src(const StageArgs<T>& a, hipStream_t s) {
+  const dim3 grid = stage_grid(a), block(kWave * kWavesPerBlock);
+  if (STAGE == 2 && a.k2_out != nullptr)
+    hipLaunchKernelGGL((stage_kernel<T, STAGE, A, SRC, STAGE == 2>), grid, block, 0, s, a);
   else
-    hipLaunchKernelGGL((stage_kernel<T, STAGE, A, false>), stage_grid(a), dim3(kWave * kWavesPerBlock), 0, s, a);
+    hipLaunchKernelGGL((stage_kernel<T, STAGE, A, SRC, false>), grid, block, 0, s, a);
+}
+
+template <class T, int STAGE, class A> void launch_stage_as(const StageArgs<T>& a, hipStream_t s) {
+  if (stage_has_sources(a)) launch_stage_src<T, STAGE, A, true>(a, s);
+  else launch_stage_src<T, STAGE, A, false>(a, s);
 }
 
 template <class T, int STAGE> int launch_stage(const StageArgs<T>& a, hipStream_t s) {
