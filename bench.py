@@ -524,10 +524,15 @@ def main():
     valu = None
     if n == 8192 and tfile.exists():
         prof = json.loads(tfile.read_text())
-        traffic = round(prof["step_hbm_bytes"])
+        # the committed counters of this run's strip height when profiled, else the default's
+        rows_run = int(model.rows_hint)
+        at = prof.get("by_rows", {}).get(str(rows_run))
+        rows_prof = rows_run if at is not None else prof.get("rows")
+        at = at or prof
+        traffic = round(at["step_hbm_bytes"])
         # second roof: VALU issue.  Instructions per cell from the committed SQ_INSTS_VALU run of
         # this workload, the ceiling from the probe kernel measured here.
-        per_cell = prof["stage1"]["valu_lane_instr_per_cell"] + prof["stage2"]["valu_lane_instr_per_cell"]
+        per_cell = at["stage1"]["valu_lane_instr_per_cell"] + at["stage2"]["valu_lane_instr_per_cell"]
         got = per_cell * cells / (step_kernel_ms * 1e-3)
         mix = prof.get("valu_mix_probe")
         sec_probe = valu_mix_probe_seconds(dev, MIX_PROBE_BLOCKS, MIX_PROBE_ITERS)
@@ -544,10 +549,12 @@ def main():
             note = "peak is the measured issue rate of a plain FP32 mul/add stream"
         valu = {"bound": "valu", "achieved": round(got / 1e12, 2), "peak": round(peak / 1e12, 2),
                 "unit": "T lane-instr/s", "frac": round(got / peak, 4),
-                "lane_instr_per_cell_update": round(per_cell, 1), "source": src, "note": note}
+                "lane_instr_per_cell_update": round(per_cell, 1), "rows_per_strip_profiled": rows_prof,
+                "source": src, "note": note}
         print(f"# VALU: {got / 1e12:.2f} of {peak / 1e12:.2f} T lane-instr/s ({got / peak:.2f})", file=sys.stderr,
               flush=True)
-        traffic_src = "profiles/basin8192_fp32_traffic.json (FETCH_SIZE x2.0 + WRITE_SIZE x1.0, copy-calibrated)"
+        traffic_src = (f"profiles/basin8192_fp32_traffic.json at {rows_prof} rows per strip (FETCH_SIZE "
+                       f"x{prof['fetch_factor']} + WRITE_SIZE x{prof['write_factor']}, copy-calibrated)")
 
     inv = None
     if a.inversion_epochs > 0:
