@@ -423,6 +423,7 @@ def warmstart_leg(dev, n_scenarios, epochs):
             "one_off": {"ensemble_seconds": round(rep["ensemble"]["seconds"], 2),
                         "ensemble_rows_tuning_seconds": round(rep["ensemble"].get("tune_seconds", 0.0), 2),
                         "ensemble_gcell_per_s": round(rep["ensemble"]["gcell_per_s"], 2),
+                        "ensemble_arithmetic": rep["ensemble"].get("arithmetic"),
                         "training_seconds": round(rep["surrogate"]["seconds"], 2)},
             "ratio_warm_over_plain": None if t_warm is None else round(t_warm / plain["seconds"], 4),
             "ratio_with_one_off": None if t_warm is None else round((t_warm + one_off) / plain["seconds"], 4)}

@@ -14,7 +14,7 @@ import torch
 HERE = Path(__file__).resolve().parent
 LIB_PATH = HERE / "libswe2d_hip.so"
 
-SWE_ABI_VERSION = 3
+SWE_ABI_VERSION = 4
 BC_CODES = {"transmissive": 0, "periodic": 1, "constant": 2, "water_level": 3, "wall": 4, "absorbing": 5}
 FRICTION_CODES = {"manning": 0}
 
@@ -138,7 +138,7 @@ _SIGNATURES = {
     "swe_ensemble_workspace_bytes": ([_I], ctypes.c_size_t),
     "swe_ensemble_last_error_message": ([], ctypes.c_char_p),
     "swe_ensemble_init": ([_C, _I, _P, _P, ctypes.c_size_t, ctypes.POINTER(ctypes.c_longlong), _P], _I),
-    "swe_ensemble_advance": ([_P, _I, ctypes.c_longlong, _I, _I, _P], _I),
+    "swe_ensemble_advance": ([_P, _I, ctypes.c_longlong, _I, _I, _I, _P], _I),
     "swe_ensemble_clock_offset": ([_I], ctypes.c_size_t),
     "swe_ensemble_clock_bytes": ([], ctypes.c_size_t),
     "sur_last_error_message": ([], ctypes.c_char_p),

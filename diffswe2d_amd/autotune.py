@@ -97,23 +97,23 @@ def ensemble_mix(scenarios):
     return mix
 
 
-def _ens_key(device, mix):
+def _ens_key(device, mix, arithmetic="reference"):
     name = torch.cuda.get_device_name(device) if torch.cuda.is_available() else "cpu"
     sizes = ",".join(f"{ny}x{nx}:{c}" for (ny, nx), c in sorted(mix.items()))
-    return f"{name}|ensemble|{sizes}|float32"
+    return f"{name}|ensemble|{sizes}|float32|{arithmetic}"
 
 
-def lookup_ensemble(device, mix) -> dict:
-    """Tuned rows {(ny, nx): S} for an ensemble of this mix of grid sizes, or {} when not tuned."""
-    entry = _load().get(_ens_key(device, mix), {})
+def lookup_ensemble(device, mix, arithmetic="reference") -> dict:
+    """Tuned rows {(ny, nx): S} for an ensemble of this mix of grid sizes and arithmetic, or {}."""
+    entry = _load().get(_ens_key(device, mix, arithmetic), {})
     return {tuple(int(v) for v in k.split("x")): int(S) for k, S in entry.get("rows", {}).items()}
 
 
-def _time_ensemble(cfg, scenarios, rows, device, steps):
+def _time_ensemble(cfg, scenarios, rows, device, steps, arithmetic):
     from .ensemble import Ensemble
 
     ens = Ensemble(cfg, [dict(sc, rows=rows[tuple(int(v) for v in sc["bed"].shape)]) for sc in scenarios],
-                   device, graph=False)
+                   device, graph=False, arithmetic=arithmetic)
     stream = _lib.stream_ptr(device)
     ens._advance(0, 2, stream)
     ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
@@ -124,7 +124,8 @@ def _time_ensemble(cfg, scenarios, rows, device, steps):
     return ev[0].elapsed_time(ev[1]) / steps
 
 
-def tune_ensemble_rows(cfg, scenarios, device, candidates=ENSEMBLE_CANDIDATES, steps=32, save=True):
+def tune_ensemble_rows(cfg, scenarios, device, candidates=ENSEMBLE_CANDIDATES, steps=32, save=True,
+                       arithmetic="reference"):
     """Rows per strip for each grid size of an ensemble; returns ({(ny, nx): S}, report).
 
     The whole ensemble is timed as it will run (every scenario, one launch per
@@ -132,7 +133,7 @@ def tune_ensemble_rows(cfg, scenarios, device, candidates=ENSEMBLE_CANDIDATES, s
     ensemble built from copies of the inputs.  One pass of coordinate descent,
     largest share of the cells first: each size's S is varied over the
     candidates with the others held, and the fastest kept.  With ``save`` the
-    result goes to the table under (device, mix of sizes)."""
+    result goes to the table under (device, mix of sizes, arithmetic)."""
     if steps % 2:
         raise ValueError("steps must be even (the ensemble's buffers alternate every step)")
     device = torch.device(device)
@@ -145,7 +146,7 @@ def tune_ensemble_rows(cfg, scenarios, device, candidates=ENSEMBLE_CANDIDATES, s
     def trial(rows):
         key = tuple(rows[sh] for sh in shapes)
         if key not in timed:
-            timed[key] = _time_ensemble(cfg, scenarios, rows, device, steps)
+            timed[key] = _time_ensemble(cfg, scenarios, rows, device, steps, arithmetic)
         return timed[key]
 
     best_ms = trial(current)
@@ -162,7 +163,7 @@ def tune_ensemble_rows(cfg, scenarios, device, candidates=ENSEMBLE_CANDIDATES, s
               "trials": {",".join(f"{ny}x{nx}:{S}" for (ny, nx), S in zip(shapes, key)): round(ms, 4)
                          for key, ms in timed.items()}}
     if save:
-        _load()[_ens_key(device, mix)] = report
+        _load()[_ens_key(device, mix, arithmetic)] = report
         try:
             TABLE.write_text(json.dumps(_load(), indent=1, sort_keys=True))
         except OSError:

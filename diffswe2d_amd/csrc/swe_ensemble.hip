@@ -67,7 +67,7 @@ __device__ inline int find_scenario(const EnsEntry* tab, int n, long long wid) {
   return lo;
 }
 
-template <int STAGE>
+template <int STAGE, class A>
 __global__ __launch_bounds__(256) void ens_stage_kernel(const EnsEntry* tab, EnsClock* clk, const EnsHeader* hdr,
                                                         int parity) {
   const int lane = threadIdx.x & (kWave - 1);
@@ -96,7 +96,7 @@ __global__ __launch_bounds__(256) void ens_stage_kernel(const EnsEntry* tab, Ens
     a.out = parity ? e.Ua : e.Ub;
   }
   float ax, ay;
-  stage_strip<float, STAGE, RefRound, false, false>(a, strip, seg, 0, dt, half, ax, ay);  // no sources, no k2 record
+  stage_strip<float, STAGE, A, false, false>(a, strip, seg, 0, dt, half, ax, ay);  // no sources, no k2 record
   if (STAGE == 1) {
     ax = wave_max(ax);
     ay = wave_max(ay);
@@ -274,8 +274,11 @@ int swe_ensemble_init(const swe_config* cfg, int n, const swe_scenario* sc, void
   return 0;
 }
 
-int swe_ensemble_advance(void* ws, int n, long long total_waves, int first_step, int n_steps, void* stream) {
+int swe_ensemble_advance(void* ws, int n, long long total_waves, int first_step, int n_steps, int arithmetic,
+                         void* stream) {
   if (!ws || n <= 0 || total_waves <= 0 || first_step < 0 || n_steps < 0) return efail(SWE_ERR_ARG, "bad argument");
+  if (arithmetic != SWE_ARITH_REFERENCE && arithmetic != SWE_ARITH_FAST) return efail(SWE_ERR_ARG, "bad arithmetic");
+  const bool fast = arithmetic == SWE_ARITH_FAST;
   EnsLayout L = ens_layout(n);
   const EnsEntry* tab = (const EnsEntry*)((char*)ws + L.tab);
   EnsClock* clk = (EnsClock*)((char*)ws + L.clk);
@@ -284,8 +287,13 @@ int swe_ensemble_advance(void* ws, int n, long long total_waves, int first_step,
   dim3 grid((unsigned)((total_waves + kWavesPerBlock - 1) / kWavesPerBlock)), block(kWave * kWavesPerBlock);
   for (int g = first_step; g < first_step + n_steps; ++g) {
     int p = g & 1;
-    hipLaunchKernelGGL((ens_stage_kernel<1>), grid, block, 0, st, tab, clk, (const EnsHeader*)hdr, p);
-    hipLaunchKernelGGL((ens_stage_kernel<2>), grid, block, 0, st, tab, clk, (const EnsHeader*)hdr, p);
+    if (fast) {
+      hipLaunchKernelGGL((ens_stage_kernel<1, FastRound>), grid, block, 0, st, tab, clk, (const EnsHeader*)hdr, p);
+      hipLaunchKernelGGL((ens_stage_kernel<2, FastRound>), grid, block, 0, st, tab, clk, (const EnsHeader*)hdr, p);
+    } else {
+      hipLaunchKernelGGL((ens_stage_kernel<1, RefRound>), grid, block, 0, st, tab, clk, (const EnsHeader*)hdr, p);
+      hipLaunchKernelGGL((ens_stage_kernel<2, RefRound>), grid, block, 0, st, tab, clk, (const EnsHeader*)hdr, p);
+    }
     hipLaunchKernelGGL(ens_clock_kernel, dim3((n + 63) / 64), dim3(64), 0, st, tab, clk, hdr, p);
   }
   hipError_t err = hipGetLastError();

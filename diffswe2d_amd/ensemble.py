@@ -21,11 +21,18 @@ from . import _lib, autotune
 
 
 class Ensemble:
-    def __init__(self, cfg, scenarios, device, graph=True, steps_per_graph=64):
+    def __init__(self, cfg, scenarios, device, graph=True, steps_per_graph=64, arithmetic="reference"):
         """scenarios: dicts with bed [ny,nx], rough [ny,nx], dx, dy, U0 [3,ny,nx], levels [n_chunks],
         chunk (s), gauges [(iy, ix)], optional rows (rows per wavefront strip; without it the
-        rows tuned for that grid size and count (``autotune.tune_ensemble_rows``), else the
-        library's heuristic)."""
+        rows tuned for that mix of grid sizes and this arithmetic (``autotune.tune_ensemble_rows``),
+        else the library's heuristic).
+
+        arithmetic: "reference" (the reference's rounding; every scenario equals its batch-1
+        ``SWE2D.forward`` run bit for bit) or "fast" (``SWE2D.arithmetic = "fast"``'s
+        tolerance-bounded FP32 arithmetic; equal to batch-1 runs made with it)."""
+        if arithmetic not in ("reference", "fast"):
+            raise ValueError(f"arithmetic must be 'reference' or 'fast', not {arithmetic!r}")
+        self.arithmetic = arithmetic
         if steps_per_graph % 2:
             raise ValueError("steps_per_graph must be even (buffers alternate every step)")
         self.lib = _lib.load()
@@ -38,7 +45,7 @@ class Ensemble:
         self.rows = []  # rows per strip asked for each scenario (0: the library's heuristic)
         table = (_lib.SweScenario * self.n)()
         self.views = []
-        tuned = autotune.lookup_ensemble(self.device, autotune.ensemble_mix(scenarios))
+        tuned = autotune.lookup_ensemble(self.device, autotune.ensemble_mix(scenarios), arithmetic)
         f32 = dict(dtype=torch.float32, device=self.device)
         for i, sc in enumerate(scenarios):
             ny, nx = np.asarray(sc["bed"]).shape
@@ -77,7 +84,8 @@ class Ensemble:
         self._graph = None
 
     def _advance(self, first, count, stream):
-        rc = self.lib.swe_ensemble_advance(_lib.ptr(self.ws), self.n, self.total_waves, first, count, stream)
+        rc = self.lib.swe_ensemble_advance(_lib.ptr(self.ws), self.n, self.total_waves, first, count,
+                                           int(self.arithmetic == "fast"), stream)
         if rc != 0:
             raise RuntimeError(f"ensemble advance failed: {self.lib.swe_ensemble_last_error_message().decode()}")
 
@@ -176,16 +184,17 @@ if __name__ == "__main__":
     ap.add_argument("--scenarios", type=int, default=256)
     ap.add_argument("--chunks", type=int, default=450)
     ap.add_argument("--tune", action="store_true", help="tune rows per grid size first (cached in the table)")
+    ap.add_argument("--arithmetic", default="reference", choices=("reference", "fast"))
     a = ap.parse_args()
     dev = torch.device("cuda")
     sc, _ = scenarios.ensemble_scenarios(n=a.scenarios, n_chunks=a.chunks)
     tune_s, tuned = 0.0, None
     if a.tune:
         t0 = time.perf_counter()
-        _, tuned = autotune.tune_ensemble_rows(runup_config(), sc, dev)
+        _, tuned = autotune.tune_ensemble_rows(runup_config(), sc, dev, arithmetic=a.arithmetic)
         tune_s = time.perf_counter() - t0
     t0 = time.perf_counter()
-    ens = Ensemble(runup_config(), sc, dev)
+    ens = Ensemble(runup_config(), sc, dev, arithmetic=a.arithmetic)
     torch.cuda.synchronize(dev)
     t1 = time.perf_counter()
     tl = [(0, t1, 0)]
@@ -196,7 +205,7 @@ if __name__ == "__main__":
     own = [r["steps"] for r in res]
     total = float(np.dot(cells, own))
     print(json.dumps(dict(
-        scenarios=a.scenarios, chunks=a.chunks, global_steps=steps, cell_updates=total,
+        scenarios=a.scenarios, chunks=a.chunks, arithmetic=a.arithmetic, global_steps=steps, cell_updates=total,
         rows={f"{s['bed'].shape[0]}x{s['bed'].shape[1]}": r for s, r in zip(sc, ens.rows)},
         tune_seconds=round(tune_s, 2), tuning=tuned,
         construct_seconds=round(t1 - t0, 3), run_seconds=round(t2 - t1, 3),

@@ -5,7 +5,11 @@ Examples/inverse/.../bc_inversion_train.py):
 
 1. ``generate_pairs``: an ensemble of seeded scenarios (uneven grids, own dt,
    csrc/swe_ensemble.hip) produces (boundary series, 6 gauge series) pairs,
-   sampled at the same physical gauge points at every resolution.
+   sampled at the same physical gauge points at every resolution.  The pairs
+   only train the surrogate, so by default the ensemble runs the library's
+   tolerance-bounded fast FP32 arithmetic (gauge series within ~1e-6 of the
+   reference rounding's; the surrogate's own error is ~1e-2).  The inversions
+   themselves and the plain-solver judgement use the reference rounding.
 2. ``surrogate.train``: the BF16 MFMA 1-D conv surrogate learns series -> gauges,
    conditioned on the resolution (log2 of dx / 1 cm) of the run.
 3. ``surrogate_start``: the controls that make the surrogate match the observed
@@ -38,8 +42,10 @@ def resolution_condition(dx):
 
 
 def generate_pairs(n, device, seed=5, n_controls=64, n_chunks=450, sample_dt=0.05, grids=scenarios.RUNUP_GRIDS,
-                   log=None, tune=True):
+                   log=None, tune=True, arithmetic="fast"):
     """Run the ensemble; returns (series [n, n_chunks], gauges [n, 6, n_chunks], controls, stats).
+
+    arithmetic: the ensemble's FP32 arithmetic ("fast" or "reference", see ``Ensemble``).
 
     With ``tune`` the rows per strip of each grid size are tuned first unless
     the table already holds them (``autotune.tune_ensemble_rows``; its time is
@@ -49,12 +55,12 @@ def generate_pairs(n, device, seed=5, n_controls=64, n_chunks=450, sample_dt=0.0
                                             sample_dt=sample_dt, grids=grids)
     tune_s = 0.0
     if tune:
-        if not autotune.lookup_ensemble(device, autotune.ensemble_mix(sc)):
+        if not autotune.lookup_ensemble(device, autotune.ensemble_mix(sc), arithmetic):
             t = time.perf_counter()
-            autotune.tune_ensemble_rows(runup_config(), sc, device)
+            autotune.tune_ensemble_rows(runup_config(), sc, device, arithmetic=arithmetic)
             tune_s = time.perf_counter() - t
     t0 = time.perf_counter()
-    ens = Ensemble(runup_config(), sc, device)
+    ens = Ensemble(runup_config(), sc, device, arithmetic=arithmetic)
     steps = ens.run(log=log)
     sec = time.perf_counter() - t0
     res = ens.results()
@@ -62,7 +68,8 @@ def generate_pairs(n, device, seed=5, n_controls=64, n_chunks=450, sample_dt=0.0
     series = torch.tensor(np.stack([s["levels"] for s in sc]), dtype=torch.float32, device=device)
     cells = sum(s["bed"].size * r["steps"] for s, r in zip(sc, res))
     cond = torch.tensor([[resolution_condition(s["dx"])] for s in sc], dtype=torch.float32, device=device)
-    stats = dict(scenarios=n, seconds=sec, tune_seconds=tune_s, global_steps=steps, cell_updates=cells,
+    stats = dict(scenarios=n, arithmetic=arithmetic, seconds=sec, tune_seconds=tune_s, global_steps=steps,
+                 cell_updates=cells,
                  gcell_per_s=cells / sec / 1e9, per_scenario_steps=[r["steps"] for r in res], cond=cond)
     if log:
         log(f"ensemble: {n} scenarios, {cells / 1e9:.2f} G cell-updates in {sec:.2f} s "
@@ -166,7 +173,8 @@ class InversionSetup:
 
 
 def run(device, n_scenarios=256, n_controls=64, n_chunks=450, grids=scenarios.RUNUP_GRIDS, epochs_plain=60,
-        surrogate_epochs=40, channels=160, accept=0.5, log=print, seed=5, n_starts=8, truth_seed=4):
+        surrogate_epochs=40, channels=160, accept=0.5, log=print, seed=5, n_starts=8, truth_seed=4,
+        ensemble_arithmetic="fast"):
     """Full pipeline; returns a report with the time-to-misfit of both inversions.
 
     The plain inversion starts from the prior (level 0) and runs ``epochs_plain``
@@ -186,7 +194,8 @@ def run(device, n_scenarios=256, n_controls=64, n_chunks=450, grids=scenarios.RU
     report["plain"] = dict(seconds=t_plain, epochs=len(hist_p), best_misfit=bar, history=hist_p,
                            steps_per_forward=setup.problem.stats.get("steps"))
     series, gauges, ctrl, stats = generate_pairs(n_scenarios, device, seed=seed, n_controls=n_controls,
-                                                 n_chunks=n_chunks, grids=grids, log=log)
+                                                 n_chunks=n_chunks, grids=grids, log=log,
+                                                 arithmetic=ensemble_arithmetic)
     report["ensemble"] = {k: v for k, v in stats.items() if k not in ("per_scenario_steps", "cond")}
     net = Surrogate(default_specs(channels=channels, n_cond=1), device=device, seed=0)
     t1 = time.perf_counter()

@@ -28,7 +28,7 @@
 extern "C" {
 #endif
 
-#define SWE_ABI_VERSION 3
+#define SWE_ABI_VERSION 4
 
 enum {
   SWE_OK = 0,
@@ -474,9 +474,13 @@ const char* swe_ensemble_last_error_message(void);
 int swe_ensemble_init(const swe_config* cfg, int n_scenarios, const swe_scenario* scenarios, void* ws,
                       size_t ws_bytes, long long* total_waves, void* stream);
 /* Launches n_steps global steps (3 launches each); finished scenarios exit at
- * once.  Capturable in a HIP graph (parity comes from first_step + i). */
+ * once.  Capturable in a HIP graph (parity comes from first_step + i).
+ * arithmetic: SWE_ARITH_REFERENCE (the reference's rounding, results equal to
+ * batch-1 swe_advance_f32 runs bit for bit) or SWE_ARITH_FAST (the
+ * tolerance-bounded arithmetic of swe_config.arithmetic; equal to batch-1 runs
+ * with that arithmetic). */
 int swe_ensemble_advance(void* ws, int n_scenarios, long long total_waves, int first_step, int n_steps,
-                         void* stream);
+                         int arithmetic, void* stream);
 size_t swe_ensemble_clock_offset(int n_scenarios);
 size_t swe_ensemble_clock_bytes(void);
 

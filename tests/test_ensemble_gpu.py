@@ -75,3 +75,38 @@ def test_rows_per_strip_change_nothing_but_time(cuda_device, monkeypatch, tmp_pa
             assert a["steps"] == b["steps"]
             assert torch.equal(a["U"], b["U"])
             assert torch.equal(a["gauges"], b["gauges"])
+
+
+def test_fast_arithmetic_matches_fast_single_runs(cuda_device):
+    """arithmetic="fast": every scenario equals its batch-1 SWE2D.forward run made with
+    SWE2D.arithmetic = "fast", bit for bit, and stays within the parity bars of the
+    reference-rounding ensemble."""
+    cfg = swe.SWEConfig(boundary_left="water_level", boundary_right="wall", boundary_top="wall",
+                        boundary_bottom="wall")
+    sc = make()
+    runs = {}
+    for arith in ("reference", "fast"):
+        ens = Ensemble(cfg, sc, cuda_device, graph=True, steps_per_graph=16, arithmetic=arith)
+        ens.run()
+        runs[arith] = ens.results()
+    for i, s in enumerate(sc):
+        m = swe.SWE2D(torch.tensor(s["bed"], dtype=torch.float32)[None, None],
+                      torch.tensor(s["rough"], dtype=torch.float32)[None, None], s["dx"], s["dy"],
+                      config=cfg, train_dem=False).to(cuda_device)
+        m.arithmetic = "fast"
+        U = torch.tensor(s["U0"], dtype=torch.float32, device=cuda_device)[None]
+        with torch.no_grad():
+            for k, lvl in enumerate(s["levels"]):
+                U, _ = m(U, t_end=s["chunk"], start_time=k * s["chunk"], boundary_level=float(np.float32(lvl)))
+        fast, ref = runs["fast"][i], runs["reference"][i]
+        assert torch.equal(U[0], fast["U"]), f"scenario {i}: fast ensemble differs from the fast single run"
+        rel = float(torch.linalg.norm((fast["U"] - ref["U"]).double()) / torch.linalg.norm(ref["U"].double()))
+        print(f"scenario {i}: fast vs reference rounding rel-L2 {rel:.2e}, steps {fast['steps']}/{ref['steps']}")
+        assert rel <= 1e-5
+        assert torch.equal(fast["U"][0] > 1e-6, ref["U"][0] > 1e-6)
+
+
+def test_bad_arithmetic_is_refused(cuda_device):
+    cfg = swe.SWEConfig(boundary_left="wall", boundary_right="wall", boundary_top="wall", boundary_bottom="wall")
+    with pytest.raises(ValueError, match="arithmetic"):
+        Ensemble(cfg, make(n=1, n_chunks=4), cuda_device, arithmetic="approximate")
