@@ -488,16 +488,17 @@ __global__ __launch_bounds__(256) void adj_ghost_side_kernel(AdjArgs<T> a, int s
 
 // Small grids: all four sides in one block per member (sides in turn, separated
 // by barriers), level adjoint summed in the block -- one launch instead of five.
-// PREFETCH (sides of at most 256 cells): one position per thread on every side,
-// all four sides' inputs loaded at once, then the sides applied in turn; the
-// 256-thread bound leaves the loaded inputs their registers.
-template <class T, bool PREFETCH>
-__global__ __launch_bounds__(PREFETCH ? 256 : 1024) void adj_ghost_all_kernel(AdjArgs<T> a, int stage,
-                                                                              T* lam_level) {
-  __shared__ T part[PREFETCH ? 256 : 1024];
+// PREFETCH > 0 (sides of at most PREFETCH cells, blocks of at most PREFETCH
+// threads): one position per thread on every side, all four sides' inputs loaded
+// at once -- one memory round trip instead of four -- then the sides applied in
+// turn; the thread bound leaves the loaded inputs their registers.
+template <class T, int PREFETCH>
+__global__ __launch_bounds__(PREFETCH > 0 ? PREFETCH : 1024) void adj_ghost_all_kernel(AdjArgs<T> a, int stage,
+                                                                                    T* lam_level) {
+  __shared__ T part[PREFETCH > 0 ? PREFETCH : 1024];
   const int b = blockIdx.x;
   T glev = T(0);
-  if (PREFETCH) {
+  if constexpr (PREFETCH > 0) {
     FoldIn<T> in[4];
     for (int side = 0; side < 4; ++side) {
       const int len = (side == LEFT || side == RIGHT) ? a.f.ny : a.f.nx;
@@ -620,10 +621,17 @@ void ghost_fold(const AdjArgs<T>& a, int stage, int batch, T* lam_level, hipStre
     // the fold's four phases and the sum then synchronise as few wavefronts as possible
     int threads = 64;
     while (threads < a.lmax && threads < 1024) threads <<= 1;
+    // FP64 inputs of four sides take twice the registers: prefetch up to 512 positions
+    constexpr int kPrefetchMax = sizeof(T) == 4 ? 1024 : 512;
+    const dim3 grid(batch), block(threads);
     if (a.lmax <= 256)
-      hipLaunchKernelGGL((adj_ghost_all_kernel<T, true>), dim3(batch), dim3(threads), 0, s, a, stage, lam_level);
+      hipLaunchKernelGGL((adj_ghost_all_kernel<T, 256>), grid, block, 0, s, a, stage, lam_level);
+    else if (a.lmax <= 512)
+      hipLaunchKernelGGL((adj_ghost_all_kernel<T, 512>), grid, block, 0, s, a, stage, lam_level);
+    else if (a.lmax <= kPrefetchMax)
+      hipLaunchKernelGGL((adj_ghost_all_kernel<T, kPrefetchMax>), grid, block, 0, s, a, stage, lam_level);
     else
-      hipLaunchKernelGGL((adj_ghost_all_kernel<T, false>), dim3(batch), dim3(threads), 0, s, a, stage, lam_level);
+      hipLaunchKernelGGL((adj_ghost_all_kernel<T, 0>), grid, block, 0, s, a, stage, lam_level);
     return;
   }
   dim3 grid((unsigned)a.part_blocks, (unsigned)batch);
