@@ -210,8 +210,16 @@ SWE_HD void face_flux_bwd(const Side<T>& L, T hL_cell, T zL_cell, const Side<T>&
   T hL = clamp_lo(hcL, T(0)), hR = clamp_lo(hcR, T(0));
   bool wetL = hL > k.dry, wetR = hR > k.dry;
   T sfL = clamp_lo(hL, k.floor), sfR = clamp_lo(hR, k.floor);
-  T nL = wetL ? div_rn(qnL, sfL) : T(0), tL = wetL ? div_rn(qtL, sfL) : T(0), cL = wetL ? sqrt_rn(k.g * sfL) : T(0);
-  T nR = wetR ? div_rn(qnR, sfR) : T(0), tR = wetR ? div_rn(qtR, sfR) : T(0), cR = wetR ? sqrt_rn(k.g * sfR) : T(0);
+  // the forward's face_flux forms: the same values
+  T nL = T(0), tL = T(0), nR = T(0), tR = T(0), cL = T(0), cR = T(0);
+  if (wetL) {
+    adiv2<RefRound>(qnL, qtL, sfL, nL, tL);
+    cL = sqrt_rn(k.g * sfL);
+  }
+  if (wetR) {
+    adiv2<RefRound>(qnR, qtR, sfR, nR, tR);
+    cR = sqrt_rn(k.g * sfR);
+  }
   T um = (T(0.5) * (nL + nR) + cL) - cR;
   T cm = T(0.5) * (cL + cR) + T(0.25) * (nL - nR);
   bool dryL = hL <= k.dry, dryR = hR <= k.dry;

@@ -27,8 +27,22 @@ template <class T> SWE_HD T tmax(T a, T b) { return __builtin_elementwise_maximu
 template <class T> SWE_HD T tmin(T a, T b) { return __builtin_elementwise_minimum(a, b); }
 template <class T> SWE_HD T clamp_lo(T x, T lo) { return __builtin_elementwise_maximum(x, lo); }
 template <class T> SWE_HD T clamp_hi(T x, T hi) { return __builtin_elementwise_minimum(x, hi); }
-template <class T> SWE_HD T clamp2(T x, T lo, T hi) { return clamp_hi(clamp_lo(x, lo), hi); }
 template <class T> SWE_HD T tabs(T x) { return __builtin_fabs(x); }
+
+// Median of three for lo <= hi: max(lo, min(x, hi)).  FP32 device code issues
+// gfx950's single v_med3_f32 (min and max in one instruction); the value is
+// the same as the two-instruction form for every ordered input, up to the sign
+// of an exact zero (unobservable, as above).  Used where an input cannot be a
+// NaN short of a non-finite state, which fails the step's dt check anyway.
+template <class T> SWE_HD T med3(T x, T lo, T hi) { return clamp_hi(clamp_lo(x, lo), hi); }
+template <> SWE_HD float med3<float>(float x, float lo, float hi) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return __builtin_amdgcn_fmed3f(x, lo, hi);
+#else
+  return clamp_hi(clamp_lo(x, lo), hi);
+#endif
+}
+template <class T> SWE_HD T clamp2(T x, T lo, T hi) { return med3(x, lo, hi); }
 
 // ---------------------------------------------------------------- division
 // FP32 a / b rounded like IEEE division.  The sequence is the one the compiler
@@ -50,13 +64,18 @@ SWE_HD float rcp_estimate(float b) {
 #endif
 }
 
-// the sequence from a reciprocal estimate r within an ulp of 1 / b
-SWE_HD float div_rn_from(float a, float b, float r) {
-  r = __builtin_fmaf(__builtin_fmaf(-b, r, 1.0f), r, r);
+// the Newton step on a reciprocal estimate r within an ulp of 1 / b
+SWE_HD float rcp_refine(float b, float r) { return __builtin_fmaf(__builtin_fmaf(-b, r, 1.0f), r, r); }
+
+// the quotient and its two remainder corrections from a refined reciprocal r
+SWE_HD float div_rn_refined(float a, float b, float r) {
   float q = a * r;
   q = __builtin_fmaf(__builtin_fmaf(-b, q, a), r, q);
   return __builtin_fmaf(__builtin_fmaf(-b, q, a), r, q);
 }
+
+// the whole sequence from a reciprocal estimate r
+SWE_HD float div_rn_from(float a, float b, float r) { return div_rn_refined(a, b, rcp_refine(b, r)); }
 
 template <> SWE_HD float div_rn<float>(float a, float b) { return div_rn_from(a, b, rcp_estimate(b)); }
 
@@ -136,6 +155,14 @@ template <class A> inline constexpr bool is_fast = false;
 template <> inline constexpr bool is_fast<FastRound> = true;
 
 template <class A, class T> SWE_HD T adiv(T a, T b) { return div_rn(a, b); }
+
+// a1 / b and a2 / b, each rounded as adiv rounds it, from one reciprocal of b
+// (the compiler does not always share it when the quotients sit in separate
+// branches)
+template <class A, class T> SWE_HD void adiv2(T a1, T a2, T b, T& q1, T& q2) {
+  q1 = adiv<A>(a1, b);
+  q2 = adiv<A>(a2, b);
+}
 template <class A, class T> SWE_HD T adiv_by(T a, const Divisor<T>& d) { return div_by(a, d); }
 template <class A, class T> SWE_HD T asqrt(T x) { return sqrt_rn(x); }
 
@@ -147,6 +174,15 @@ template <> SWE_HD float adiv<FastRound, float>(float a, float b) {
 template <> SWE_HD float adiv_by<FastRound, float>(float a, const Divisor<float>& d) {
   const float q = a * d.r;
   return __builtin_fmaf(__builtin_fmaf(-d.b, q, a), d.r, q);
+}
+template <> SWE_HD void adiv2<RefRound, float>(float a1, float a2, float b, float& q1, float& q2) {
+  const float r = rcp_refine(b, rcp_estimate(b));
+  q1 = div_rn_refined(a1, b, r);
+  q2 = div_rn_refined(a2, b, r);
+}
+template <> SWE_HD void adiv2<FastRound, float>(float a1, float a2, float b, float& q1, float& q2) {
+  q1 = adiv<FastRound>(a1, b);
+  q2 = adiv<FastRound>(a2, b);
 }
 template <> SWE_HD float asqrt<FastRound, float>(float x) {
 #if defined(__HIP_DEVICE_COMPILE__)
@@ -161,9 +197,13 @@ template <> SWE_HD float asqrt<FastRound, float>(float x) {
 // all three share a strict sign, else 0 -- equal to sign(a)*min(|a|,|b|,|c|).
 // All three are > 0 exactly when their minimum is (and < 0 when their maximum
 // is); a NaN makes both tests false, as it does for the three separate tests.
+// With lo <= hi that is 0 clamped to [lo, hi]: one median instruction in FP32
+// (a NaN argument, which the three tests map to 0, only arises from a
+// non-finite state).
 template <class T> SWE_HD T minmod3(T a, T b, T c) {
   T lo = tmin(a, tmin(b, c));
   T hi = tmax(a, tmax(b, c));
+  if constexpr (sizeof(T) == 4) return med3(T(0), lo, hi);
   return lo > T(0) ? lo : (hi < T(0) ? hi : T(0));
 }
 
@@ -187,8 +227,9 @@ SWE_HD void primitive_cell(T h_raw, T hu, T hv, T z, const SweConsts<T>& k, T& h
   h = clamp_lo(h_raw, T(0));
   bool wet = h > k.dry;
   T safe = clamp_lo(h, k.floor);
-  u = wet ? adiv<A>(hu, safe) : T(0);
-  v = wet ? adiv<A>(hv, safe) : T(0);
+  u = T(0);
+  v = T(0);
+  if (wet) adiv2<A>(hu, hv, safe, u, v);
   eta = h + z;
 }
 
@@ -291,21 +332,26 @@ SWE_HD FaceFlux<T> face_flux(const Side<T>& L, T hL_cell, T zL_cell, const Side<
   T hL = clamp_lo(hcL, T(0)), hR = clamp_lo(hcR, T(0));
   bool wetL = hL > k.dry, wetR = hR > k.dry;
   T sfL = clamp_lo(hL, k.floor), sfR = clamp_lo(hR, k.floor);
-  T nL, tL, nR, tR;
+  T nL = T(0), tL = T(0), nR = T(0), tR = T(0), cL = T(0), cR = T(0);
   if constexpr (is_fast<A> && sizeof(T) == 4) {
     // the face velocity is the reconstructed one: (h_cn * u) / h_cn without the re-division
     nL = wetL ? L.un : T(0);
     tL = wetL ? L.ut : T(0);
     nR = wetR ? R.un : T(0);
     tR = wetR ? R.ut : T(0);
+    cL = wetL ? asqrt<A>(k.g * sfL) : T(0);
+    cR = wetR ? asqrt<A>(k.g * sfR) : T(0);
   } else {
-    nL = wetL ? div_rn(qnL, sfL) : T(0);
-    tL = wetL ? div_rn(qtL, sfL) : T(0);
-    nR = wetR ? div_rn(qnR, sfR) : T(0);
-    tR = wetR ? div_rn(qtR, sfR) : T(0);
+    // where(wet, q / sf, 0) and where(wet, sqrt(g sf), 0), one branch per side
+    if (wetL) {
+      adiv2<RefRound>(qnL, qtL, sfL, nL, tL);
+      cL = sqrt_rn(k.g * sfL);
+    }
+    if (wetR) {
+      adiv2<RefRound>(qnR, qtR, sfR, nR, tR);
+      cR = sqrt_rn(k.g * sfR);
+    }
   }
-  T cL = wetL ? asqrt<A>(k.g * sfL) : T(0);
-  T cR = wetR ? asqrt<A>(k.g * sfR) : T(0);
 
   T um = (T(0.5) * (nL + nR) + cL) - cR;
   T cm = T(0.5) * (cL + cR) + T(0.25) * (nL - nR);
