@@ -48,6 +48,9 @@ def parse():
                          "0 to skip")
     ap.add_argument("--warmstart-epochs", type=int, default=40, help="adjoint epochs of the plain inversion")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-small", action="store_true", help="skip the dam-break and run-up workloads")
+    ap.add_argument("--small-cpu-steps", type=int, default=40,
+                    help="timed oracle steps of each smaller workload (after 1 warm-up)")
     ap.add_argument("--no-fast", action="store_true",
                     help="skip the leg with the library's tolerance-bounded FP32 arithmetic")
     ap.add_argument("--adjoint-steps", type=int, default=10,
@@ -410,6 +413,107 @@ def cpu_baseline(sc, cfg, steps, run_steps):
                              "for one GPU")
 
 
+def oracle_rate(U, bed, rough, cfg, dx, dy, steps, level=None):
+    """Oracle (eager PyTorch CPU, FP32) steps per second on one grid: 1 warm-up + `steps` timed."""
+    import torch
+
+    from oracle import swe_oracle as O
+
+    with torch.no_grad():
+        U, _ = O.step(U, bed, rough, cfg, dx, dy, 1e30, level=level)
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            U, _ = O.step(U, bed, rough, cfg, dx, dy, 1e30, level=level)
+        return steps / (time.perf_counter() - t0)
+
+
+def small_workloads(dev, cpu_steps):
+    """The two smaller forward workloads, each on the GPU next to the oracle on the host:
+    the 256 x 256 dam break (500 steps, walls, flat bed, n = 0) and the full 544 x 344 coastal
+    run-up (450 chunks of 0.05 s with the west level of its pulse train, 6 gauges sampled on the
+    device).  The 32-B roofline fraction is reported for each; both are latency-bound, one
+    wavefront's strip per SIMD or less."""
+    import torch
+
+    import diffswe2d_amd as swe
+    from diffswe2d_amd import chunked, scenarios
+
+    out = {}
+    # dam break
+    sc = scenarios.dam_break(n=256)
+    cfg = swe.SWEConfig(boundary_left="wall", boundary_right="wall", boundary_top="wall", boundary_bottom="wall")
+    bed, rough = torch.tensor(sc["bed"], dtype=torch.float32), torch.tensor(sc["rough"], dtype=torch.float32)
+    m = swe.SWE2D(bed, rough, sc["dx"], sc["dy"], config=cfg, train_dem=False).to(dev)
+    U = torch.tensor(sc["U"], dtype=torch.float32, device=dev)
+    cells = U.shape[-1] * U.shape[-2]
+    timed_run(m, U, 1, sc["steps"], fast=False)  # warm-up run
+    ev = []
+    sec, _, _ = timed_run(m, U, 1, sc["steps"], fast=False, events=ev)
+    gpu = cells * sc["steps"] / (ev[0] * 1e-3) / 1e9
+    rate = (oracle_rate(torch.tensor(sc["U"], dtype=torch.float32), bed, rough, cfg, sc["dx"], sc["dy"], cpu_steps)
+            if cpu_steps > 0 else None)
+    out["dam_break_256x256"] = {
+        "steps": sc["steps"], "ms_per_step": round(ev[0] / sc["steps"], 4), "gcell_updates_per_s": round(gpu, 3),
+        "hbm_frac_32B": round(gpu * 32 / HBM_PEAK_GBS, 4),
+        "cpu_gcell_updates_per_s": None if rate is None else round(cells * rate / 1e9, 5),
+        "cpu_sample": f"oracle FP32, 1 warm-up + {cpu_steps} steps from the initial state"}
+    # coastal run-up, the whole 22.5 s record
+    sc = scenarios.coastal_runup()
+    cfg = swe.SWEConfig(boundary_left="water_level", boundary_right="wall", boundary_top="wall",
+                        boundary_bottom="wall")
+    bed, rough = torch.tensor(sc["bed"], dtype=torch.float32), torch.tensor(sc["rough"], dtype=torch.float32)
+    m = swe.SWE2D(bed, rough, sc["dx"], sc["dy"], config=cfg, train_dem=False).to(dev)
+    U = torch.tensor(sc["U"], dtype=torch.float32, device=dev)
+    levels = torch.tensor(sc["chunk_levels"], dtype=torch.float32, device=dev)
+    cells = U.shape[-1] * U.shape[-2]
+    with torch.no_grad():
+        chunked.gauge_run(m, U, levels, sc["sample_dt"], sc["gauges"])  # warm-up run
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        chunked.gauge_run(m, U, levels, sc["sample_dt"], sc["gauges"])
+        e1.record()
+        torch.cuda.synchronize()
+    steps = len(m.last_dt_series)
+    ms = e0.elapsed_time(e1)
+    gpu = cells * steps / (ms * 1e-3) / 1e9
+    rate = (oracle_rate(torch.tensor(sc["U"], dtype=torch.float32), bed, rough, cfg, sc["dx"], sc["dy"], cpu_steps,
+                        level=float(sc["chunk_levels"][0])) if cpu_steps > 0 else None)
+    out["coastal_runup_544x344"] = {
+        "chunks": len(sc["chunk_levels"]), "steps": steps, "seconds": round(ms * 1e-3, 4),
+        "ms_per_step": round(ms / steps, 4), "gcell_updates_per_s": round(gpu, 3),
+        "hbm_frac_32B": round(gpu * 32 / HBM_PEAK_GBS, 4),
+        "cpu_gcell_updates_per_s": None if rate is None else round(cells * rate / 1e9, 5),
+        "cpu_sample": f"oracle FP32, 1 warm-up + {cpu_steps} steps from the initial state (first chunk's level)"}
+    return out
+
+
+def per_workload(small, gcells, achieved, cpu, inv, warm, adj):
+    """Gcell-updates/s and the 32-B roofline fraction of every workload the run measured,
+    with the oracle's rate on the host where it was timed."""
+    out = dict(small or {})
+    out["tsunami_basin_8192x8192"] = {
+        "gcell_updates_per_s": round(gcells, 3), "hbm_frac_32B": round(achieved / HBM_PEAK_GBS, 4),
+        "cpu_gcell_updates_per_s": None if cpu is None else round(cpu["value"], 5)}
+    if adj is not None:
+        out["tsunami_basin_8192x8192_adjoint"] = {
+            "reverse_gcell_updates_per_s": adj["value"],
+            "taped_backward_over_forward": adj["taped_backward_over_forward"],
+            "checkpointed_backward_over_forward": adj["checkpointed_backward_over_forward"]}
+    if inv is not None:
+        out["monai_boundary_inversion_68x42_fp64"] = {"seconds_per_epoch": inv["seconds_per_epoch"],
+                                                     "speedup_vs_published": inv["speedup_vs_published"]}
+    if warm is not None:
+        out["runup_boundary_inversion_544x344"] = {
+            "seconds_per_epoch": round(warm["plain"]["seconds"] / max(1, warm["plain"]["epochs"]), 3),
+            "warm_over_plain": warm["ratio_warm_over_plain"], "with_one_off": warm["ratio_with_one_off"]}
+        out["ensemble_256_scenarios"] = {"gcell_updates_per_s": warm["one_off"]["ensemble_gcell_per_s"],
+                                         "arithmetic": warm["one_off"]["ensemble_arithmetic"],
+                                         "hbm_frac_32B": round(warm["one_off"]["ensemble_gcell_per_s"] * 32
+                                                               / HBM_PEAK_GBS, 4)}
+    return out
+
+
 def warmstart_leg(dev, n_scenarios, epochs):
     """The learned warm start on the coastal run-up inversion (544 x 344, 64 spline controls,
     450 chunks): plain adjoint inversion from the prior vs surrogate-started inversion, time to
@@ -569,6 +673,16 @@ def main():
         traffic_src = (f"profiles/basin8192_fp32_traffic.json at {rows_prof} rows per strip (FETCH_SIZE "
                        f"x{prof['fetch_factor']} + WRITE_SIZE x{prof['write_factor']}, copy-calibrated)")
 
+    # the host's CPU share for one GPU, for every oracle leg below
+    torch.set_num_threads(max(1, min(os.cpu_count() or 1, int(os.environ.get("OMP_NUM_THREADS", "16")))))
+    small = None
+    if not a.no_small:
+        small = small_workloads(dev, 0 if a.no_cpu else a.small_cpu_steps)
+        for name, r in small.items():
+            print(f"# {name}: {r['steps']} steps, {r['ms_per_step']:.4f} ms/step, {r['gcell_updates_per_s']:.3f} "
+                  f"Gcell/s ({100 * r['hbm_frac_32B']:.2f} % of 8 TB/s); oracle "
+                  f"{r['cpu_gcell_updates_per_s'] or float('nan'):.5f} Gcell/s", file=sys.stderr, flush=True)
+
     inv = None
     if a.inversion_epochs > 0:
         inv = inversion_leg(dev, a.inversion_epochs)
@@ -585,7 +699,6 @@ def main():
 
     cpu = None
     if not a.no_cpu:
-        torch.set_num_threads(max(1, min(os.cpu_count() or 1, int(os.environ.get("OMP_NUM_THREADS", "16")))))
         cpu = cpu_baseline(sc, cfg, a.cpu_steps, sc["steps"])
         print(f"# cpu baseline {cpu['value']:.4f} Gcell/s ({cpu['sample']})", file=sys.stderr, flush=True)
 
@@ -610,6 +723,7 @@ def main():
         "inversion": inv,
         "warmstart": warm,
         "cpu_baseline": cpu,
+        "per_workload": per_workload(small, gcells, achieved, cpu, inv, warm, adj),
     }
     print(json.dumps(out), flush=True)
 
