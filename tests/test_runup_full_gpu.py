@@ -10,7 +10,7 @@ inversion driver (Examples/inverse/.../bc_inversion_train.py:332-341): every
 step's dt, the six gauge depths at every chunk end, the wet/dry mask every 50
 chunks and the final state.
 
-Bars (FP32, the north star's, after the full run): the same number of steps;
+Bars (FP32, after the full run): the same number of steps;
 final state rel-L2 <= 1e-5 against the reference FP32 run, with its error
 against the reference FP64 run stated; identical wet/dry masks at every stored
 chunk; dt series within 1e-6 relative; gauge depths within 1e-5 of the largest
