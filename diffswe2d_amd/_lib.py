@@ -14,7 +14,7 @@ import torch
 HERE = Path(__file__).resolve().parent
 LIB_PATH = HERE / "libswe2d_hip.so"
 
-SWE_ABI_VERSION = 4
+SWE_ABI_VERSION = 5
 BC_CODES = {"transmissive": 0, "periodic": 1, "constant": 2, "water_level": 3, "wall": 4, "absorbing": 5}
 FRICTION_CODES = {"manning": 0}
 
@@ -34,7 +34,7 @@ class SweConfigC(ctypes.Structure):
                 ("constant_value", ctypes.c_double), ("boundary", ctypes.c_int32 * 4),
                 ("water_slope_reset", ctypes.c_int32), ("positivity_limiter", ctypes.c_int32),
                 ("apply_friction", ctypes.c_int32), ("friction_model", ctypes.c_int32),
-                ("grad_bed", ctypes.c_int32), ("arithmetic", ctypes.c_int32)]
+                ("grad_bed", ctypes.c_int32), ("arithmetic", ctypes.c_int32), ("fixed_dt", ctypes.c_int32)]
 
 
 class SweScenario(ctypes.Structure):

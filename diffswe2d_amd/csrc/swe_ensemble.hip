@@ -216,6 +216,7 @@ int swe_ensemble_init(const swe_config* cfg, int n, const swe_scenario* sc, void
   if (bytes < L.total) return efail(SWE_ERR_ARG, "ensemble workspace too small");
   for (int q = 0; q < 4; ++q)
     if (cfg->boundary[q] < 0 || cfg->boundary[q] > 5) return efail(SWE_ERR_BC, "bad boundary code");
+  if (cfg->fixed_dt) return efail(SWE_ERR_ARG, "fixed_dt applies to host-driven steps; the ensemble picks its dt");
   EnsEntry* host = (EnsEntry*)malloc(sizeof(EnsEntry) * (size_t)n);
   if (!host) return efail(SWE_ERR_ARG, "host allocation failed");
   memset(host, 0, sizeof(EnsEntry) * (size_t)n);

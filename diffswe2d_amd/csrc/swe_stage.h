@@ -57,6 +57,7 @@ template <class T> struct StageArgs {
   T* dt_out;             // stage 2, optional: the step's dt, written whether or not it is usable
   int record_wave;       // stage 1: accumulate wave speeds
   int fast;              // FP32: FastRound arithmetic (see swe_math.h); forward runs only
+  int fixed_dt;          // host-driven stage 2: dt = max_dt (the dt-frozen map), not min(CFL dt, max_dt)
 };
 
 // dt of the step from the accumulated maxima (solver.py:618-632):

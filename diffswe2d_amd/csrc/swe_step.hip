@@ -59,7 +59,7 @@ __global__ __launch_bounds__(256) void stage_kernel(StageArgs<T> a) {
       dt = step_dt<T>(a.acc, k, (T)cap);
     }
   } else if (STAGE == 2) {
-    dt = step_dt<T>(a.acc, k, (T)a.max_dt);
+    dt = a.fixed_dt ? (T)a.max_dt : step_dt<T>(a.acc, k, (T)a.max_dt);
   }
   if (STAGE == 2) {
     if (blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x < 2 * kSlots) a.acc_next[threadIdx.x] = 0;
@@ -323,6 +323,7 @@ int check_cfg(const swe_config* c) {
   if (c->friction_model < 0 || c->friction_model > 1) return fail(SWE_ERR_ARG, "unknown friction model");
   if (c->arithmetic != SWE_ARITH_REFERENCE && c->arithmetic != SWE_ARITH_FAST)
     return fail(SWE_ERR_ARG, "unknown arithmetic mode");
+  if (c->fixed_dt != 0 && c->fixed_dt != 1) return fail(SWE_ERR_ARG, "fixed_dt must be 0 or 1");
   return 0;
 }
 
@@ -372,6 +373,7 @@ StageArgs<T> make_args(const swe_grid* g, const swe_config* c, const T* bed, con
   a.friction = c->apply_friction;
   a.friction_model = c->friction_model;
   a.fast = c->arithmetic == SWE_ARITH_FAST;
+  a.fixed_dt = c->fixed_dt;
   return a;
 }
 
@@ -505,6 +507,7 @@ int advance_impl(const swe_grid* g, const swe_config* c, const T* bed, const T* 
     return fail(SWE_ERR_ARG, "null pointer argument");
   if (tape_k1 && (!tape_k2 || first_step < tape_first)) return fail(SWE_ERR_ARG, "bad tape arguments");
   if (first_step < 0 || n_steps < 0) return fail(SWE_ERR_ARG, "negative step index");
+  if (c->fixed_dt) return fail(SWE_ERR_ARG, "fixed_dt applies to host-driven steps; a device-driven run picks its dt");
   if (dt_series && first_step + n_steps > dt_capacity) return fail(SWE_ERR_CAPACITY, "dt series capacity exceeded");
   if (n_gauges > 0 && (!gauges || !gauge_depth || first_step + n_steps > gauge_capacity))
     return fail(SWE_ERR_CAPACITY, "gauge series capacity exceeded");

@@ -28,7 +28,7 @@
 extern "C" {
 #endif
 
-#define SWE_ABI_VERSION 4
+#define SWE_ABI_VERSION 5
 
 enum {
   SWE_OK = 0,
@@ -81,6 +81,14 @@ typedef struct swe_config {
                                      results agree with the reference to a tolerance, not
                                      bit for bit.  Ignored by FP64, by steps that record
                                      k1 / k2 or a tape, by chunked runs and by the adjoint. */
+  int32_t fixed_dt;               /* 1: host-driven steps (swe_step_*, swe_step_stage_f32,
+                                     swe_step_forced_*, swe_step_record_*, swe_step_noalloc)
+                                     take max_dt itself as dt instead of min(CFL dt, max_dt):
+                                     the dt-frozen map whose derivative the reference's
+                                     detached dt defines (src/solver.py:625-626), for finite-
+                                     difference checks of gradients (the reference has no
+                                     such switch).  Device-driven runs reject it.  0 (the
+                                     reference's behaviour) otherwise. */
 } swe_config;
 
 /* swe_config.arithmetic */
