@@ -536,6 +536,8 @@ def warmstart_leg(dev, n_scenarios, epochs):
                         "ensemble_gcell_per_s": round(rep["ensemble"]["gcell_per_s"], 2),
                         "ensemble_arithmetic": rep["ensemble"].get("arithmetic"),
                         "training_seconds": round(rep["surrogate"]["seconds"], 2)},
+            "bar": "the plain run's best gauge misfit after its epochs (fewer epochs, a looser bar); the warm run "
+                   "starts from the surrogate and stops at that bar",
             "ratio_warm_over_plain": None if t_warm is None else round(t_warm / plain["seconds"], 4),
             "ratio_with_one_off": None if t_warm is None else round((t_warm + one_off) / plain["seconds"], 4)}
 
