@@ -3,7 +3,7 @@
 The grid is the benchmark generator's 512 x 512 basin (``scenarios.tsunami_basin
 (n=512)``: fractal bathymetry, ~15 % land on the east coast, a 2 m hump of
 sigma 20 km, Manning n = 0.025, transmissive sides; the benchmark itself runs
-8192 x 8192, where the oracle takes 19 s per step).  Over 2000 steps (255 s of
+8192 x 8192, where the oracle takes 19 s per step).  Over 2000 steps (222 s of
 simulated time) the wave crosses the whole 51.2 km basin and leaves through its
 open sides.  The fixture ``basin512_2000.npz`` holds three oracle runs
 (tests/golden/make_basin_fixture.py): FP32 with correctly rounded square roots
