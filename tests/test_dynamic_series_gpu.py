@@ -10,7 +10,8 @@ step's start time, src/solver.py:391-415) evaluated on the device.
   gives the per-step loop's gradients with respect to U0, the series' water
   levels and a trainable bed.
 * A backward through the device run's hmax raises instead of dropping that part
-  of the gradient.
+  of the gradient; through the per-step loop (``differentiable_hmax``) the hmax
+  gradient equals the oracle's autograd.
 """
 import pytest
 import torch
@@ -104,3 +105,34 @@ def test_device_run_hmax_gradient_raises(cuda_device):
     out, hmax = m(Ug, t_end=0.2, boundary_level=0.02)
     (g,) = torch.autograd.grad(hmax.sum(), Ug)
     assert float(g.abs().sum()) > 0
+
+
+def test_per_step_hmax_gradient_matches_oracle_autograd(cuda_device):
+    """d(sum hmax)/dU0 through the per-step loop (``differentiable_hmax``) against the oracle's
+    autograd of the same run (src/solver.py:765: hmax = maximum(hmax, h) at every step), FP64.
+
+    The oracle takes correctly rounded square roots, so both runs meet the same ties
+    (dry cells tie at every step; torch splits a tie's gradient 0.5 / 0.5 on both sides)."""
+    import oracle.swe_oracle as O
+
+    m, U0, sc = model(cuda_device, torch.float64)
+    m.differentiable_hmax = True
+    Ug = U0.clone().requires_grad_(True)
+    _, hmax = m(Ug, t_end=0.2, boundary_level=0.02)
+    (g,) = torch.autograd.grad(hmax.sum(), Ug)
+    old = O.SQRT_MODE
+    O.SQRT_MODE = "ieee"
+    try:
+        Uc = U0.detach().cpu().clone().requires_grad_(True)
+        bed = m.bed.detach().cpu()
+        rough = m.roughness_length.detach().cpu()
+        _, hc = O.forward(Uc, bed, rough, m.cfg, 0.08, 0.08, 0.2, boundary_level=0.02)
+        (gc,) = torch.autograd.grad(hc.sum(), Uc)
+    finally:
+        O.SQRT_MODE = old
+    e_h = relerr(hmax.detach(), hc.detach())
+    e_g = relerr(g, gc)
+    print(f"hmax vs oracle {e_h:.2e}; d(sum hmax)/dU0 vs oracle autograd {e_g:.2e}", flush=True)
+    assert e_h <= 1e-13
+    assert e_g <= 1e-10
+    assert float(gc.abs().sum()) > 0
