@@ -564,7 +564,12 @@ def main():
     else:
         from diffswe2d_amd import autotune
 
-        best, rep = autotune.tune_rows(model, U, candidates=(16, 32, 40, 48, 56, 64, 128), reps=10, save=False)
+        # tune on the state the timed steps start from: the initial state, whose momenta are
+        # exactly zero, steps ~6 % faster than the states a few steps on
+        _, _, U_tune = timed_run(model, U, 0, max(a.warmup, 1), fast=False)
+        best, rep = autotune.tune_rows(model, U_tune, candidates=(16, 32, 40, 48, 56, 64, 128), reps=10,
+                                       save=False)
+        del U_tune
         tuned = {str(k): round(v, 4) for k, v in rep.items()}
         print(f"# autotuned rows per strip: {best} ({tuned})", file=sys.stderr, flush=True)
     n = a.n

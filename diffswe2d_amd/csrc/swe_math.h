@@ -396,12 +396,20 @@ SWE_HD FaceFlux<T> face_flux(const Side<T>& L, T hL_cell, T zL_cell, const Side<
 
 // ---------------------------------------------------------------- positivity and friction
 // Positivity limiter and dry-cell momentum removal (solver.py:547-570).
+// theta = min(oh / max(-dh, eps), 1) where dh < 0.  When oh >= max(-dh, eps)
+// the quotient is at least 1 and rounds to at least 1 (rounding is monotonic),
+// so theta is 1 without the division; only a cell losing more than its depth
+// divides.  Most wavefronts then skip the division altogether.  (The fast
+// arithmetic's division can fall an ulp short of 1 there; it too gets 1.)
 template <class T, class A = RefRound>
 SWE_HD void clean_and_limit(T oh, T ou, T ov, T& ch, T& cu, T& cv, const SweConsts<T>& k, bool positivity) {
   if (positivity) {
     T dh = ch - oh, du = cu - ou, dv = cv - ov;
     T theta = T(1);
-    if (dh < T(0)) theta = clamp_hi(adiv<A>(oh, clamp_lo(-dh, k.eps)), T(1));
+    if (dh < T(0)) {
+      const T d = clamp_lo(-dh, k.eps);
+      if (!(oh >= d)) theta = clamp_hi(adiv<A>(oh, d), T(1));
+    }
     ch = oh + theta * dh;
     cu = ou + theta * du;
     cv = ov + theta * dv;
