@@ -1,0 +1,189 @@
+"""The adjoint on the full 8192 x 8192 tsunami basin, against the oracle's autograd.
+
+The benchmark's adjoint figures are taken on this grid; these tests check the
+gradient there.  The basin (fractal bathymetry, ~15 % land on the east coast,
+2 m Gaussian hump, Manning n = 0.025, transmissive sides) is advanced 2.5 s
+(about 20 steps) on the GPU; from that state a differentiable forward of 0.3 s
+(three CFL steps, the last one cut to the duration) runs with the loss
+sum(w * U) over 64 x 64 windows with seeded weights w: one on the hump's flank,
+one across the wet/dry coastline, one in the domain's corner (two transmissive
+sides).  The gradient with respect to the starting state:
+
+* is exactly zero beyond 4 cells per step from the windows (the step's stencil
+  reaches 2 cells per stage), checked over the whole grid;
+* inside that reach equals, window by window, the oracle's autograd gradient
+  (torch autograd through the reference's op sequence, with every step's dt
+  fixed to the GPU run's, as the reference detaches dt) on a crop of the
+  starting state around the window: the crop reaches 8 cells per step plus 8
+  beyond the window, where its own edges no longer matter;
+* is the same, bit for bit, from the taped backward and from the checkpointed
+  one (checkpoints every 2 steps, the segment replayed before its sweep).
+
+Water the hump has not reached is at rest up to rounding residue (momenta of
+~1e-11 m^2/s in FP64, ~5e-3 in FP32), and the limiter's branches there follow
+the last bits of the state: the gradient in such water is defined only by
+those bits (on a crop of the coastline at rest, a random relative change of
+1e-15 in the FP64 state moves the oracle's own gradient by half its norm), so
+a comparison there needs forward states that agree bit for bit.  FP32 with the
+oracle's correctly rounded square roots and friction powers (the GPU kernels'
+rounding) gives exactly that: all three windows, state and gradient bars 1e-6
+and 1e-4, the north star's bar for the adjoint against the reference's
+autograd (measured: states equal, gradients 1.1e-7).  The FP64 steps agree to
+~1e-15 but not bit for bit, with or without friction, so FP64 is compared
+where the hump's flow, not the residue, sets the branches: the flank window,
+bars 1e-13 and 1e-12 (measured 7.5e-16 and 2.2e-16).  (At the coastline at
+rest the FP64 gradients differ by 7e-4, inside the residue's spread above.)
+
+Reference: src/solver.py:572-663 (step, dt detached at :625-626),
+src/boundary.py:227-238 (transmissive padding), src/friction.py:102-131.
+"""
+import numpy as np
+import pytest
+import torch
+
+import diffswe2d_amd as swe
+from diffswe2d_amd import scenarios
+from oracle import swe_oracle as O
+
+pytestmark = pytest.mark.gpu
+
+CFG = dict(boundary_left="transmissive", boundary_right="transmissive", boundary_top="transmissive",
+           boundary_bottom="transmissive")
+N = 8192
+WIN = 64
+WARM_SECONDS = 2.5
+DURATION = 0.3
+
+
+@pytest.fixture(scope="module")
+def basin():
+    sc = scenarios.tsunami_basin(n=N)
+    bed = sc["bed"][0, 0]
+    h0 = sc["U"][0, 0]
+    wet = h0 > 0
+    eta = np.where(wet, h0.astype(np.float64) + bed, -np.inf)
+    cy, cx = np.unravel_index(int(np.argmax(eta)), eta.shape)
+    land_cols = np.nonzero(bed[cy] > 0)[0]
+    coast = int(land_cols[0])
+    windows = {
+        "hump flank": (cy - WIN // 2, cx + 150),
+        "coastline": (cy - WIN // 2, coast - 40),
+        "corner": (0, 0),
+    }
+    for r0, c0 in windows.values():
+        assert 0 <= r0 and r0 + WIN <= N and 0 <= c0 and c0 + WIN <= N
+    return sc, windows
+
+
+def run_gpu(sc, windows, dtype, device, friction=True, checkpointed=False):
+    cfg = swe.SWEConfig(**CFG, apply_friction=friction)
+    m = swe.SWE2D(torch.tensor(sc["bed"], dtype=dtype), torch.tensor(sc["rough"], dtype=dtype), sc["dx"], sc["dy"],
+                  config=cfg, train_dem=False).to(device)
+    if checkpointed:
+        m.adjoint_tape = False
+        m.adjoint_checkpoint_every = 2
+    U0 = torch.tensor(sc["U"], dtype=dtype, device=device)
+    with torch.no_grad():
+        Us, _ = m(U0, t_end=WARM_SECONDS)
+        Us = Us.clone()
+        del U0
+        Uref, _ = m(Us, t_end=DURATION)
+        dts = [float(v) for v in m.last_dt_series]
+        Uref = Uref.clone()
+    gen = torch.Generator(device="cpu").manual_seed(2024)
+    weights = {k: torch.randn(3, WIN, WIN, generator=gen, dtype=torch.float64) for k in windows}
+    U_in = Us.clone().requires_grad_(True)
+    Uo, _ = m(U_in, t_end=DURATION)
+    assert torch.equal(Uo.detach(), Uref)
+    loss = Uo.new_zeros(())
+    for k, (r0, c0) in windows.items():
+        loss = loss + (weights[k].to(device=device, dtype=dtype) * Uo[0, :, r0:r0 + WIN, c0:c0 + WIN]).sum()
+    loss.backward()
+    return Us.detach(), Uref, U_in.grad.detach(), dts, weights
+
+
+def oracle_window(sc, Us, grad, Uref, dts, r0, c0, w, dtype, friction):
+    """The oracle's autograd on a crop around one window; returns the compared errors."""
+    n = len(dts)
+    reach, margin = 4 * n, 8 * n + 8
+    R0, R1 = max(r0 - margin, 0), min(r0 + WIN + margin, N)
+    C0, C1 = max(c0 - margin, 0), min(c0 + WIN + margin, N)
+    cfg = swe.SWEConfig(**CFG, apply_friction=friction)
+    bed = torch.tensor(sc["bed"][:, :, R0:R1, C0:C1], dtype=dtype)
+    rough = torch.tensor(sc["rough"][:, :, R0:R1, C0:C1], dtype=dtype)
+    U = Us[:, :, R0:R1, C0:C1].cpu().to(dtype).clone().requires_grad_(True)
+    X = U
+    for dt in dts:
+        X, _ = O.step(X, bed, rough, cfg, sc["dx"], sc["dy"], 1e30, fixed_dt=dt)
+    wr, wc = r0 - R0, c0 - C0
+    (w.to(dtype) * X[0, :, wr:wr + WIN, wc:wc + WIN]).sum().backward()
+    g_or = U.grad.detach()
+    # the forward state where the crop's own edges have not reached
+    s = reach + 8
+    a0, a1 = max(r0 - s, 0), min(r0 + WIN + s, N)
+    b0, b1 = max(c0 - s, 0), min(c0 + WIN + s, N)
+    st_gpu = Uref[:, :, a0:a1, b0:b1].cpu().double()
+    st_or = X.detach()[:, :, a0 - R0:a1 - R0, b0 - C0:b1 - C0].double()
+    e_state = float(torch.linalg.norm(st_gpu - st_or) / torch.linalg.norm(st_or))
+    # the gradient over the window's reach (and the oracle's zero beyond it)
+    g0, g1 = max(r0 - reach, 0), min(r0 + WIN + reach, N)
+    h0, h1 = max(c0 - reach, 0), min(c0 + WIN + reach, N)
+    g_gpu = grad[:, :, g0:g1, h0:h1].cpu().double()
+    g_o = g_or[:, :, g0 - R0:g1 - R0, h0 - C0:h1 - C0].double()
+    e_grad = float(torch.linalg.norm(g_gpu - g_o) / torch.linalg.norm(g_o))
+    outside = g_or.clone()
+    outside[:, :, g0 - R0:g1 - R0, h0 - C0:h1 - C0] = 0
+    return e_state, e_grad, float(outside.abs().max()), float(torch.linalg.norm(g_o))
+
+
+def zero_beyond_reach(grad, windows, n_steps):
+    reach = 4 * n_steps
+    mask = torch.ones(grad.shape[-2:], dtype=torch.bool, device=grad.device)
+    for r0, c0 in windows.values():
+        mask[max(r0 - reach, 0):r0 + WIN + reach, max(c0 - reach, 0):c0 + WIN + reach] = False
+    return float(grad[0][:, mask].abs().max())
+
+
+CASES = {
+    # id: (dtype, friction, windows compared, state bar, gradient bar, oracle square root / power rounding)
+    "fp32": (torch.float32, True, ("hump flank", "coastline", "corner"), 1e-6, 1e-4, "ieee"),
+    "fp64": (torch.float64, True, ("hump flank",), 1e-13, 1e-12, None),
+}
+
+
+@pytest.mark.parametrize("case", list(CASES))
+def test_basin_fullsize_adjoint_matches_oracle_autograd(basin, case, cuda_device):
+    sc, all_windows = basin
+    dtype, friction, names, state_bar, grad_bar, sqrt_mode = CASES[case]
+    windows = {k: all_windows[k] for k in names}
+    Us, Uref, grad, dts, weights = run_gpu(sc, windows, dtype, cuda_device, friction=friction)
+    assert 2 <= len(dts) <= 4, dts
+    assert bool(torch.isfinite(grad).all())
+    beyond = zero_beyond_reach(grad, windows, len(dts))
+    print(f"basin {N}^2 {case}: {len(dts)} steps {dts}; |gradient| beyond the windows' reach {beyond:.1e}",
+          flush=True)
+    assert beyond == 0.0
+    old = O.SQRT_MODE
+    if sqrt_mode is not None:
+        O.SQRT_MODE = sqrt_mode
+    try:
+        for name, (r0, c0) in windows.items():
+            e_state, e_grad, out_or, norm = oracle_window(sc, Us, grad, Uref, dts, r0, c0, weights[name], dtype,
+                                                          friction)
+            print(f"  {name} window at ({r0}, {c0}): state {e_state:.2e}, gradient {e_grad:.2e} (norm {norm:.3e}); "
+                  f"oracle gradient beyond the reach {out_or:.1e}", flush=True)
+            assert out_or == 0.0
+            assert e_state <= state_bar
+            assert e_grad <= grad_bar
+    finally:
+        O.SQRT_MODE = old
+
+
+def test_basin_fullsize_checkpointed_gradient_equals_taped(basin, cuda_device):
+    sc, windows = basin
+    _, _, g_tape, dts, _ = run_gpu(sc, windows, torch.float32, cuda_device)
+    _, _, g_ckpt, dts2, _ = run_gpu(sc, windows, torch.float32, cuda_device, checkpointed=True)
+    assert dts == dts2
+    print(f"basin {N}^2 FP32: taped and checkpointed (every 2 steps) gradients over {len(dts)} steps "
+          f"{'equal' if torch.equal(g_tape, g_ckpt) else 'differ'}", flush=True)
+    assert torch.equal(g_tape, g_ckpt)
