@@ -28,8 +28,8 @@ a comparison there needs forward states that agree bit for bit.  The oracle
 therefore runs with correctly rounded square roots (and, in FP32, friction
 powers), the GPU kernels' rounding; with torch's CPU square root, one ulp off
 for ~1 % of arguments, the FP64 coastline gradients differ by 7e-4.  Bars: FP32
-state 1e-6 and gradient 1e-4, the north star's bar for the adjoint against the
-reference's autograd; FP64 1e-13 and 1e-12.  Measured: the forward states equal
+state 1e-6 and gradient 1e-4 (the adjoint's bar against the reference's
+autograd); FP64 1e-13 and 1e-12.  Measured: the forward states equal
 bit for bit in every window and both precisions; gradients 1.1e-7 (FP32) and
 2.0-2.1e-16 (FP64).
 
