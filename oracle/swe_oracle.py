@@ -1,6 +1,6 @@
 """CPU oracle for the DiffSWE2d per-time-step finite-volume update.
 
-TEST INFRASTRUCTURE ONLY.  This module is the checker: only ``tests/``, the
+TEST INFRASTRUCTURE ONLY.  Results are checked against it: only ``tests/``, the
 package's smoke check and the ``cpu_baseline`` leg of ``bench.py`` may import
 it.  The product path (``diffswe2d_amd``) never calls it and fails
 loudly when its HIP library is missing.
