@@ -168,26 +168,27 @@ __device__ inline void stage_strip(const StageArgs<T>& a, int strip, int seg, in
   // sweep are skipped with the values they would produce (+0 fluxes, exactly).
   // dry1..dry3: rows r-1..r-3 of the iteration; wave-uniform.
   bool dry1 = false, dry2 = false, dry3 = false;
+  // stage 2: the step-start state of rows r-1 and r-2 as loaded (an interior
+  // output cell's loads carry no ghost rule), for the output row's update
+  T u1h = T(0), u1u = T(0), u1v = T(0), u2h = T(0), u2u = T(0), u2v = T(0);
   for (int it = 0; it < rows + 4; ++it) {
     const int r = y0 - kHalo + it;
     // row r's loads go out first; the x sweep of row o = r - 2 does not need them
     CellLoads<T> raw = fetch_loads<T, STAGE>(a, b, r, col, strip_in, &q, voff);
     // stage 2: the output row's step-start state and roughness, needed at its update
     T oh = T(0), ou = T(0), ov = T(0), coef = T(0);
-    if (STAGE == 2 && it >= 4 && lane_out) {
-      if (strip_in) {
-        const unsigned so = (unsigned)(r - 2) * q.row_bytes;
-        oh = buf_load<T>(q.U, voff, so);
-        ou = buf_load<T>(q.U, voff, so + q.plane_bytes);
-        ov = buf_load<T>(q.U, voff, so + 2 * q.plane_bytes);
-        if (a.friction) coef = f.rough[(int64_t)b * f.rough_bstride + (int64_t)(r - 2) * f.nx + col];
-      } else {
-        const int64_t cell = (int64_t)(r - 2) * f.nx + col;
-        oh = a.U[mbase + cell];
-        ou = a.U[mbase + cell + f.plane];
-        ov = a.U[mbase + cell + 2 * f.plane];
-        if (a.friction) coef = f.rough[(int64_t)b * f.rough_bstride + cell];
-      }
+    if (STAGE == 2) {
+      oh = u2h;
+      ou = u2u;
+      ov = u2v;
+      u2h = u1h;
+      u2u = u1u;
+      u2v = u1v;
+      u1h = raw.h;
+      u1u = raw.hu;
+      u1v = raw.hv;
+      if (it >= 4 && lane_out && a.friction)
+        coef = f.rough[(int64_t)b * f.rough_bstride + (int64_t)(r - 2) * f.nx + col];
     }
     T rx_h = T(0), rx_u = T(0), rx_v = T(0);
     if (it >= 4 && !dry2) {
