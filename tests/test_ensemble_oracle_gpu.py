@@ -12,6 +12,11 @@ Examples/inverse/.../bc_inversion_train.py:332-410).
   start a bore: step count equal to the oracle's, final state rel-L2 <= 1e-5 with
   identical wet/dry masks, gauge samples (h + z) within 1e-5 of the largest
   depth.
+* The full coastal run-up (544 x 344, 450 chunks of 0.05 s, the committed
+  oracle run ``runup_full_ieee_f32.npz``, correctly rounded square roots and
+  friction powers as the kernels round) as one member of a launch with a
+  272 x 172 scenario: the same 10,934 steps, the final state and every gauge
+  sample (h + z at the six gauges, at each chunk end) bit for bit.
 * 256 scenarios cycling through the three grids, 20 chunks: every scenario
   finishes, states are finite with h >= 0 and zero momentum on dry cells, each
   scenario's last gauge sample is h + z of its final state, the finer grids
@@ -88,6 +93,34 @@ def test_ensemble_three_grids_match_oracle(cuda_device):
         assert e <= 1e-5
         assert ge <= 1e-5
         assert mism == 0
+
+
+def test_ensemble_full_runup_matches_oracle_record(golden, cuda_device):
+    d = golden("runup_full_ieee_f32")
+    ru = scenarios.coastal_runup()
+    gauges = [tuple(int(v) for v in g) for g in d["gauges"]]
+    assert gauges == list(ru["gauges"])
+    runup = dict(bed=ru["bed"][0, 0], rough=ru["rough"][0, 0], dx=ru["dx"], dy=ru["dy"], U0=ru["U"][0],
+                 levels=np.asarray(d["levels"]), chunk=ru["sample_dt"], gauges=gauges)
+    small, _ = scenarios.ensemble_scenarios(n=1, seed=5, n_chunks=450)
+    assert small[0]["bed"].shape == (172, 272)
+    ens = Ensemble(cfg(), [small[0], runup], cuda_device, graph=True, steps_per_graph=64)
+    global_steps = ens.run()
+    r = ens.results()[1]
+    bed = np.asarray(ru["bed"][0, 0], dtype=np.float32)
+    ref_obs = np.array([[np.float32(h) + bed[iy, ix] for h, (iy, ix) in zip(row, gauges)] for row in d["gauge"]],
+                       dtype=np.float32)
+    obs = r["gauges"].cpu().numpy()
+    U = r["U"].cpu().numpy()
+    same_U = bool(np.array_equal(U, d["U"][0]))
+    same_obs = bool(np.array_equal(obs, ref_obs))
+    print(f"run-up 544x344 in a two-scenario launch: {r['steps']} steps (oracle {len(d['dt'])}), {global_steps} "
+          f"global steps; final state {'equal' if same_U else 'differs'} (rel-L2 {relerr(U, d['U'][0]):.2e}), "
+          f"gauge samples {'equal' if same_obs else 'differ'} (max {float(np.abs(obs - ref_obs).max()):.2e})",
+          flush=True)
+    assert r["done"] and r["steps"] == len(d["dt"])
+    assert same_U
+    assert same_obs
 
 
 def test_ensemble_256_scenarios(cuda_device):
