@@ -166,13 +166,13 @@ template <class T> __device__ inline PredLoads<T> predictor_loads(const AdjArgs<
   return p;
 }
 
-// one thread per interior cell: lam_k2 = adjoint of k2, acc = adjoint reaching U
+// one thread per interior cell (blockIdx.y: the member): lam_k2 = adjoint of k2,
+// acc = adjoint reaching U
 template <class T>
-__global__ __launch_bounds__(256) void adj_corrector_kernel(AdjArgs<T> a, T* lam_k2, int64_t n) {
-  int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  if (t >= n) return;
-  int b = (int)(t / a.f.plane);
-  int64_t cell = t - (int64_t)b * a.f.plane;
+__global__ __launch_bounds__(256) void adj_corrector_kernel(AdjArgs<T> a, T* lam_k2) {
+  const int64_t cell = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (cell >= a.f.plane) return;
+  const int b = blockIdx.y;
   int64_t i = (int64_t)b * 3 * a.f.plane + cell, P = a.f.plane;
   Lam3<T> lam, acc;
   corrector_bwd(a, b, cell, i, lam, acc);
@@ -700,8 +700,7 @@ int step_adjoint_impl(const swe_grid* g, const swe_config* c, const T* U, const 
   dim3 lg((unsigned)((waves + kWavesPerBlock - 1) / kWavesPerBlock), (unsigned)g->batch);
   dim3 lb(kWave * kWavesPerBlock);
 
-  int64_t n = (int64_t)g->batch * a.f.plane;
-  dim3 eb(256), eg((unsigned)((n + 255) / 256));
+  dim3 eb(256), eg((unsigned)((a.f.plane + 255) / 256), (unsigned)g->batch);
   // stage 2: corrector + friction reversed, then L^T at the predictor
   AdjArgs<T> a2 = a;
   a2.k2 = k2;
@@ -710,7 +709,7 @@ int step_adjoint_impl(const swe_grid* g, const swe_config* c, const T* U, const 
   a2.friction = c->apply_friction;
   a2.friction_model = c->friction_model;
   a2.acc = w.lam_acc;
-  hipLaunchKernelGGL((adj_corrector_kernel<T>), eg, eb, 0, s, a2, w.lam_k, n);
+  hipLaunchKernelGGL((adj_corrector_kernel<T>), eg, eb, 0, s, a2, w.lam_k);
   a2.lam = w.lam_k;
   a2.base = nullptr;
   a2.out = w.lam_P;
