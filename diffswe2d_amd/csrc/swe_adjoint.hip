@@ -2,17 +2,19 @@
 //
 // The step U -> U' (solver.py:572-663) with dt held fixed (the reference
 // detaches dt, :625-626) is reversed by
-//   adj_corrector  friction and corrector clean-up reversed  -> lam_k2, lam_acc
-//   adj_L<2>       L^T at the predictor P = clean(U, U + dt/2 k1) -> lam_P (+ ghosts)
+//   adj_L<2>       friction and the corrector clean-up reversed on each row as
+//                  the sweep fetches it (-> lam_k2, and lam_acc on the
+//                  wavefront's own cells), then L^T at the predictor
+//                  P = clean(U, U + dt/2 k1)                 -> lam_P (+ ghosts)
 //   adj_ghost      ghost-cell adjoints folded back onto the edge cells (four
 //                  per-side launches) and the boundary-level reduction
 //   adj_L<1>       the predictor clean-up reversed on each row as the sweep
 //                  fetches it (-> lam_k1 and its direct adjoint of U), then
 //                  L^T at U                                  -> lam_U = lam_acc + ...
 //   adj_ghost      as above
-// The corrector reverse stays a separate launch: folded into the L^T sweep it
-// would be recomputed on every halo cell at the sweep's low occupancy, which
-// measured slower than one streaming pass.
+// Folding the corrector reverse into stage-2 L^T recomputes it on the halo
+// cells (~24 % more of it) but saves a streaming pass over U, k2, the output
+// adjoint and two stored adjoints: -2 % per reverse step on the 8192^2 basin.
 // L^T mirrors the forward stage's decomposition: one wavefront streams down a
 // strip of 64 columns, now with a 3-cell halo (58 outputs): a cell's adjoint
 // depends on the four faces within two cells of it and those faces on the
@@ -44,7 +46,8 @@ template <class T> struct AdjArgs {
   const T* U;            // state at step start
   const T* k1;           // first-stage tendency (stage 2 rebuilds the predictor from it)
   T half_dt;
-  const T* lam;          // adjoint of the stage tendency [B,3,ny,nx] (unused when lam_P is set)
+  const T* lam;          // adjoint of the stage tendency [B,3,ny,nx] (unused when lam_P is set, and by
+                         // stage 2, which forms it from lam_out, k2 and U)
   const T* base;         // added to the interior output (nullable; required when lam_P is set)
   T* out;                // adjoint of the stage input state [B,3,ny,nx]
   T* ghost;              // [B][4 sides][2 layers][lmax][3]
@@ -57,7 +60,7 @@ template <class T> struct AdjArgs {
                          // clean-up's direct adjoint of U added to its output)
   T dt;
   int friction, friction_model;
-  T* acc;                // adjoint reaching U directly past the corrector (written by the corrector)
+  T* acc;                // adjoint reaching U directly past the corrector (written by stage-2 L^T)
   T* part;               // per-block level-gradient partials of the ghost fold
   int part_blocks;       // blocks per side in the ghost fold
   T* bedgrad;            // adjoint of the bed, [B, ny, nx], accumulated (nullable)
@@ -112,21 +115,41 @@ __device__ inline int64_t ghost_slot(int b, int r, int c, int ny, int nx, int lm
   return ((((int64_t)b * 4 + side) * 2 + layer) * lmax + i) * kGhostVals;
 }
 
-// friction and the corrector clean-up reversed at interior cell i (solver.py:646-661):
-// lam = adjoint of k2, acc = adjoint reaching U directly
+// loads of the corrector reverse at padded cell (r, c): k2, the step-output adjoint
+// and the roughness, issued ahead of their use
+template <class T> struct CorrLoads {
+  T kh, ku, kv, gh, gu, gv, coef;
+  bool in;  // interior cell
+};
+template <class T> __device__ inline CorrLoads<T> corrector_loads(const AdjArgs<T>& a, int b, int r, int c) {
+  CorrLoads<T> q{T(0), T(0), T(0), T(0), T(0), T(0), T(0), r >= 0 && r < a.f.ny && c >= 0 && c < a.f.nx};
+  if (q.in) {
+    const int64_t P = a.f.plane, cell = (int64_t)r * a.f.nx + c, i = (int64_t)b * 3 * P + cell;
+    q.kh = a.k2[i];
+    q.ku = a.k2[i + P];
+    q.kv = a.k2[i + 2 * P];
+    q.gh = a.lam_out[i];
+    q.gu = a.lam_out[i + P];
+    q.gv = a.lam_out[i + 2 * P];
+    if (a.friction) q.coef = a.f.rough[b * a.f.rough_bstride + cell];
+  }
+  return q;
+}
+
+// friction and the corrector clean-up reversed at one interior cell (solver.py:646-661)
+// from its step-start state o* and the loads q: lam = adjoint of k2, acc = adjoint
+// reaching U directly
 template <class T>
-__device__ inline void corrector_bwd(const AdjArgs<T>& a, int b, int64_t cell, int64_t i, Lam3<T>& lam, Lam3<T>& acc) {
+__device__ inline void corrector_core(const AdjArgs<T>& a, T oh, T ou, T ov, const CorrLoads<T>& q, Lam3<T>& lam,
+                                      Lam3<T>& acc) {
   const SweConsts<T>& k = a.k;
-  const int64_t P = a.f.plane;
   const T dt = a.dt;
-  T oh = a.U[i], ou = a.U[i + P], ov = a.U[i + 2 * P];
-  T ch = oh + dt * a.k2[i], cu = ou + dt * a.k2[i + P], cv = ov + dt * a.k2[i + 2 * P];
-  T gh = a.lam_out[i], gu = a.lam_out[i + P], gv = a.lam_out[i + 2 * P];
+  T ch = oh + dt * q.kh, cu = ou + dt * q.ku, cv = ov + dt * q.kv;
+  T gh = q.gh, gu = q.gu, gv = q.gv;
   if (a.friction) {
     T vh = ch, vu = cu, vv = cv;
     clean_and_limit(oh, ou, ov, vh, vu, vv, k, a.positivity != 0);
-    T coef = a.f.rough[b * a.f.rough_bstride + cell];
-    friction_bwd(vh, vu, vv, coef, dt, k, a.friction_model, gh, gu, gv, gh, gu, gv);
+    friction_bwd(vh, vu, vv, q.coef, dt, k, a.friction_model, gh, gu, gv, gh, gu, gv);
   }
   T go_h = 0, go_u = 0, go_v = 0, gc_h, gc_u, gc_v;
   clean_and_limit_bwd(oh, ou, ov, ch, cu, cv, k, a.positivity != 0, gh, gu, gv, go_h, go_u, go_v, gc_h, gc_u, gc_v);
@@ -164,24 +187,6 @@ template <class T> __device__ inline PredLoads<T> predictor_loads(const AdjArgs<
     p.gv = a.lam_P[i + 2 * P];
   }
   return p;
-}
-
-// one thread per interior cell (blockIdx.y: the member): lam_k2 = adjoint of k2,
-// acc = adjoint reaching U
-template <class T>
-__global__ __launch_bounds__(256) void adj_corrector_kernel(AdjArgs<T> a, T* lam_k2) {
-  const int64_t cell = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  if (cell >= a.f.plane) return;
-  const int b = blockIdx.y;
-  int64_t i = (int64_t)b * 3 * a.f.plane + cell, P = a.f.plane;
-  Lam3<T> lam, acc;
-  corrector_bwd(a, b, cell, i, lam, acc);
-  a.acc[i] = acc.h;
-  a.acc[i + P] = acc.u;
-  a.acc[i + 2 * P] = acc.v;
-  lam_k2[i] = lam.h;
-  lam_k2[i + P] = lam.u;
-  lam_k2[i + 2 * P] = lam.v;
 }
 
 // x sweep of one row (all lanes take part): adjoint of the primitives of each
@@ -268,6 +273,10 @@ void adj_L_kernel(AdjArgs<T> a) {
     const bool fused_pred = STAGE == 1 && a.lam_P != nullptr;
     PredLoads<T> pl{};
     if (fused_pred) pl = predictor_loads(a, b, n, col);
+    // stage 2 forms lam of each row itself: friction and the corrector clean-up
+    // reversed from the step-output adjoint, k2 and U (on interior cells the
+    // stage-2 fetch loaded the cell's own step-start state)
+    const bool fused_corr = STAGE == 2 && a.lam_out != nullptr;
     PrimGrad<T> gx{};
     if (it >= 6 && n - 3 >= 0 && n - 3 < f.ny && !dry3) gx = x_sweep_bwd(a, P3, L3);  // x-faces: interior rows
     Prim<T> Pn = fetch_finish<T, STAGE, AdjArgs<T>>(a, b, raw, half);
@@ -282,6 +291,20 @@ void adj_L_kernel(AdjArgs<T> a) {
         ring[slot][0][lane] = pacc.h;
         ring[slot][1][lane] = pacc.u;
         ring[slot][2][lane] = pacc.v;
+      }
+    } else if (fused_corr) {
+      const CorrLoads<T> cl = corrector_loads(a, b, n, col);
+      if (cl.in) {
+        Lam3<T> cacc;
+        corrector_core(a, raw.h, raw.hu, raw.hv, cl, Ln, cacc);
+        // the corrector's direct adjoint of U, once per cell: on the wavefront's own
+        // rows and columns
+        if (lane_out && n >= y0 && n < y0 + rows) {
+          const int64_t i = (int64_t)b * 3 * f.plane + (int64_t)n * f.nx + col;
+          a.acc[i] = cacc.h;
+          a.acc[i + f.plane] = cacc.u;
+          a.acc[i + 2 * f.plane] = cacc.v;
+        }
       }
     } else {
       Ln = load_lam(a, b, n, col);
@@ -564,7 +587,7 @@ constexpr int kGhostThreads = 256;
 constexpr int kGhostOneBlock = 2048;  // longest side folded by one block per member
 
 template <class T> struct AdjWs {
-  T *lam_acc, *lam_P, *lam_k, *ghost, *part;  // lam_k holds lam_k2
+  T *lam_acc, *lam_P, *ghost, *part;
   T* bedgrad;                                 // [B, ny, nx], accumulated over calls
   int lmax, part_blocks;
 };
@@ -574,14 +597,14 @@ inline int ghost_blocks(const swe_grid* g) {
   return (lmax + kGhostThreads - 1) / kGhostThreads;
 }
 
-// [lam_acc | lam_P | lam_k | ghost adjoints | level partials | bed adjoint],
+// [lam_acc | lam_P | ghost adjoints | level partials | bed adjoint],
 // each 256-byte aligned
 template <class T> size_t adj_bed_offset(const swe_grid* g) {
   size_t state = (size_t)g->batch * 3 * (size_t)g->ny * g->nx * sizeof(T);
   int lmax = g->ny > g->nx ? g->ny : g->nx;
   size_t ghost = (size_t)g->batch * 4 * 2 * lmax * kGhostVals * sizeof(T);
   size_t part = (size_t)g->batch * 4 * ghost_blocks(g) * sizeof(T);
-  return 3 * al256(state) + al256(ghost) + al256(part);
+  return 2 * al256(state) + al256(ghost) + al256(part);
 }
 
 template <class T> size_t adj_ws_bytes(const swe_grid* g) {
@@ -597,9 +620,8 @@ template <class T> AdjWs<T> adj_ws(void* ws, const swe_grid* g) {
   w.bedgrad = (T*)(p + adj_bed_offset<T>(g));
   w.lam_acc = (T*)p;
   w.lam_P = (T*)(p + state);
-  w.lam_k = (T*)(p + 2 * state);
-  w.ghost = (T*)(p + 3 * state);
-  w.part = (T*)(p + 3 * state + ghost);
+  w.ghost = (T*)(p + 2 * state);
+  w.part = (T*)(p + 2 * state + ghost);
   w.lmax = lmax;
   w.part_blocks = ghost_blocks(g);
   return w;
@@ -700,7 +722,6 @@ int step_adjoint_impl(const swe_grid* g, const swe_config* c, const T* U, const 
   dim3 lg((unsigned)((waves + kWavesPerBlock - 1) / kWavesPerBlock), (unsigned)g->batch);
   dim3 lb(kWave * kWavesPerBlock);
 
-  dim3 eb(256), eg((unsigned)((a.f.plane + 255) / 256), (unsigned)g->batch);
   // stage 2: corrector + friction reversed, then L^T at the predictor
   AdjArgs<T> a2 = a;
   a2.k2 = k2;
@@ -709,8 +730,7 @@ int step_adjoint_impl(const swe_grid* g, const swe_config* c, const T* U, const 
   a2.friction = c->apply_friction;
   a2.friction_model = c->friction_model;
   a2.acc = w.lam_acc;
-  hipLaunchKernelGGL((adj_corrector_kernel<T>), eg, eb, 0, s, a2, w.lam_k);
-  a2.lam = w.lam_k;
+  a2.lam = nullptr;  // stage-2 L^T forms it from lam_out, k2 and U
   a2.base = nullptr;
   a2.out = w.lam_P;
   launch_adj_L<T, 2>(a2, lg, lb, s);
