@@ -12,9 +12,13 @@
 //                  fetches it (-> lam_k1 and its direct adjoint of U), then
 //                  L^T at U                                  -> lam_U = lam_acc + ...
 //   adj_ghost      as above
-// Folding the corrector reverse into stage-2 L^T recomputes it on the halo
-// cells (~24 % more of it) but saves a streaming pass over U, k2, the output
-// adjoint and two stored adjoints: -2 % per reverse step on the 8192^2 basin.
+// On grids that fill the GPU many times over, folding the corrector reverse
+// into stage-2 L^T (recomputed on the halo cells, ~24 % more of it) saves a
+// streaming pass over U, k2, the output adjoint and two stored adjoints: -2 %
+// per reverse step on the 8192^2 basin.  Where a launch holds few wavefronts
+// the sweep's time is its rows' dependent chain, which the fold lengthens
+// (+6.5 % per epoch of the 68 x 42 Monai inversion, +3 % for the 544 x 344
+// run-up), and adj_corrector, an elementwise launch, reverses it first.
 // L^T mirrors the forward stage's decomposition: one wavefront streams down a
 // strip of 64 columns, now with a 3-cell halo (58 outputs): a cell's adjoint
 // depends on the four faces within two cells of it and those faces on the
@@ -37,6 +41,10 @@ namespace swe {
 
 constexpr int kHaloA = 3;
 constexpr int kOutA = kWave - 2 * kHaloA;  // 58
+// L^T wavefronts per launch (times members) from which stage-2 L^T reverses the
+// corrector itself: 8 per SIMD of the 1,024 (above that the launch is bound by
+// issue, not by one wavefront's chain)
+constexpr long long kFoldCorrectorWaves = 8192;
 
 template <class T> struct AdjArgs {
   SweConsts<T> k;
@@ -47,7 +55,7 @@ template <class T> struct AdjArgs {
   const T* k1;           // first-stage tendency (stage 2 rebuilds the predictor from it)
   T half_dt;
   const T* lam;          // adjoint of the stage tendency [B,3,ny,nx] (unused when lam_P is set, and by
-                         // stage 2, which forms it from lam_out, k2 and U)
+                         // stage 2 when it forms it from lam_out, k2 and U)
   const T* base;         // added to the interior output (nullable; required when lam_P is set)
   T* out;                // adjoint of the stage input state [B,3,ny,nx]
   T* ghost;              // [B][4 sides][2 layers][lmax][3]
@@ -60,7 +68,7 @@ template <class T> struct AdjArgs {
                          // clean-up's direct adjoint of U added to its output)
   T dt;
   int friction, friction_model;
-  T* acc;                // adjoint reaching U directly past the corrector (written by stage-2 L^T)
+  T* acc;                // adjoint reaching U directly past the corrector (written by its reverse)
   T* part;               // per-block level-gradient partials of the ghost fold
   int part_blocks;       // blocks per side in the ghost fold
   T* bedgrad;            // adjoint of the bed, [B, ny, nx], accumulated (nullable)
@@ -121,19 +129,18 @@ template <class T> struct CorrLoads {
   T kh, ku, kv, gh, gu, gv, coef;
   bool in;  // interior cell
 };
+// at interior cell `cell` of member b (state index i)
+template <class T>
+__device__ inline CorrLoads<T> corrector_loads_at(const AdjArgs<T>& a, int b, int64_t cell, int64_t i) {
+  const int64_t P = a.f.plane;
+  return CorrLoads<T>{a.k2[i], a.k2[i + P], a.k2[i + 2 * P], a.lam_out[i], a.lam_out[i + P], a.lam_out[i + 2 * P],
+                      a.friction ? a.f.rough[b * a.f.rough_bstride + cell] : T(0), true};
+}
 template <class T> __device__ inline CorrLoads<T> corrector_loads(const AdjArgs<T>& a, int b, int r, int c) {
-  CorrLoads<T> q{T(0), T(0), T(0), T(0), T(0), T(0), T(0), r >= 0 && r < a.f.ny && c >= 0 && c < a.f.nx};
-  if (q.in) {
-    const int64_t P = a.f.plane, cell = (int64_t)r * a.f.nx + c, i = (int64_t)b * 3 * P + cell;
-    q.kh = a.k2[i];
-    q.ku = a.k2[i + P];
-    q.kv = a.k2[i + 2 * P];
-    q.gh = a.lam_out[i];
-    q.gu = a.lam_out[i + P];
-    q.gv = a.lam_out[i + 2 * P];
-    if (a.friction) q.coef = a.f.rough[b * a.f.rough_bstride + cell];
-  }
-  return q;
+  if (!(r >= 0 && r < a.f.ny && c >= 0 && c < a.f.nx))
+    return CorrLoads<T>{T(0), T(0), T(0), T(0), T(0), T(0), T(0), false};
+  const int64_t cell = (int64_t)r * a.f.nx + c;
+  return corrector_loads_at(a, b, cell, (int64_t)b * 3 * a.f.plane + cell);
 }
 
 // friction and the corrector clean-up reversed at one interior cell (solver.py:646-661)
@@ -187,6 +194,25 @@ template <class T> __device__ inline PredLoads<T> predictor_loads(const AdjArgs<
     p.gv = a.lam_P[i + 2 * P];
   }
   return p;
+}
+
+// one thread per interior cell (blockIdx.y: the member): lam_k2 = adjoint of k2,
+// acc = adjoint reaching U
+template <class T>
+__global__ __launch_bounds__(256) void adj_corrector_kernel(AdjArgs<T> a, T* lam_k2) {
+  const int64_t cell = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (cell >= a.f.plane) return;
+  const int b = blockIdx.y;
+  const int64_t i = (int64_t)b * 3 * a.f.plane + cell, P = a.f.plane;
+  const CorrLoads<T> q = corrector_loads_at(a, b, cell, i);
+  Lam3<T> lam, acc;
+  corrector_core(a, a.U[i], a.U[i + P], a.U[i + 2 * P], q, lam, acc);
+  a.acc[i] = acc.h;
+  a.acc[i + P] = acc.u;
+  a.acc[i + 2 * P] = acc.v;
+  lam_k2[i] = lam.h;
+  lam_k2[i + P] = lam.u;
+  lam_k2[i + 2 * P] = lam.v;
 }
 
 // x sweep of one row (all lanes take part): adjoint of the primitives of each
@@ -273,9 +299,9 @@ void adj_L_kernel(AdjArgs<T> a) {
     const bool fused_pred = STAGE == 1 && a.lam_P != nullptr;
     PredLoads<T> pl{};
     if (fused_pred) pl = predictor_loads(a, b, n, col);
-    // stage 2 forms lam of each row itself: friction and the corrector clean-up
-    // reversed from the step-output adjoint, k2 and U (on interior cells the
-    // stage-2 fetch loaded the cell's own step-start state)
+    // stage 2 with lam_out set forms lam of each row itself: friction and the
+    // corrector clean-up reversed from the step-output adjoint, k2 and U (on
+    // interior cells the stage-2 fetch loaded the cell's own step-start state)
     const bool fused_corr = STAGE == 2 && a.lam_out != nullptr;
     PrimGrad<T> gx{};
     if (it >= 6 && n - 3 >= 0 && n - 3 < f.ny && !dry3) gx = x_sweep_bwd(a, P3, L3);  // x-faces: interior rows
@@ -587,7 +613,7 @@ constexpr int kGhostThreads = 256;
 constexpr int kGhostOneBlock = 2048;  // longest side folded by one block per member
 
 template <class T> struct AdjWs {
-  T *lam_acc, *lam_P, *ghost, *part;
+  T *lam_acc, *lam_P, *lam_k, *ghost, *part;  // lam_k: lam_k2 when the corrector runs alone
   T* bedgrad;                                 // [B, ny, nx], accumulated over calls
   int lmax, part_blocks;
 };
@@ -597,14 +623,14 @@ inline int ghost_blocks(const swe_grid* g) {
   return (lmax + kGhostThreads - 1) / kGhostThreads;
 }
 
-// [lam_acc | lam_P | ghost adjoints | level partials | bed adjoint],
+// [lam_acc | lam_P | lam_k | ghost adjoints | level partials | bed adjoint],
 // each 256-byte aligned
 template <class T> size_t adj_bed_offset(const swe_grid* g) {
   size_t state = (size_t)g->batch * 3 * (size_t)g->ny * g->nx * sizeof(T);
   int lmax = g->ny > g->nx ? g->ny : g->nx;
   size_t ghost = (size_t)g->batch * 4 * 2 * lmax * kGhostVals * sizeof(T);
   size_t part = (size_t)g->batch * 4 * ghost_blocks(g) * sizeof(T);
-  return 2 * al256(state) + al256(ghost) + al256(part);
+  return 3 * al256(state) + al256(ghost) + al256(part);
 }
 
 template <class T> size_t adj_ws_bytes(const swe_grid* g) {
@@ -620,8 +646,9 @@ template <class T> AdjWs<T> adj_ws(void* ws, const swe_grid* g) {
   w.bedgrad = (T*)(p + adj_bed_offset<T>(g));
   w.lam_acc = (T*)p;
   w.lam_P = (T*)(p + state);
-  w.ghost = (T*)(p + 2 * state);
-  w.part = (T*)(p + 2 * state + ghost);
+  w.lam_k = (T*)(p + 2 * state);
+  w.ghost = (T*)(p + 3 * state);
+  w.part = (T*)(p + 3 * state + ghost);
   w.lmax = lmax;
   w.part_blocks = ghost_blocks(g);
   return w;
@@ -730,7 +757,14 @@ int step_adjoint_impl(const swe_grid* g, const swe_config* c, const T* U, const 
   a2.friction = c->apply_friction;
   a2.friction_model = c->friction_model;
   a2.acc = w.lam_acc;
-  a2.lam = nullptr;  // stage-2 L^T forms it from lam_out, k2 and U
+  if (waves * g->batch >= kFoldCorrectorWaves) {
+    a2.lam = nullptr;  // stage-2 L^T forms it from lam_out, k2 and U
+  } else {
+    dim3 eb(256), eg((unsigned)((a.f.plane + 255) / 256), (unsigned)g->batch);
+    hipLaunchKernelGGL((adj_corrector_kernel<T>), eg, eb, 0, s, a2, w.lam_k);
+    a2.lam = w.lam_k;
+    a2.lam_out = nullptr;  // stage-2 L^T reads lam instead
+  }
   a2.base = nullptr;
   a2.out = w.lam_P;
   launch_adj_L<T, 2>(a2, lg, lb, s);
